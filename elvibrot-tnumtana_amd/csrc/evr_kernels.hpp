@@ -1,0 +1,58 @@
+// Device-side data layout shared by the plan code and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace evr {
+
+constexpr int kMaxDev = 12;   // modes supported by the term kernel (5-bit digits in a u64)
+constexpr int kMaxN = 20;     // largest nb or nq of a 1D slot
+constexpr int kKeoFactors = 7;
+
+// Per Smolyak term, as the kernel needs it.
+struct TermDev {
+  int64_t off_nb;             // start of the term in the concatenated basis (map, R)
+  int64_t off_nq;             // start of the term in the concatenated grid (V)
+  int nbt, nqt;               // basis / grid size of the term
+  int m;                      // modes 0..m-1 are contracted inside an LDS slab
+  int nbA, QA, nbB, QB;       // inner/outer basis and grid sizes
+  int lds_mats;               // doubles of 1D matrices staged in LDS
+  double w;                   // Smolyak weight (SGT2:754-854)
+  int slot[kMaxDev];          // 1D slot of each mode
+};
+
+// Per 1D slot (mode, level): offsets into the matrix pools.
+struct SlotDev {
+  int nb, nq;
+  int64_t offB;    // B: nq x nb  (pool "bw")
+  int64_t offW;    // W: nb x nq  (pool "bw")
+  int64_t offK;    // 7 factors of nb x nb (pool "keo")
+  int64_t offX;    // nq grid points (pool "x")
+};
+
+enum TermMode { kModeHpsi = 0, kModeB2G = 1, kModeG2B = 2 };
+
+template <typename T>
+struct TermParams {
+  int D;
+  int mode;
+  int hasV, hasKeo, hasA, hasF1, diagT2;
+  int nvec;                   // vectors in this call (blockIdx decode)
+  int64_t nunits;             // padded count of terms in this launch (multiple of 8)
+  const int *order;           // term ids for this launch (-1 = padding)
+  const TermDev *terms;
+  const SlotDev *slots;
+  const T *bw;                // pool of B and W matrices
+  const T *keo;               // pool of basis-space KEO factors
+  const int *map;             // term-local -> packed
+  const T *V;                 // grid potential (term-concatenated)
+  const double *coefC;        // D x D: c[i*D+j]
+  const double *coefA;        // D^3: a[(i*D+j)*D+k]
+  const double *coefF1;       // D
+  const T *X;                 // packed input, [nb_packed][nvec]
+  T *R;                       // term results, [sum nb_t][nvec]
+  T *G;                       // grid output (B2G), [sum nq_t]
+  const T *Gin;               // grid input (G2B), [sum nq_t]
+};
+
+}  // namespace evr

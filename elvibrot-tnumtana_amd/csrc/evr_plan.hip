@@ -1,0 +1,700 @@
+// Plans and the C ABI of the H|psi> engine (see include/evr_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/evr_hip.h"
+#include "evr_kernels.hpp"
+#include "evr_setup.hpp"
+#include "evr_term.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(EVR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+template <typename X>
+struct DevBuf {
+  X *ptr = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count)
+  {
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void **)&ptr, sizeof(X) * count);
+  }
+  hipError_t upload(const X *h, size_t count, hipStream_t s)
+  {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    e = hipMemcpyAsync(ptr, h, sizeof(X) * count, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+  void release()
+  {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+};
+
+// work class of the term kernel: threads per workgroup x elements per thread
+struct KClass {
+  int nt, ept;
+};
+constexpr KClass kClasses[] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {512, 8}, {512, 16}};
+constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
+constexpr size_t kLdsLimit = 160 * 1024 - 1024;
+
+}  // namespace
+
+struct evr_desc_owner {
+  evr_basis_desc desc;
+  evr::BasisTables tab;
+};
+
+struct evr_plan {
+  int D = 0, type = 0, max_nvec = 0;
+  bool dp_order = false;
+  int64_t nterms = 0, nb_packed = 0, sum_nb = 0, sum_nq = 0, max_nb = 0, max_nq = 0;
+  hipStream_t stream = nullptr;
+  bool hasV = false, hasKeo = false, hasA = false, hasF1 = false, diagT2 = false;
+  // host copies
+  std::vector<evr::TermDev> terms;
+  std::vector<evr::SlotDev> slots;
+  std::vector<int> dp_perm;  // DP: reference grid index of each term-grid point
+  // device tables
+  DevBuf<evr::TermDev> d_terms;
+  DevBuf<evr::SlotDev> d_slots;
+  DevBuf<double> d_bw64, d_keo64, d_x;
+  DevBuf<float> d_bw32, d_keo32, d_V32;
+  DevBuf<int> d_map, d_csr_ptr, d_csr_idx;
+  DevBuf<double> d_V;
+  DevBuf<double> d_coefC, d_coefA, d_coefF1;
+  DevBuf<int> d_order[kNumClasses];
+  int64_t nunits[kNumClasses] = {};
+  size_t lds_bytes64[kNumClasses] = {}, lds_bytes32[kNumClasses] = {};
+  // work buffers
+  DevBuf<double> d_X, d_R, d_psi, d_hpsi, d_G;
+  bool V32_valid = false;
+};
+
+namespace {
+
+template <typename T, int NT, int EPT>
+void set_lds(size_t bytes)
+{
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_term<T, NT, EPT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <typename T>
+hipError_t launch_class(int c, const evr::TermParams<T> &prm, int64_t units, int nvec,
+                        size_t lds, hipStream_t s)
+{
+  const int64_t nblk = units * nvec;
+  if (nblk <= 0) return hipSuccess;
+  dim3 grid((unsigned)nblk);
+  switch (c) {
+    case 0: set_lds<T, 256, 1>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 1>), grid, dim3(256), lds, s, prm); break;
+    case 1: set_lds<T, 256, 2>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 2>), grid, dim3(256), lds, s, prm); break;
+    case 2: set_lds<T, 256, 4>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 4>), grid, dim3(256), lds, s, prm); break;
+    case 3: set_lds<T, 256, 8>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 8>), grid, dim3(256), lds, s, prm); break;
+    case 4: set_lds<T, 512, 8>(lds); hipLaunchKernelGGL((evr::k_term<T, 512, 8>), grid, dim3(512), lds, s, prm); break;
+    case 5: set_lds<T, 512, 16>(lds); hipLaunchKernelGGL((evr::k_term<T, 512, 16>), grid, dim3(512), lds, s, prm); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+evr::TermParams<T> base_params(evr_plan *p, int mode, int nvec)
+{
+  evr::TermParams<T> prm{};
+  prm.D = p->D;
+  prm.mode = mode;
+  prm.hasV = p->hasV;
+  prm.hasKeo = p->hasKeo;
+  prm.hasA = p->hasA;
+  prm.hasF1 = p->hasF1;
+  prm.diagT2 = p->diagT2;
+  prm.nvec = nvec;
+  prm.terms = p->d_terms.ptr;
+  prm.slots = p->d_slots.ptr;
+  prm.map = p->d_map.ptr;
+  prm.coefC = p->d_coefC.ptr;
+  prm.coefA = p->d_coefA.ptr;
+  prm.coefF1 = p->d_coefF1.ptr;
+  return prm;
+}
+
+// Apply every term class for `nvec` vectors held in X ([nb_packed][nvec]).
+template <typename T>
+int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *Gin, const T *bw,
+              const T *keo, const T *V)
+{
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (p->nunits[c] == 0) continue;
+    evr::TermParams<T> prm = base_params<T>(p, mode, nvec);
+    prm.nunits = p->nunits[c];
+    prm.order = p->d_order[c].ptr;
+    prm.bw = bw;
+    prm.keo = keo;
+    prm.V = V;
+    prm.X = X;
+    prm.R = R;
+    prm.G = G;
+    prm.Gin = Gin;
+    size_t lds = sizeof(T) == 8 ? p->lds_bytes64[c] : p->lds_bytes32[c];
+    hipError_t e = launch_class<T>(c, prm, p->nunits[c], nvec, lds, p->stream);
+    if (e != hipSuccess) return fail(EVR_ERR_HIP, std::string("term kernel: ") + hipGetErrorString(e));
+  }
+  return EVR_OK;
+}
+
+unsigned blocks_for(int64_t n)
+{
+  int64_t b = (n + 255) / 256;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+const char *evr_last_error(void) { return g_err.c_str(); }
+
+static int wrap_desc(std::unique_ptr<evr_desc_owner> &o, evr_basis_desc **out)
+{
+  evr::BasisTables &t = o->tab;
+  evr_basis_desc &d = o->desc;
+  d.D = t.D;
+  d.nlev = t.nlev;
+  d.nterms = t.nterms;
+  d.term_levels = t.term_levels.data();
+  d.weights = t.weights.data();
+  d.nb = t.nb.data();
+  d.nq = t.nq.data();
+  d.x = t.x.data();
+  d.B = t.B.data();
+  d.W = t.W.data();
+  d.D1 = t.D1.data();
+  d.D2 = t.D2.data();
+  d.nb_packed = t.nb_packed;
+  d.map = t.map.data();
+  d.grid_last_fastest = t.dp_order ? 1 : 0;
+  *out = &o.release()->desc;  // desc is the first member of evr_desc_owner
+  return EVR_OK;
+}
+
+int evr_sg4_hermite_desc(int D, int LG, int LB, int nbA, int nbB, int nqA, int nqB,
+                         evr_basis_desc **out)
+{
+  if (!out) return fail(EVR_ERR_ARG, "out is NULL");
+  std::unique_ptr<evr_desc_owner> o(new evr_desc_owner());
+  std::string err = evr::build_sg4_hermite(D, LG, LB, nbA, nbB, nqA, nqB, o->tab);
+  if (!err.empty()) return fail(EVR_ERR_ARG, err);
+  return wrap_desc(o, out);
+}
+
+int evr_dp_hermite_desc(int D, const int *nb, const int *nq, evr_basis_desc **out)
+{
+  if (!out || !nb || !nq) return fail(EVR_ERR_ARG, "NULL argument");
+  std::unique_ptr<evr_desc_owner> o(new evr_desc_owner());
+  std::string err = evr::build_dp_hermite(D, nb, nq, o->tab);
+  if (!err.empty()) return fail(EVR_ERR_ARG, err);
+  return wrap_desc(o, out);
+}
+
+void evr_basis_desc_free(evr_basis_desc *d)
+{
+  if (d) delete reinterpret_cast<evr_desc_owner *>(d);
+}
+
+static void term_sizes(const evr_basis_desc *d, int64_t t, int64_t &nb, int64_t &nq)
+{
+  nb = nq = 1;
+  for (int k = 0; k < d->D; ++k) {
+    const int s = k * d->nlev + d->term_levels[t * d->D + k];
+    nb *= d->nb[s];
+    nq *= d->nq[s];
+  }
+}
+
+int evr_basis_desc_sizes(const evr_basis_desc *d, int64_t *out)
+{
+  if (!d || !out) return fail(EVR_ERR_ARG, "NULL argument");
+  int64_t snb = 0, snq = 0, mnb = 0, mnq = 0;
+  for (int64_t t = 0; t < d->nterms; ++t) {
+    int64_t nb, nq;
+    term_sizes(d, t, nb, nq);
+    snb += nb;
+    snq += nq;
+    mnb = std::max(mnb, nb);
+    mnq = std::max(mnq, nq);
+  }
+  out[0] = snb; out[1] = snq; out[2] = mnb; out[3] = mnq;
+  return EVR_OK;
+}
+
+// offsets of the per-slot arrays inside the descriptor pools
+static void slot_offsets(const evr_basis_desc *d, std::vector<int64_t> &ox, std::vector<int64_t> &oB,
+                         std::vector<int64_t> &oD)
+{
+  const int ns = d->D * d->nlev;
+  ox.assign(ns + 1, 0);
+  oB.assign(ns + 1, 0);
+  oD.assign(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    ox[s + 1] = ox[s] + d->nq[s];
+    oB[s + 1] = oB[s] + (int64_t)d->nq[s] * d->nb[s];
+    oD[s + 1] = oD[s] + (int64_t)d->nq[s] * d->nq[s];
+  }
+}
+
+int evr_basis_desc_grid(const evr_basis_desc *d, double *q)
+{
+  if (!d || !q) return fail(EVR_ERR_ARG, "NULL argument");
+  std::vector<int64_t> ox, oB, oD;
+  slot_offsets(d, ox, oB, oD);
+  if (d->grid_last_fastest) {
+    int64_t nq = 1;
+    for (int k = 0; k < d->D; ++k) nq *= d->nq[k * d->nlev];
+    for (int64_t i = 0; i < nq; ++i) {
+      int64_t r = i;
+      for (int k = d->D - 1; k >= 0; --k) {
+        const int s = k * d->nlev;
+        q[i * d->D + k] = d->x[ox[s] + r % d->nq[s]];
+        r /= d->nq[s];
+      }
+    }
+    return EVR_OK;
+  }
+  int64_t off = 0;
+  for (int64_t t = 0; t < d->nterms; ++t) {
+    int64_t nb, nq;
+    term_sizes(d, t, nb, nq);
+    for (int64_t i = 0; i < nq; ++i) {
+      int64_t r = i;
+      for (int k = 0; k < d->D; ++k) {
+        const int s = k * d->nlev + d->term_levels[t * d->D + k];
+        q[(off + i) * d->D + k] = d->x[ox[s] + r % d->nq[s]];
+        r /= d->nq[s];
+      }
+    }
+    off += nq;
+  }
+  return EVR_OK;
+}
+
+int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nvec,
+                    evr_plan **out)
+{
+  if (!bd || !op || !out) return fail(EVR_ERR_ARG, "NULL argument");
+  if (bd->D < 1 || bd->D > evr::kMaxDev)
+    return fail(EVR_ERR_ARG, "D must be in 1..12 for the term kernel");
+  if (op->type != 0 && op->type != 1 && op->type != 10)
+    return fail(EVR_ERR_ARG, "operator type must be 0, 1 or 10");
+  if ((op->type == 10 && !op->G0) || (op->type == 1 && !op->F2))
+    return fail(EVR_ERR_ARG, "type 10 needs G0, type 1 needs F2");
+  if (max_nvec < 1) return fail(EVR_ERR_ARG, "max_nvec must be >= 1");
+  std::unique_ptr<evr_plan> p(new evr_plan());
+  const int D = bd->D, nlev = bd->nlev;
+  p->D = D;
+  p->type = op->type;
+  p->max_nvec = max_nvec;
+  p->dp_order = bd->grid_last_fastest != 0;
+  p->nterms = bd->nterms;
+  p->nb_packed = bd->nb_packed;
+  HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+
+  // ---- slots: 1D matrices and basis-space KEO factors
+  const int ns = D * nlev;
+  std::vector<int64_t> ox, oB, oD;
+  slot_offsets(bd, ox, oB, oD);
+  std::vector<double> bw, keo, xs;
+  p->slots.resize(ns);
+  for (int s = 0; s < ns; ++s) {
+    const int nb = bd->nb[s], nq = bd->nq[s];
+    if (nb < 1 || nq < nb || nq > evr::kMaxN)
+      return fail(EVR_ERR_ARG, "each slot needs 1 <= nb <= nq <= 20");
+    evr::SlotDev &sd = p->slots[s];
+    sd.nb = nb;
+    sd.nq = nq;
+    sd.offB = (int64_t)bw.size();
+    bw.insert(bw.end(), bd->B + oB[s], bd->B + oB[s + 1]);
+    sd.offW = (int64_t)bw.size();
+    bw.insert(bw.end(), bd->W + oB[s], bd->W + oB[s + 1]);
+    sd.offX = (int64_t)xs.size();
+    xs.insert(xs.end(), bd->x + ox[s], bd->x + ox[s + 1]);
+    evr::Slot1D s1;
+    s1.nb = nb;
+    s1.nq = nq;
+    s1.x.assign(bd->x + ox[s], bd->x + ox[s + 1]);
+    s1.B.assign(bd->B + oB[s], bd->B + oB[s + 1]);
+    s1.W.assign(bd->W + oB[s], bd->W + oB[s + 1]);
+    s1.D1.assign(bd->D1 + oD[s], bd->D1 + oD[s + 1]);
+    s1.D2.assign(bd->D2 + oD[s], bd->D2 + oD[s + 1]);
+    std::vector<double> f;
+    evr::keo_factors(s1, f);
+    sd.offK = (int64_t)keo.size();
+    keo.insert(keo.end(), f.begin(), f.end());
+  }
+
+  // ---- operator coefficients in the kernel's (i, j, k) convention
+  p->hasV = op->V != nullptr;
+  p->hasKeo = op->type != 0;
+  p->diagT2 = op->type == 1;
+  std::vector<double> cC((size_t)D * D, 0.0), cA, cF1;
+  if (op->type == 10) {
+    // reference: phi_i = sum_j G_ji d_j psi; H = -1/2 sum_i d_i phi_i
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) cC[i * D + j] = -0.5 * op->G0[j + D * i];
+    if (op->Ga) {
+      p->hasA = true;
+      cA.assign((size_t)D * D * D, 0.0);
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j)
+          for (int k = 0; k < D; ++k) cA[(i * D + j) * D + k] = -0.5 * op->Ga[j + D * (i + D * k)];
+    }
+  } else if (op->type == 1) {
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) cC[i * D + j] = op->F2[i + D * j];
+    if (op->F1) {
+      p->hasF1 = true;
+      cF1.assign(op->F1, op->F1 + D);
+    }
+  }
+
+  // ---- terms
+  p->terms.resize(p->nterms);
+  int64_t onb = 0, onq = 0;
+  std::vector<int> cls(p->nterms);
+  std::vector<double> cost(p->nterms);
+  for (int64_t t = 0; t < p->nterms; ++t) {
+    evr::TermDev &td = p->terms[t];
+    int nbd[evr::kMaxDev], nqd[evr::kMaxDev];
+    int64_t nbt = 1, nqt = 1;
+    int mats = 0;
+    for (int d = 0; d < D; ++d) {
+      const int lv = bd->term_levels[t * D + d];
+      if (lv < 0 || lv >= nlev) return fail(EVR_ERR_ARG, "term level out of range");
+      const int s = d * nlev + lv;
+      td.slot[d] = s;
+      nbd[d] = bd->nb[s];
+      nqd[d] = bd->nq[s];
+      nbt *= nbd[d];
+      nqt *= nqd[d];
+      mats += 2 * nbd[d] * nqd[d] + (p->hasKeo ? evr::kKeoFactors * nbd[d] * nbd[d] : 0);
+    }
+    if (nqt > INT32_MAX) return fail(EVR_ERR_SIZE, "term grid exceeds int32");
+    td.off_nb = onb;
+    td.off_nq = onq;
+    td.nbt = (int)nbt;
+    td.nqt = (int)nqt;
+    td.w = bd->weights[t];
+    td.lds_mats = mats + (mats & 1);  // keep the data arrays 16-byte aligned
+    int c = 0;
+    while (c < kNumClasses && (int64_t)kClasses[c].nt * kClasses[c].ept < nbt) ++c;
+    if (c == kNumClasses) return fail(EVR_ERR_SIZE, "term basis larger than 8192 functions");
+    cls[t] = c;
+    // largest slab (modes 0..m-1) that fits the LDS budget (double words)
+    const size_t budget = kLdsLimit / 8;
+    const size_t base = td.lds_mats + nbt;
+    int best = -1;
+    int64_t nbA = 1, QA = 1;
+    for (int m = 1; m <= D; ++m) {
+      nbA *= nbd[m - 1];
+      QA *= nqd[m - 1];
+      const int64_t nbB = nbt / nbA;
+      const size_t work = std::max<size_t>(p->hasKeo ? nbt : 0, 2 * QA + 2 * nbB);
+      if (base + work <= budget) best = m; else break;
+    }
+    if (best < 1) return fail(EVR_ERR_SIZE, "term does not fit in LDS");
+    td.m = best;
+    td.nbA = 1;
+    td.QA = 1;
+    for (int d = 0; d < best; ++d) { td.nbA *= nbd[d]; td.QA *= nqd[d]; }
+    td.nbB = (int)(nbt / td.nbA);
+    td.QB = (int)(nqt / td.QA);
+    const size_t work = std::max<size_t>(p->hasKeo ? nbt : 0, 2 * (size_t)td.QA + 2 * td.nbB);
+    const size_t words = base + work;
+    p->lds_bytes64[c] = std::max(p->lds_bytes64[c], words * 8);
+    p->lds_bytes32[c] = std::max(p->lds_bytes32[c], words * 4);
+    cost[t] = (double)nqt * 50.0 + (double)nbt * D * D * 6.0;
+    onb += nbt;
+    onq += nqt;
+    p->max_nb = std::max(p->max_nb, nbt);
+    p->max_nq = std::max(p->max_nq, nqt);
+  }
+  p->sum_nb = onb;
+  p->sum_nq = onq;
+  if (onb > INT32_MAX) return fail(EVR_ERR_SIZE, "sum of term basis sizes exceeds int32");
+
+  // ---- per-class work lists, most expensive terms first
+  for (int c = 0; c < kNumClasses; ++c) {
+    std::vector<int> ids;
+    for (int64_t t = 0; t < p->nterms; ++t)
+      if (cls[t] == c) ids.push_back((int)t);
+    if (ids.empty()) continue;
+    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    while (ids.size() % 8) ids.push_back(-1);
+    p->nunits[c] = (int64_t)ids.size();
+    HIP_TRY(p->d_order[c].upload(ids.data(), ids.size(), p->stream));
+  }
+
+  // ---- deterministic scatter: CSR of term rows per packed function
+  std::vector<int> cnt(p->nb_packed + 1, 0), idx(onb);
+  for (int64_t r = 0; r < onb; ++r) {
+    const int mp = bd->map[r];
+    if (mp >= p->nb_packed) return fail(EVR_ERR_ARG, "map entry beyond nb_packed");
+    if (mp >= 0) cnt[mp + 1]++;
+  }
+  for (int64_t k = 0; k < p->nb_packed; ++k) cnt[k + 1] += cnt[k];
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t r = 0; r < onb; ++r) {
+      const int mp = bd->map[r];
+      if (mp >= 0) idx[fill[mp]++] = (int)r;
+    }
+  }
+
+  // ---- DP grid permutation (reference DP order <-> term grid order)
+  if (p->dp_order) {
+    p->dp_perm.resize(onq);
+    int n[evr::kMaxDev] = {0};
+    for (int64_t i = 0; i < onq; ++i) {
+      int64_t g = 0;
+      for (int d = 0; d < D; ++d) g = g * bd->nq[d * nlev] + n[d];
+      p->dp_perm[i] = (int)g;
+      for (int d = 0; d < D; ++d) {
+        if (++n[d] < bd->nq[d * nlev]) break;
+        n[d] = 0;
+      }
+    }
+  }
+
+  // ---- uploads
+  std::vector<float> bw32(bw.begin(), bw.end()), keo32(keo.begin(), keo.end());
+  HIP_TRY(p->d_terms.upload(p->terms.data(), p->terms.size(), p->stream));
+  HIP_TRY(p->d_slots.upload(p->slots.data(), p->slots.size(), p->stream));
+  HIP_TRY(p->d_bw64.upload(bw.data(), bw.size(), p->stream));
+  HIP_TRY(p->d_keo64.upload(keo.data(), keo.size(), p->stream));
+  HIP_TRY(p->d_bw32.upload(bw32.data(), bw32.size(), p->stream));
+  HIP_TRY(p->d_keo32.upload(keo32.data(), keo32.size(), p->stream));
+  HIP_TRY(p->d_x.upload(xs.data(), xs.size(), p->stream));
+  HIP_TRY(p->d_map.upload(bd->map, onb, p->stream));
+  HIP_TRY(p->d_csr_ptr.upload(cnt.data(), cnt.size(), p->stream));
+  HIP_TRY(p->d_csr_idx.upload(idx.data(), idx.size(), p->stream));
+  HIP_TRY(p->d_coefC.upload(cC.data(), cC.size(), p->stream));
+  if (!cA.empty()) HIP_TRY(p->d_coefA.upload(cA.data(), cA.size(), p->stream));
+  if (!cF1.empty()) HIP_TRY(p->d_coefF1.upload(cF1.data(), cF1.size(), p->stream));
+  HIP_TRY(p->d_V.alloc(onq));
+  HIP_TRY(p->d_X.alloc((size_t)p->nb_packed * max_nvec));
+  HIP_TRY(p->d_R.alloc((size_t)onb * max_nvec));
+  HIP_TRY(p->d_psi.alloc((size_t)p->nb_packed * max_nvec));
+  HIP_TRY(p->d_hpsi.alloc((size_t)p->nb_packed * max_nvec));
+  if (op->V) {
+    int rc = evr_plan_set_potential(p.get(), op->V);
+    if (rc) return rc;
+  } else {
+    HIP_TRY(hipMemsetAsync(p->d_V.ptr, 0, sizeof(double) * onq, p->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  *out = p.release();
+  return EVR_OK;
+}
+
+int evr_plan_destroy(evr_plan *p)
+{
+  if (!p) return EVR_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  p->d_terms.release(); p->d_slots.release(); p->d_bw64.release(); p->d_keo64.release();
+  p->d_x.release(); p->d_bw32.release(); p->d_keo32.release(); p->d_V32.release();
+  p->d_map.release(); p->d_csr_ptr.release(); p->d_csr_idx.release(); p->d_V.release();
+  p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release();
+  for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
+  p->d_X.release(); p->d_R.release(); p->d_psi.release(); p->d_hpsi.release(); p->d_G.release();
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return EVR_OK;
+}
+
+int evr_plan_set_potential(evr_plan *p, const double *V)
+{
+  if (!p || !V) return fail(EVR_ERR_ARG, "NULL argument");
+  std::vector<double> tmp;
+  const double *src = V;
+  if (p->dp_order) {
+    tmp.resize(p->sum_nq);
+    for (int64_t i = 0; i < p->sum_nq; ++i) tmp[i] = V[p->dp_perm[i]];
+    src = tmp.data();
+  }
+  HIP_TRY(hipMemcpyAsync(p->d_V.ptr, src, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice,
+                         p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->hasV = true;
+  p->V32_valid = false;
+  return EVR_OK;
+}
+
+int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
+                                const double *d22, const double *e4)
+{
+  if (!p) return fail(EVR_ERR_ARG, "NULL plan");
+  const int D = p->D;
+  std::vector<double> coef((size_t)D + D * D * D + D * D + D, 0.0);
+  if (v2) std::copy(v2, v2 + D, coef.begin());
+  if (c3) std::copy(c3, c3 + D * D * D, coef.begin() + D);
+  if (d22) std::copy(d22, d22 + D * D, coef.begin() + D + D * D * D);
+  if (e4) std::copy(e4, e4 + D, coef.begin() + D + D * D * D + D * D);
+  DevBuf<double> dcoef;
+  HIP_TRY(dcoef.upload(coef.data(), coef.size(), p->stream));
+  hipLaunchKernelGGL(evr::k_gen_poly, dim3((unsigned)p->nterms), dim3(256), 0, p->stream, D,
+                     p->d_terms.ptr, p->d_slots.ptr, p->d_x.ptr, dcoef.ptr, p->d_V.ptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  dcoef.release();
+  p->hasV = true;
+  p->V32_valid = false;
+  return EVR_OK;
+}
+
+int evr_plan_sizes(const evr_plan *p, int64_t *out)
+{
+  if (!p || !out) return fail(EVR_ERR_ARG, "NULL argument");
+  out[0] = p->nb_packed; out[1] = p->nterms; out[2] = p->sum_nb; out[3] = p->sum_nq;
+  out[4] = p->max_nb; out[5] = p->max_nq; out[6] = p->D;
+  return EVR_OK;
+}
+
+void *evr_plan_stream(evr_plan *p) { return p ? (void *)p->stream : nullptr; }
+
+static int ensure_v32(evr_plan *p)
+{
+  if (p->V32_valid) return EVR_OK;
+  if (!p->d_V32.ptr) HIP_TRY(p->d_V32.alloc(p->sum_nq));
+  hipLaunchKernelGGL(evr::k_to_float, dim3(blocks_for(p->sum_nq)), dim3(256), 0, p->stream,
+                     p->d_V.ptr, p->d_V32.ptr, p->sum_nq);
+  HIP_TRY(hipGetLastError());
+  p->V32_valid = true;
+  return EVR_OK;
+}
+
+int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
+{
+  if (!p || !psi || !hpsi) return fail(EVR_ERR_ARG, "NULL argument");
+  if (nvec < 1 || nvec > p->max_nvec) return fail(EVR_ERR_ARG, "nvec must be in 1..max_nvec");
+  const bool dev = (flags & EVR_DEVICE_PTRS) != 0;
+  const bool fp32 = (flags & EVR_FP32) != 0;
+  const int64_t n = p->nb_packed * nvec;
+  const double *dpsi = psi;
+  double *dh = hpsi;
+  if (!dev) {
+    HIP_TRY(hipMemcpyAsync(p->d_psi.ptr, psi, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+    dpsi = p->d_psi.ptr;
+    dh = p->d_hpsi.ptr;
+  }
+  int rc;
+  if (!fp32) {
+    double *X = p->d_X.ptr, *R = p->d_R.ptr;
+    hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                       p->nb_packed, nvec, X);
+    HIP_TRY(hipGetLastError());
+    rc = run_terms<double>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw64.ptr,
+                           p->d_keo64.ptr, p->d_V.ptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(evr::k_reduce<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, R,
+                       p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, dh);
+    HIP_TRY(hipGetLastError());
+  } else {
+    rc = ensure_v32(p);
+    if (rc) return rc;
+    float *X = reinterpret_cast<float *>(p->d_X.ptr), *R = reinterpret_cast<float *>(p->d_R.ptr);
+    hipLaunchKernelGGL(evr::k_pack_in<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                       p->nb_packed, nvec, X);
+    HIP_TRY(hipGetLastError());
+    rc = run_terms<float>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw32.ptr,
+                          p->d_keo32.ptr, p->d_V32.ptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(evr::k_reduce<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, R,
+                       p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, dh);
+    HIP_TRY(hipGetLastError());
+  }
+  if (!dev) {
+    HIP_TRY(hipMemcpyAsync(hpsi, dh, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+  }
+  return EVR_OK;
+}
+
+int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG)
+{
+  if (!p || !psiB || !psiG) return fail(EVR_ERR_ARG, "NULL argument");
+  if (!p->d_G.ptr) HIP_TRY(p->d_G.alloc(p->sum_nq));
+  HIP_TRY(hipMemcpyAsync(p->d_X.ptr, psiB, sizeof(double) * p->nb_packed, hipMemcpyHostToDevice,
+                         p->stream));
+  int rc = run_terms<double>(p, evr::kModeB2G, 1, p->d_X.ptr, nullptr, p->d_G.ptr, nullptr,
+                             p->d_bw64.ptr, p->d_keo64.ptr, p->d_V.ptr);
+  if (rc) return rc;
+  std::vector<double> tmp;
+  double *dst = psiG;
+  if (p->dp_order) {
+    tmp.resize(p->sum_nq);
+    dst = tmp.data();
+  }
+  HIP_TRY(hipMemcpyAsync(dst, p->d_G.ptr, sizeof(double) * p->sum_nq, hipMemcpyDeviceToHost,
+                         p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  if (p->dp_order)
+    for (int64_t i = 0; i < p->sum_nq; ++i) psiG[p->dp_perm[i]] = tmp[i];
+  return EVR_OK;
+}
+
+int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB)
+{
+  if (!p || !psiB || !psiG) return fail(EVR_ERR_ARG, "NULL argument");
+  if (!p->d_G.ptr) HIP_TRY(p->d_G.alloc(p->sum_nq));
+  std::vector<double> tmp;
+  const double *src = psiG;
+  if (p->dp_order) {
+    tmp.resize(p->sum_nq);
+    for (int64_t i = 0; i < p->sum_nq; ++i) tmp[i] = psiG[p->dp_perm[i]];
+    src = tmp.data();
+  }
+  HIP_TRY(hipMemcpyAsync(p->d_G.ptr, src, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice,
+                         p->stream));
+  int rc = run_terms<double>(p, evr::kModeG2B, 1, nullptr, p->d_R.ptr, nullptr, p->d_G.ptr,
+                             p->d_bw64.ptr, p->d_keo64.ptr, p->d_V.ptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(evr::k_reduce<double>, dim3(blocks_for(p->nb_packed)), dim3(256), 0, p->stream,
+                     p->d_R.ptr, p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, 1, p->d_hpsi.ptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(psiB, p->d_hpsi.ptr, sizeof(double) * p->nb_packed, hipMemcpyDeviceToHost,
+                         p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return EVR_OK;
+}
+
+}  // extern "C"

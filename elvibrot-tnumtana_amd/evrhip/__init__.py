@@ -1,0 +1,139 @@
+"""Host-side mirror of ElVibRot's operator interface over the HIP engine.
+
+Names follow the reference: a Plan's `TabOpPsi` is sub_TabOpPsi
+(Source_ElVibRot/sub_Operator/sub_OpPsi.f90:701), `BasisRep_TO_GridRep` /
+`GridRep_TO_BasisRep` are sub_PsiBasisRep_TO_GridRep /
+sub_PsiGridRep_TO_BasisRep (Source_ElVibRot/sub_WP/sub_module_psi_B_TO_G.f90:60,
+178).  Everything runs in libevrhip.so; there is no CPU fallback: when the
+library is missing, loading raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import hamiltonian  # noqa: F401
+from ._lib import BasisDesc, OpDesc, check, load
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+FP64 = 0
+FP32 = 1
+DEVICE_PTRS = 2
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Basis:
+    """An SG4 or direct-product Hermite basis description (owned by the library)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+        d = ptr.contents
+        self.D, self.nlev, self.nterms, self.nb_packed = d.D, d.nlev, d.nterms, d.nb_packed
+        z = (ctypes.c_int64 * 4)()
+        check(load().evr_basis_desc_sizes(ptr, z))
+        self.sum_nb, self.sum_nq, self.max_nb, self.max_nq = list(z)
+
+    @classmethod
+    def sg4(cls, D, LG, LB=None, nbA=1, nbB=2, nqA=2, nqB=2):
+        LB = LG if LB is None else LB
+        p = ctypes.POINTER(BasisDesc)()
+        check(load().evr_sg4_hermite_desc(D, LG, LB, nbA, nbB, nqA, nqB, ctypes.byref(p)))
+        return cls(p)
+
+    @classmethod
+    def dp(cls, nb, nq):
+        nb = np.asarray(nb, dtype=np.int32)
+        nq = np.asarray(nq, dtype=np.int32)
+        p = ctypes.POINTER(BasisDesc)()
+        check(load().evr_dp_hermite_desc(len(nb), nb.ctypes.data_as(_ip), nq.ctypes.data_as(_ip),
+                                         ctypes.byref(p)))
+        return cls(p)
+
+    def grid(self):
+        q = np.zeros((self.sum_nq, self.D))
+        check(load().evr_basis_desc_grid(self.ptr, _d(q)))
+        return q
+
+    def term_levels(self):
+        d = self.ptr.contents
+        return np.ctypeslib.as_array(d.term_levels, shape=(self.nterms, self.D)).copy()
+
+    def weights(self):
+        return np.ctypeslib.as_array(self.ptr.contents.weights, shape=(self.nterms,)).copy()
+
+    def map(self):
+        return np.ctypeslib.as_array(self.ptr.contents.map, shape=(self.sum_nb,)).copy()
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            load().evr_basis_desc_free(self.ptr)
+            self.ptr = None
+
+
+class Plan:
+    """Device plan of one operator on one basis (para_Op + basis on the GPU)."""
+
+    def __init__(self, basis, type=10, V=None, G0=None, Ga=None, F2=None, F1=None, max_nvec=1):
+        self.basis = basis
+        f = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, float).ravel(order="F"))
+        self._keep = [f(V), f(G0), f(Ga), f(F2), f(F1)]
+        op = OpDesc(type, *[_d(a) for a in self._keep])
+        h = ctypes.c_void_p()
+        check(load().evr_plan_create(basis.ptr, ctypes.byref(op), max_nvec, ctypes.byref(h)))
+        self.h = h
+        self.max_nvec = max_nvec
+        z = (ctypes.c_int64 * 7)()
+        check(load().evr_plan_sizes(h, z))
+        self.nb_packed = z[0]
+
+    def set_potential(self, V):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        check(load().evr_plan_set_potential(self.h, _d(V)))
+
+    def gen_potential_poly(self, ham):
+        D = ham.D
+        c3 = np.ascontiguousarray(ham.c3.reshape(D * D * D))   # (i*D + j)*D + k
+        d22 = np.ascontiguousarray(ham.d22.reshape(D * D))     # i*D + j
+        check(load().evr_plan_gen_potential_poly(self.h, _d(np.ascontiguousarray(ham.v2)), _d(c3),
+                                                 _d(d22), _d(np.ascontiguousarray(ham.e4))))
+
+    def TabOpPsi(self, psi, flags=FP64):
+        """H|psi> for a block of vectors (nb_packed x nvec), host arrays."""
+        psi = np.asfortranarray(psi, dtype=np.float64)
+        if psi.ndim == 1:
+            psi = psi.reshape(-1, 1, order="F")
+        out = np.zeros_like(psi, order="F")
+        check(load().evr_oppsi(self.h, psi.shape[1], psi.ctypes.data, out.ctypes.data, flags))
+        return out
+
+    def TabOpPsi_device(self, psi_ptr, hpsi_ptr, nvec, flags=FP64):
+        """H|psi> on device buffers (column-major), queued on the plan stream."""
+        check(load().evr_oppsi(self.h, nvec, ctypes.c_void_p(psi_ptr), ctypes.c_void_p(hpsi_ptr),
+                               flags | DEVICE_PTRS))
+
+    def stream(self):
+        return load().evr_plan_stream(self.h)
+
+    def BasisRep_TO_GridRep(self, psiB):
+        psiB = np.ascontiguousarray(psiB, dtype=np.float64)
+        g = np.zeros(self.basis.sum_nq)
+        check(load().evr_basis_to_grid(self.h, _d(psiB), _d(g)))
+        return g
+
+    def GridRep_TO_BasisRep(self, psiG):
+        psiG = np.ascontiguousarray(psiG, dtype=np.float64)
+        b = np.zeros(self.nb_packed)
+        check(load().evr_grid_to_basis(self.h, _d(psiG), _d(b)))
+        return b
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().evr_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

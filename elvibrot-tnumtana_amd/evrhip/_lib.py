@@ -1,0 +1,72 @@
+"""ctypes binding of libevrhip.so (the HIP engine's C ABI, include/evr_hip.h)."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libevrhip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class BasisDesc(ctypes.Structure):
+    _fields_ = [
+        ("D", ctypes.c_int), ("nlev", ctypes.c_int), ("nterms", ctypes.c_int64),
+        ("term_levels", _ip), ("weights", _dp), ("nb", _ip), ("nq", _ip),
+        ("x", _dp), ("B", _dp), ("W", _dp), ("D1", _dp), ("D2", _dp),
+        ("nb_packed", ctypes.c_int64), ("map", _ip), ("grid_last_fastest", ctypes.c_int),
+    ]
+
+
+class OpDesc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("V", _dp), ("G0", _dp), ("Ga", _dp), ("F2", _dp),
+                ("F1", _dp)]
+
+
+# name: (restype, argtypes)
+_SIGS = {
+    "evr_last_error": (ctypes.c_char_p, []),
+    "evr_sg4_hermite_desc": (ctypes.c_int, [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.POINTER(BasisDesc))]),
+    "evr_dp_hermite_desc": (ctypes.c_int, [ctypes.c_int, _ip, _ip, ctypes.POINTER(ctypes.POINTER(BasisDesc))]),
+    "evr_basis_desc_free": (None, [ctypes.POINTER(BasisDesc)]),
+    "evr_basis_desc_sizes": (ctypes.c_int, [ctypes.POINTER(BasisDesc), _i64p]),
+    "evr_basis_desc_grid": (ctypes.c_int, [ctypes.POINTER(BasisDesc), _dp]),
+    "evr_plan_create": (ctypes.c_int, [ctypes.POINTER(BasisDesc), ctypes.POINTER(OpDesc), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "evr_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "evr_plan_set_potential": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "evr_plan_gen_potential_poly": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "evr_plan_sizes": (ctypes.c_int, [ctypes.c_void_p, _i64p]),
+    "evr_plan_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "evr_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "evr_basis_to_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "evr_grid_to_basis": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+}
+
+_LIB = None
+
+
+def load():
+    """Load libevrhip.so; raises if it is not built (there is no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libevrhip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def declared_symbols():
+    return sorted(_SIGS)
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError("evrhip error %d: %s" % (rc, load().evr_last_error().decode()))
+    return rc

@@ -1,0 +1,163 @@
+/*
+ * evr_hip.h -- C ABI of the MI355X H|psi> engine (libevrhip.so).
+ *
+ * Drop-in boundary for ElVibRot-TnumTana's operator action on Smolyak (SG4)
+ * and direct-product (DP) vibrational bases.  Every entry point returns an
+ * int status: 0 = OK, < 0 = error (message via evr_last_error()); it never
+ * exits the process.  The reference STOPs on the same conditions; its Fortran
+ * host maps a nonzero status to STOP (see evr_hip_f.f90).
+ *
+ * Conventions
+ *  - Matrices are Fortran column-major, as in the reference's basis type
+ *    (dnRGB%d0 is nq x nb, dnRBGwrho%d0 is nb x nq, dnRGG%d1/d2 are nq x nq).
+ *  - Vectors psi are packed-basis, column-major nb_packed x nvec (one column
+ *    per psi, the reference's RvecB layout, PSIAL:88).  Complex psi is passed
+ *    as two real columns, as the reference does (OPPSI:392-408).
+ *  - Term-local tensors are first-dimension-fastest (BGSG4:2443-2476); the
+ *    concatenated Smolyak grid follows tab_Sum_nq_OF_SRep order.
+ *  - The caller owns host arrays; a plan owns its device buffers and stream.
+ *    One caller thread per plan; different plans are independent.
+ *  - Reductions are deterministic: the weighted Smolyak scatter is a
+ *    segmented sum in a fixed order, not atomics.
+ *
+ * Reference file abbreviations (relative to the reference root):
+ *   OPPSI Source_ElVibRot/sub_Operator/sub_OpPsi.f90
+ *   OPSG4 Source_ElVibRot/sub_Operator/sub_OpPsi_SG4.f90
+ *   BGSG4 Source_ElVibRot/sub_Basis/sub_Basis_SG4/sub_module_basis_BtoG_GtoB_SG4.f90
+ *   BG    Source_ElVibRot/sub_Basis/sub_module_basis_BtoG_GtoB.f90
+ *   PSIBG Source_ElVibRot/sub_WP/sub_module_psi_B_TO_G.f90
+ *   BSET  Source_ElVibRot/sub_Basis/sub_module_basis_set_alloc.f90
+ *   SGT2  Source_ElVibRot/sub_Basis/sub_module_param_SGType2.f90
+ *   DAV   Source_ElVibRot/sub_propagation/sub_module_Davidson.f90
+ *   MARCH Source_ElVibRot/sub_propagation/sub_module_propa_march.f90
+ *   CTRL  Source_ElVibRot/sub_propagation/sub_control.f90
+ */
+#ifndef EVR_HIP_H
+#define EVR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EVR_OK 0
+#define EVR_ERR_ARG -1
+#define EVR_ERR_HIP -2
+#define EVR_ERR_SIZE -3
+#define EVR_ERR_STATE -4
+
+/* evr_oppsi flags */
+#define EVR_FP64 0          /* all contractions in FP64                         */
+#define EVR_FP32 1          /* contractions in FP32, FP32 accumulation          */
+#define EVR_DEVICE_PTRS 2   /* psi/hpsi are device pointers on the plan stream  */
+
+/* ------------------------------------------------------------------------
+ * Basis description: what the reference's SG4 basis type holds
+ * (tab_basisPrimSG(0:L,1:D) with dnRGB/dnRBGwrho/dnRGG, BSET:85-102;
+ * param_SGType2 term table, weights and packed map, SGT2:54-102).  A DP basis
+ * is one term with weight 1 whose map is the DP ordering.
+ * ---------------------------------------------------------------------- */
+typedef struct evr_basis_desc {
+  int D;                     /* number of modes                                  */
+  int nlev;                  /* 1D basis slots per mode (L+1 for SG4, 1 for DP)  */
+  int64_t nterms;
+  const int *term_levels;    /* nterms x D, term-major: slot of mode d in term t */
+  const double *weights;     /* nterms: WeightSG (SGT2:754-854)                  */
+  const int *nb;             /* D x nlev (index d*nlev + l)                      */
+  const int *nq;             /* D x nlev                                         */
+  const double *x;           /* grid points, per slot nq values, slots in order  */
+  const double *B;           /* per slot nq x nb: dnRGB%d0                       */
+  const double *W;           /* per slot nb x nq: dnRBGwrho%d0                   */
+  const double *D1;          /* per slot nq x nq: dnRGG%d1                       */
+  const double *D2;          /* per slot nq x nq: dnRGG%d2                       */
+  int64_t nb_packed;
+  const int *map;            /* sum_t nb_t: packed index (0-based) of each
+                                term-local basis function, -1 if pruned
+                                (tab_iB_OF_SRep_TO_iB, BGSG4:625-949)           */
+  int grid_last_fastest;     /* 1 for a DP basis: V and grid vectors are in the
+                                reference's DP grid order, last mode fastest
+                                (BG:730-798); 0: concatenated term grids       */
+} evr_basis_desc;
+
+/* ------------------------------------------------------------------------
+ * Operator description (nb0 = 1, one electronic surface).
+ *  type 0 : V psi                                   (OPSG4:1448-1488)
+ *  type 1 : sum_ij F2_ij d_i d_j + sum_i F1_i d_i + V, constant F2/F1,
+ *           d_i d_i through dnRGG%d2                (OPSG4:1490-1546)
+ *  type 10: -1/2 sum_ij d_i G_ij(q) d_j + V with G_ij(q) = G0_ij +
+ *           sum_k Ga_ijk q_k, J = sqRhoOVERJac = 1   (OPSG4:1548-1650)
+ * V is one value per grid point of the concatenated term grids (the
+ * reference's OpGrid%Grid with Save_MemGrid, OPSG4:2656-2666), or NULL and
+ * set later with evr_plan_set_potential / evr_plan_gen_potential_poly.
+ * ---------------------------------------------------------------------- */
+typedef struct evr_op_desc {
+  int type;
+  const double *V;           /* sum_t nq_t or NULL                               */
+  const double *G0;          /* D x D                                            */
+  const double *Ga;          /* D x D x D (i + D*(j + D*k)) or NULL              */
+  const double *F2;          /* D x D (type 1)                                   */
+  const double *F1;          /* D (type 1) or NULL                               */
+} evr_op_desc;
+
+typedef struct evr_plan evr_plan;
+
+/* Last error message of this thread (never NULL). */
+const char *evr_last_error(void);
+
+/* ---- basis construction helpers (host) -------------------------------- */
+/* Hermite (HO) Smolyak SG4 basis with nb(l) = nbA + nbB*l, nq(l) = nqA +
+ * nqB*l, grid level LG, basis level LB (RecSparseGrid_ForDP_type4,
+ * Source_ElVibRot/sub_Basis/sub_quadra_SparseBasis.f90:924-1435).  The
+ * returned descriptor owns its arrays; free with evr_basis_desc_free. */
+int evr_sg4_hermite_desc(int D, int LG, int LB, int nbA, int nbB, int nqA, int nqB,
+                         evr_basis_desc **out);
+/* Hermite direct-product basis, nb[d] functions on nq[d] points per mode,
+ * packed order last-dimension-fastest (BG:730-798). */
+int evr_dp_hermite_desc(int D, const int *nb, const int *nq, evr_basis_desc **out);
+void evr_basis_desc_free(evr_basis_desc *d);
+/* Sizes: out[0]=sum nb_t, out[1]=sum nq_t, out[2]=max nb_t, out[3]=max nq_t. */
+int evr_basis_desc_sizes(const evr_basis_desc *d, int64_t *out);
+/* Coordinates of every grid point (sum nq_t x D, point-major, host). */
+int evr_basis_desc_grid(const evr_basis_desc *d, double *q);
+
+/* ---- plans -------------------------------------------------------------- */
+/* Replaces the per-call setup of sub_TabOpPsi_FOR_SGtype4 (OPSG4:678-979):
+ * uploads tables, 1D matrices, the basis-space KEO factors and V.
+ * max_nvec bounds the vectors per evr_oppsi call. */
+int evr_plan_create(const evr_basis_desc *basis, const evr_op_desc *op, int max_nvec,
+                    evr_plan **out);
+int evr_plan_destroy(evr_plan *p);
+/* Replace V (host array, sum nq_t values). */
+int evr_plan_set_potential(evr_plan *p, const double *V);
+/* Generate V on the device: V(q) = sum_i v2_i q_i^2 + sum_{i<=j<=k} c_ijk
+ * q_i q_j q_k + sum_{i<j} d_ij q_i^2 q_j^2 + sum_i e_i q_i^4, with c given as a
+ * dense D^3 array (only i<=j<=k read), d as D x D (only i<j read). */
+int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
+                                const double *d22, const double *e4);
+/* Sizes: out[0]=nb_packed, [1]=nterms, [2]=sum nb_t, [3]=sum nq_t,
+ * [4]=max nb_t, [5]=max nq_t, [6]=D. */
+int evr_plan_sizes(const evr_plan *p, int64_t *out);
+/* hipStream_t of the plan (as void*), for callers that pass device pointers. */
+void *evr_plan_stream(evr_plan *p);
+
+/* ---- operator action ---------------------------------------------------- */
+/* H|psi> for nvec vectors: sub_TabOpPsi(Psi(:), OpPsi(:), para_Op)
+ * (OPPSI:701-793) dispatched to sub_TabOpPsi_FOR_SGtype4 (OPSG4:678) for
+ * SG4 and to sub_TabOpPsi_WITH_MemGrid_BGG_Hamil10 (OPPSI:1797) for DP.
+ * psi, hpsi: nb_packed x nvec column-major.  flags: EVR_FP64 | EVR_FP32,
+ * optionally | EVR_DEVICE_PTRS. */
+int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags);
+
+/* BasisTOGrid: sub_PsiBasisRep_TO_GridRep (PSIBG:60) -> RecRvecB_TO_RVecG
+ * (BG:605), SG4 branch BSmolyakRep_TO_GSmolyakRep (BGSG4:2216).  psiG is the
+ * concatenated term grid (sum nq_t).  One vector, host pointers. */
+int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG);
+/* GridTOBasis: sub_PsiGridRep_TO_BasisRep (PSIBG:178) -> RecRVecG_TO_RvecB
+ * (BG:59), SG4 branch with the weighted Smolyak sum (BGSG4:951-1028). */
+int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
