@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""H|psi> throughput of the MI355X engine on the D=9 Smolyak SG4 L=8 workload.
+
+One step = one H|psi> application (type-10 Hamiltonian, q-linear G) to a block
+of `--nvec` vectors resident in HBM, through the C ABI (evr_oppsi with device
+pointers).  The value is H|psi> applications per second per vector
+(vectors x steps / time).  Prints one JSON line.
+
+The CPU baseline leg times the oracle (a C/OpenMP restatement of the
+reference's per-term OpenMP loop, PSG4_omp=1) on a strided sample of the same
+terms on the host cores and scales by the grid-point fraction of the sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "elvibrot-tnumtana_amd"))
+
+PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 (matrix and vector are equal on gfx950)
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(D, L, ham, nvec_cpu, stride, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    from evrhip import hamiltonian as H
+    s = oracle.SG4(D, L)
+    _, _, _, onq = s.terms()
+    terms = np.arange(0, s.nterms, stride, dtype=np.int64)
+    q_sample = float(sum(onq[t + 1] - onq[t] for t in terms))
+    # V values do not change the work; a zero-filled (lazily mapped) array
+    # keeps host memory to the sampled terms' pages.
+    V = np.zeros(s.sum_nq)
+    op = oracle.Op(10, V=V, G0=ham.G0, Ga=ham.Ga_fortran())
+    X = H.test_vectors(s.nb_packed, nvec_cpu, 2)
+    t0 = time.perf_counter()
+    s.oppsi_terms(op, X, terms, nthreads=threads)
+    dt = time.perf_counter() - t0
+    frac = q_sample / s.sum_nq
+    full = dt / frac
+    return {"value": nvec_cpu / full, "unit": "Hpsi/s per vector", "cores": threads,
+            "kind": "port",
+            "sample": "oracle (C/OpenMP restatement of the reference per-term loop, FP64), "
+                      "%d vector(s), every %d-th Smolyak term (%d of %d terms, %.1f%% of the grid "
+                      "points), %.1f s wall, scaled by the grid-point fraction"
+                      % (nvec_cpu, stride, len(terms), s.nterms, 100 * frac, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nvec", type=int, default=32)
+    ap.add_argument("--D", type=int, default=9)
+    ap.add_argument("--L", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the H|psi> engine runs on a single MI355X "
+              "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import evrhip
+    from evrhip import hamiltonian as H
+
+    D, L, nvec = args.D, args.L, args.nvec
+    cfg_index = 2 if (D, L) == (9, 8) else (1 if (D, L) == (6, 5) else 9)
+    ham = H.smolyak_config(cfg_index, D)
+    t0 = time.perf_counter()
+    basis = evrhip.Basis.sg4(D, L)
+    plan = evrhip.Plan(basis, type=10, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=nvec)
+    plan.gen_potential_poly(ham)
+    log("plan: D=%d L=%d terms=%d nb_packed=%d sum_nq=%d (%.1f s)"
+        % (D, L, basis.nterms, basis.nb_packed, basis.sum_nq, time.perf_counter() - t0))
+
+    X = torch.from_numpy(np.ascontiguousarray(H.test_vectors(basis.nb_packed, nvec, cfg_index).T)).cuda()
+    Hx = torch.empty_like(X)
+    stream = torch.cuda.ExternalStream(plan.stream())
+
+    def step(flags):
+        plan.TabOpPsi_device(X.data_ptr(), Hx.data_ptr(), nvec, flags)
+
+    def timed(flags):
+        for _ in range(args.warmup):
+            step(flags)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        w0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(flags)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        return e0.elapsed_time(e1) / args.steps, wall
+
+    ms64, wall64 = timed(evrhip.FP64)
+    log("fp64: %.3f ms per step (%d vectors)" % (ms64, nvec))
+    ms32 = None
+    if not args.no_fp32:
+        ms32, _ = timed(evrhip.FP32)
+        log("fp32: %.3f ms per step" % ms32)
+
+    # per-kernel roofline from HIP events around each term-kernel launch
+    plan.set_profiling(True)
+    step(evrhip.FP64)
+    step(evrhip.FP64)
+    torch.cuda.synchronize()
+    stats = plan.kernel_stats()
+    plan.set_profiling(False)
+    dom = max(stats, key=lambda s: s[1])
+    name, ms, flops, launches = dom
+    achieved = flops / (ms / 1e3) / 1e12
+    tot_ms = sum(s[1] for s in stats)
+    tot_fl = sum(s[2] for s in stats)
+    for s in stats:
+        log("  %-16s %8.3f ms/launch  %7.2f TFLOP/s" % (s[0], s[1] / max(s[3], 1),
+                                                       s[2] / (s[1] / 1e3) / 1e12 if s[1] else 0))
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
+                "kernel": name, "avg_launch_ms": round(ms / launches, 4),
+                "all_term_kernels_tflops": round(tot_fl / (tot_ms / 1e3) / 1e12, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("cpu baseline: %d threads, stride %d" % (threads, args.cpu_stride))
+        cpu = cpu_baseline(D, L, ham, 1, args.cpu_stride, threads)
+        log("cpu: %.4g Hpsi/s per vector" % cpu["value"])
+
+    value = nvec / (ms64 / 1e3)
+    line = {
+        "metric": "Hpsi/s per vector, D=%d SG4 L=%d" % (D, L),
+        "value": round(value, 3),
+        "unit": "Hpsi/s per vector",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms64, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded cubic/quartic potential, q-linear G, random normalised vectors)",
+        "config": {"workload": "SG4 D=%d L=%d type-10 H|psi>, %d Smolyak terms, block of %d "
+                               "vectors" % (D, L, basis.nterms, nvec),
+                   "nb_packed": basis.nb_packed, "grid_points": basis.sum_nq, "nvec": nvec},
+        "fp32": None if ms32 is None else {"value": round(nvec / (ms32 / 1e3), 3),
+                                           "ms_per_step": round(ms32, 4),
+                                           "speedup_vs_fp64": round(ms64 / ms32, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -97,6 +97,14 @@ struct evr_plan {
   // work buffers
   DevBuf<double> d_X, d_R, d_psi, d_hpsi, d_G;
   bool V32_valid = false;
+  // per-launch accounting: algorithmic flops per vector of each kernel class
+  double flops_class[kNumClasses] = {};
+  double flops_reduce = 0;
+  bool profiling = false;
+  hipEvent_t ev[2 * (kNumClasses + 1)] = {};
+  double ms_acc[kNumClasses + 1] = {};
+  double flops_acc[kNumClasses + 1] = {};
+  int64_t launches[kNumClasses + 1] = {};
 };
 
 namespace {
@@ -148,6 +156,44 @@ evr::TermParams<T> base_params(evr_plan *p, int mode, int nvec)
   return prm;
 }
 
+// Algorithmic flops of one term for one vector (the single-pass algorithm:
+// i-group re-computation is not counted).
+double term_flops(const evr_plan *p, const evr::TermDev &td)
+{
+  const int D = p->D;
+  int nb[evr::kMaxDev], nq[evr::kMaxDev];
+  for (int d = 0; d < D; ++d) { nb[d] = p->slots[td.slot[d]].nb; nq[d] = p->slots[td.slot[d]].nq; }
+  double f = 0.0;
+  if (p->hasV) {
+    double leaf = 2.0 * td.nbA * td.nbB + 2.0 * td.nbt + (double)td.nbB * (D - td.m) * 2.0 + td.QA;
+    for (int d = 0; d < td.m; ++d) {
+      double n1 = 1, n3 = 1, m1 = 1, m3 = 1;
+      for (int e = 0; e < d; ++e) { n1 *= nq[e]; m1 *= nb[e]; }
+      for (int e = d + 1; e < td.m; ++e) { n3 *= nb[e]; m3 *= nq[e]; }
+      leaf += 2.0 * n1 * nb[d] * nq[d] * n3 + 2.0 * m1 * nq[d] * nb[d] * m3;
+    }
+    f += leaf * td.QB;
+  }
+  if (p->hasKeo) {
+    double e = 0.0;
+    for (int j = 0; j < D; ++j) {
+      e += 2.0 * nb[j] + 2.0 * (D - 1);
+      if (p->hasA) {
+        e += 4.0 * nb[j] + 6.0 * (D - 1);
+        for (int k = 0; k < D; ++k)
+          if (k != j) e += 2.0 * nb[k] + 2.0 * (D - 2);
+      }
+    }
+    for (int i = 0; i < D; ++i) {
+      e += 4.0 * nb[i] + 2.0;
+      if (p->hasA) e += 4.0 * nb[i] + 2.0;
+      if (p->hasF1) e += 2.0 * nb[i] + 2.0;
+    }
+    f += e * td.nbt;
+  }
+  return f;
+}
+
 // Apply every term class for `nvec` vectors held in X ([nb_packed][nvec]).
 template <typename T>
 int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *Gin, const T *bw,
@@ -166,8 +212,18 @@ int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *
     prm.G = G;
     prm.Gin = Gin;
     size_t lds = sizeof(T) == 8 ? p->lds_bytes64[c] : p->lds_bytes32[c];
+    if (p->profiling) (void)hipEventRecord(p->ev[2 * c], p->stream);
     hipError_t e = launch_class<T>(c, prm, p->nunits[c], nvec, lds, p->stream);
     if (e != hipSuccess) return fail(EVR_ERR_HIP, std::string("term kernel: ") + hipGetErrorString(e));
+    if (p->profiling) {
+      (void)hipEventRecord(p->ev[2 * c + 1], p->stream);
+      (void)hipEventSynchronize(p->ev[2 * c + 1]);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, p->ev[2 * c], p->ev[2 * c + 1]);
+      p->ms_acc[c] += ms;
+      p->flops_acc[c] += p->flops_class[c] * nvec;
+      p->launches[c] += 1;
+    }
   }
   return EVR_OK;
 }
@@ -181,6 +237,8 @@ unsigned blocks_for(int64_t n)
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI --
+static void recount_flops(evr_plan *p);
+
 extern "C" {
 
 const char *evr_last_error(void) { return g_err.c_str(); }
@@ -429,7 +487,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
       QA *= nqd[m - 1];
       const int64_t nbB = nbt / nbA;
       const size_t work = std::max<size_t>(p->hasKeo ? nbt : 0, 2 * QA + 2 * nbB);
-      if (base + work <= budget) best = m; else break;
+      if (base + work <= budget) best = m;
     }
     if (best < 1) return fail(EVR_ERR_SIZE, "term does not fit in LDS");
     td.m = best;
@@ -443,6 +501,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     p->lds_bytes64[c] = std::max(p->lds_bytes64[c], words * 8);
     p->lds_bytes32[c] = std::max(p->lds_bytes32[c], words * 4);
     cost[t] = (double)nqt * 50.0 + (double)nbt * D * D * 6.0;
+    p->flops_class[c] += term_flops(p.get(), td);
     onb += nbt;
     onq += nqt;
     p->max_nb = std::max(p->max_nb, nbt);
@@ -536,6 +595,8 @@ int evr_plan_destroy(evr_plan *p)
   p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release();
   for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
   p->d_X.release(); p->d_R.release(); p->d_psi.release(); p->d_hpsi.release(); p->d_G.release();
+  for (auto &e : p->ev)
+    if (e) (void)hipEventDestroy(e);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return EVR_OK;
@@ -554,8 +615,10 @@ int evr_plan_set_potential(evr_plan *p, const double *V)
   HIP_TRY(hipMemcpyAsync(p->d_V.ptr, src, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice,
                          p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
+  const bool had = p->hasV;
   p->hasV = true;
   p->V32_valid = false;
+  if (!had) recount_flops(p);
   return EVR_OK;
 }
 
@@ -578,6 +641,7 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
   dcoef.release();
   p->hasV = true;
   p->V32_valid = false;
+  recount_flops(p);
   return EVR_OK;
 }
 
@@ -590,6 +654,44 @@ int evr_plan_sizes(const evr_plan *p, int64_t *out)
 }
 
 void *evr_plan_stream(evr_plan *p) { return p ? (void *)p->stream : nullptr; }
+
+static void recount_flops(evr_plan *p)
+{
+  for (int c = 0; c < kNumClasses; ++c) p->flops_class[c] = 0;
+  for (const evr::TermDev &td : p->terms) {
+    int c = 0;
+    while ((int64_t)kClasses[c].nt * kClasses[c].ept < td.nbt) ++c;
+    p->flops_class[c] += term_flops(p, td);
+  }
+}
+
+int evr_plan_set_profiling(evr_plan *p, int on)
+{
+  if (!p) return fail(EVR_ERR_ARG, "NULL plan");
+  if (on && !p->ev[0])
+    for (auto &e : p->ev) HIP_TRY(hipEventCreate(&e));
+  p->profiling = on != 0;
+  for (int c = 0; c <= kNumClasses; ++c) p->ms_acc[c] = p->flops_acc[c] = 0, p->launches[c] = 0;
+  return EVR_OK;
+}
+
+int evr_plan_kernel_stats(evr_plan *p, int max, char *names, double *ms, double *flops,
+                          int64_t *launches, int *n)
+{
+  if (!p || !n) return fail(EVR_ERR_ARG, "NULL argument");
+  int k = 0;
+  for (int c = 0; c < kNumClasses && k < max; ++c) {
+    if (p->nunits[c] == 0) continue;
+    if (names)
+      snprintf(names + 64 * k, 64, "k_term<%d,%d>", kClasses[c].nt, kClasses[c].ept);
+    if (ms) ms[k] = p->ms_acc[c];
+    if (flops) flops[k] = p->flops_acc[c];
+    if (launches) launches[k] = p->launches[c];
+    ++k;
+  }
+  *n = k;
+  return EVR_OK;
+}
 
 static int ensure_v32(evr_plan *p)
 {

@@ -118,6 +118,24 @@ class Plan:
     def stream(self):
         return load().evr_plan_stream(self.h)
 
+    def set_profiling(self, on=True):
+        check(load().evr_plan_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        """[(name, ms, flops, launches)] accumulated while profiling."""
+        mx = 16
+        names = ctypes.create_string_buffer(64 * mx)
+        ms = np.zeros(mx); fl = np.zeros(mx); la = np.zeros(mx, dtype=np.int64)
+        n = ctypes.c_int()
+        check(load().evr_plan_kernel_stats(self.h, mx, names, _d(ms), _d(fl),
+                                           la.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                           ctypes.byref(n)))
+        out = []
+        for k in range(n.value):
+            nm = names.raw[64 * k:64 * (k + 1)].split(b"\0")[0].decode()
+            out.append((nm, ms[k], fl[k], int(la[k])))
+        return out
+
     def BasisRep_TO_GridRep(self, psiB):
         psiB = np.ascontiguousarray(psiB, dtype=np.float64)
         g = np.zeros(self.basis.sum_nq)

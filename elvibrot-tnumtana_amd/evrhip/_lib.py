@@ -39,6 +39,9 @@ _SIGS = {
     "evr_plan_gen_potential_poly": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "evr_plan_sizes": (ctypes.c_int, [ctypes.c_void_p, _i64p]),
     "evr_plan_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "evr_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "evr_plan_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, _dp, _dp, _i64p,
+                                             ctypes.POINTER(ctypes.c_int)]),
     "evr_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "evr_basis_to_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "evr_grid_to_basis": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),

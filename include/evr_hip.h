@@ -140,6 +140,14 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
 int evr_plan_sizes(const evr_plan *p, int64_t *out);
 /* hipStream_t of the plan (as void*), for callers that pass device pointers. */
 void *evr_plan_stream(evr_plan *p);
+/* Profiling (no counterpart in the reference, whose analogue is the
+ * nb_mult_BTOG/GTOB counters, SYS:120): when on, every term-kernel launch is
+ * bracketed by HIP events on the plan stream; evr_plan_kernel_stats returns,
+ * per kernel class, the accumulated milliseconds, algorithmic flops and
+ * launch count (names: 64 chars each).  Turning it on resets the counters. */
+int evr_plan_set_profiling(evr_plan *p, int on);
+int evr_plan_kernel_stats(evr_plan *p, int max, char *names, double *ms, double *flops,
+                          int64_t *launches, int *n);
 
 /* ---- operator action ---------------------------------------------------- */
 /* H|psi> for nvec vectors: sub_TabOpPsi(Psi(:), OpPsi(:), para_Op)

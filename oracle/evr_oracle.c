@@ -649,14 +649,36 @@ static void term_grid_op(int D, const orc_1d *const *b, const orc_op *op, long l
     psiG[p] = -0.5 * OpPsiR[p] / ((J ? J[p] : 1.0) * (s ? s[p] : 1.0)) + VPsi[p];
 }
 
+static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const double *psi,
+                          double *hpsi, int nthreads, const long long *list, long long nlist);
+
 int orc_sg4_oppsi_range(const orc_sg4 *s, const orc_op *op, int nvec, const double *psi,
                         double *hpsi, int nthreads, long long t0, long long t1)
+{
+  if (t0 < 0) t0 = 0;
+  if (t1 > s->nterms) t1 = s->nterms;
+  if (t1 < t0) t1 = t0;
+  long long *list = (long long *)malloc(sizeof(long long) * (t1 - t0 + 1));
+  for (long long t = t0; t < t1; ++t) list[t - t0] = t;
+  int rc = sg4_oppsi_list(s, op, nvec, psi, hpsi, nthreads, list, t1 - t0);
+  free(list);
+  return rc;
+}
+
+int orc_sg4_oppsi_terms(const orc_sg4 *s, const orc_op *op, int nvec, const double *psi,
+                        double *hpsi, int nthreads, const long long *terms, long long nlist)
+{
+  for (long long i = 0; i < nlist; ++i)
+    if (terms[i] < 0 || terms[i] >= s->nterms) return -3;
+  return sg4_oppsi_list(s, op, nvec, psi, hpsi, nthreads, terms, nlist);
+}
+
+static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const double *psi,
+                          double *hpsi, int nthreads, const long long *list, long long nlist)
 {
   const int D = s->D;
   if (nvec < 1) return -1;                                  /* OPSG4:737-750 */
   if (op->type != 0 && op->type != 1 && op->type != 10) return -2;
-  if (t0 < 0) t0 = 0;
-  if (t1 > s->nterms) t1 = s->nterms;
   /* OpPsi = 0 (OPSG4:761-767) */
   memset(hpsi, 0, sizeof(double) * s->nb_packed * nvec);
 #ifdef _OPENMP
@@ -680,7 +702,8 @@ int orc_sg4_oppsi_range(const orc_sg4 *s, const orc_op *op, int nvec, const doub
     const orc_1d *b[64];
     /* PSG4_omp=1: static schedule over terms (OPSG4:822-865) */
 #pragma omp for schedule(static)
-    for (long long t = t0; t < t1; ++t) {
+    for (long long it = 0; it < nlist; ++it) {
+      const long long t = list[it];
       const int *l = s->levels + t * D;
       for (int d = 0; d < D; ++d) b[d] = &s->bas[l[d] + (s->LG + 1) * d];
       long long nbt = s->off_nb[t + 1] - s->off_nb[t];

@@ -103,6 +103,10 @@ int orc_sg4_oppsi_range(const orc_sg4 *s, const orc_op *op, int nvec,
                         const double *psi, double *hpsi, int nthreads,
                         long long t0, long long t1);
 
+/* Same, for an explicit list of terms. */
+int orc_sg4_oppsi_terms(const orc_sg4 *s, const orc_op *op, int nvec, const double *psi,
+                        double *hpsi, int nthreads, const long long *terms, long long nlist);
+
 /* BasisTOGrid / GridTOBasis on SG4 (CS4): psi_G is the concatenation of the
  * term grids (tab_Sum_nq order).  b2g: tabPackedBasis_TO_SmolyakRepBasis +
  * BSmolyakRep_TO_GSmolyakRep (BGSG4:1032, BGSG4:2216).  g2b:

@@ -39,6 +39,8 @@ def lib():
         L.orc_sg4_oppsi_range.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp,
                                           _dp, ctypes.c_int, ctypes.c_longlong,
                                           ctypes.c_longlong]
+        L.orc_sg4_oppsi_terms.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp,
+                                          _dp, ctypes.c_int, _llp, ctypes.c_longlong]
         L.orc_sg4_b2g.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_sg4_g2b.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_herm_quadrature.argtypes = [ctypes.c_int, _dp, _dp]
@@ -178,6 +180,18 @@ class SG4:
                                            self.nterms if t1 is None else t1)
         if rc != 0:
             raise RuntimeError("orc_sg4_oppsi failed: %d" % rc)
+        return out
+
+    def oppsi_terms(self, op, psi, terms, nthreads=0):
+        psi = np.asfortranarray(psi, dtype=np.float64)
+        if psi.ndim == 1:
+            psi = psi.reshape(-1, 1, order="F")
+        out = np.zeros_like(psi, order="F")
+        terms = np.ascontiguousarray(terms, dtype=np.int64)
+        rc = lib().orc_sg4_oppsi_terms(self.h, ctypes.addressof(op.st), psi.shape[1], _d(psi),
+                                       _d(out), nthreads, terms.ctypes.data_as(_llp), len(terms))
+        if rc != 0:
+            raise RuntimeError("orc_sg4_oppsi_terms failed: %d" % rc)
         return out
 
     def b2g(self, psiB):
