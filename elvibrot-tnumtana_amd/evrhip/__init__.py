@@ -69,9 +69,12 @@ class Basis:
         return np.ctypeslib.as_array(self.ptr.contents.map, shape=(self.sum_nb,)).copy()
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            load().evr_basis_desc_free(self.ptr)
-            self.ptr = None
+        try:
+            if getattr(self, "ptr", None):
+                load().evr_basis_desc_free(self.ptr)
+        except Exception:
+            pass
+        self.ptr = None
 
 
 class Plan:
@@ -154,4 +157,7 @@ class Plan:
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,188 @@
+"""The oracle pinned against the reference's own data and against mathematics.
+
+Fixtures (tests/golden/*.json, written by tests/golden/make_golden.py from the
+reference's data files):
+  * Internal_data/HermQuadra/herm{nq}.txt: Gauss-Hermite nodes and w*exp(x^2);
+  * Working_tests/MPI_tests/{6D,21D}_Davidson_S0/benchmark: Henon-Heiles SG4
+    Davidson levels (Type_HamilOp=1, conv_resi=5e-4 au).
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, hh_potential
+
+
+def test_quadrature_matches_reference_tables(oracle_mod):
+    tab = json.load(open(os.path.join(GOLDEN, "herm_tables.json")))
+    for nq in range(1, 41):
+        x, w = oracle_mod.herm_quadrature(nq)
+        xt = np.array([float(v) for v in tab[str(nq)]["x"]])
+        wt = np.array([float(v) for v in tab[str(nq)]["wex2"]])
+        assert np.abs(x - xt).max() <= 4e-16 * max(1.0, np.abs(xt).max()), nq
+        assert (np.abs(w - wt) / wt).max() <= 4e-16, nq
+
+
+@pytest.mark.parametrize("nb,nq", [(1, 2), (3, 4), (9, 10), (17, 18), (12, 14), (5, 5)])
+def test_hermite_basis_orthonormal_and_derivatives(oracle_mod, nb, nq):
+    x, w = oracle_mod.herm_quadrature(nq)
+    B, B1, B2 = oracle_mod.herm_basis(nb, x)
+    S = B.T @ np.diag(w) @ B                      # W B = I (exact quadrature)
+    assert np.abs(S - np.eye(nb)).max() < 1e-13
+    # closed forms of the Hermite functions: ladder relations
+    from numpy.polynomial.hermite import hermval
+    from math import factorial, pi, sqrt
+    for n in range(nb):
+        c = np.zeros(n + 1); c[n] = 1.0
+        phi = hermval(x, c) * np.exp(-x * x / 2) / sqrt(2.0 ** n * factorial(n) * sqrt(pi))
+        assert np.abs(B[:, n] - phi).max() < 1e-13
+        assert np.abs(B2[:, n] - (x * x - 2 * n - 1) * phi).max() < 1e-12
+    D0, D1, D2 = oracle_mod.herm_dngg(B, B1, B2)
+    assert np.abs(D0 @ B - B).max() < 1e-12           # BAS:2399-2440 self-check
+    assert np.abs(D1 @ B - B1).max() < 1e-12
+    assert np.abs(D2 @ B - B2).max() < 1e-11
+
+
+@pytest.mark.parametrize("D,L,sizes", [
+    (6, 5, (462, 3653, 25954, 396032, 2048)),
+    (9, 8, (24310, 598417, 14666470, 799884800, 131072)),
+    (12, 6, (18564, 369305, 4195284, 2432102400, 262144)),
+])
+def test_smolyak_sizes(oracle_mod, D, L, sizes):
+    s = oracle_mod.SG4(D, L)
+    assert (s.nterms, s.nb_packed, s.sum_nb, s.sum_nq, s.max_nq) == sizes
+    _, w, _, _ = s.terms()
+    assert abs(w.sum() - 1.0) < 1e-12                 # Smolyak weights sum to 1
+
+
+def test_smolyak_brute_force_small(oracle_mod):
+    """Terms, weights, packed basis and map by brute force at D=3, L=3."""
+    D, L = 3, 3
+    s = oracle_mod.SG4(D, L)
+    lv, w, onb, _ = s.terms()
+    Lmin = max(0, L - D + 1)                         # SPB:994
+    want = [l for l in itertools.product(range(L + 1), repeat=D) if Lmin <= sum(l) <= L]
+    want.sort(key=lambda l: tuple(reversed(l)))       # first mode fastest
+    assert [tuple(r) for r in lv] == want
+    from math import comb
+    for l, wt in zip(want, w):
+        dl = L - sum(l)
+        assert wt == ((-1) ** dl * comb(D - 1, dl) if dl <= D - 1 else 0)
+    lev = lambda n: (n + 1) // 2                       # nb(l) = 1 + 2l
+    packed = [t for t in itertools.product(range(2 * L + 1), repeat=D)
+              if sum(lev(n) for n in t) <= L]          # last mode fastest
+    assert [tuple(r) for r in s.packed()] == packed
+    idx = {t: i for i, t in enumerate(packed)}
+    m = s.map()
+    for t, l in enumerate(want):
+        nbs = [1 + 2 * li for li in l]
+        for e, tup in enumerate(itertools.product(*[range(n) for n in reversed(nbs)])):
+            assert m[onb[t] + e] == idx[tuple(reversed(tup))]
+
+
+def test_harmonic_closed_form(oracle_mod):
+    """Type 1 with F2 = -1/2 diag(w) and V = sum w_i q_i^2/2: with nq = nb + 1 the
+    quadrature integrates every term matrix exactly, so the SG4 spectrum is
+    sum_i w_i (n_i + 1/2) over the packed space, exactly."""
+    D, L = 4, 4
+    s = oracle_mod.SG4(D, L)
+    w = np.array([900.0, 1300.0, 1700.0, 2500.0])
+    V = 0.5 * (s.grid() ** 2) @ w
+    H = s.oppsi(oracle_mod.Op(1, V=V, F2=-0.5 * np.diag(w)), np.eye(s.nb_packed))
+    lev = lambda n: (n + 1) // 2
+    P = s.packed()
+    exact = np.array([sum(wi * (ni + 0.5) for wi, ni in zip(w, t)) for t in P])
+    assert np.abs(H - np.diag(exact)).max() < 1e-9 * exact.max()
+    assert len(P) == len([t for t in itertools.product(range(2 * L + 1), repeat=D)
+                          if sum(lev(n) for n in t) <= L])
+
+
+def test_type10_low_levels_near_harmonic(oracle_mod):
+    """Type 10 (d G d with dnRGG%d1 = B' pinv(B)) is not exact at the top
+    function of each term, but the low levels of a harmonic system stay close to
+    sum_i w_i (n_i + 1/2)."""
+    D, L = 4, 4
+    s = oracle_mod.SG4(D, L)
+    w = np.array([900.0, 1300.0, 1700.0, 2500.0])
+    V = 0.5 * (s.grid() ** 2) @ w
+    H = s.oppsi(oracle_mod.Op(10, V=V, G0=np.diag(w)), np.eye(s.nb_packed))
+    e = np.sort(np.linalg.eigvals(H).real)
+    assert abs(e[0] - 0.5 * w.sum()) < 1e-3
+    assert abs(e[1] - (0.5 * w.sum() + w[0])) < 1e-2
+
+
+def test_smolyak_matches_dense_kronecker(oracle_mod):
+    """D=2: the SG4 type-10 operator equals the combination of per-term dense
+    Kronecker operators built independently from the 1D matrices."""
+    D, L = 2, 3
+    s = oracle_mod.SG4(D, L)
+    rng = np.random.default_rng(5)
+    G0 = np.array([[1.0, 0.1], [0.1, 1.3]])
+    Ga = rng.uniform(-0.02, 0.02, (D, D, D)); Ga = (Ga + Ga.transpose(1, 0, 2)) / 2
+    q = s.grid()
+    V = 0.5 * (q ** 2).sum(1) + 0.1 * q[:, 0] * q[:, 1] ** 2
+    H = s.oppsi(oracle_mod.Op(10, V=V, G0=G0, Ga=Ga.ravel(order="F")), np.eye(s.nb_packed))
+    lv, w, onb, onq = s.terms()
+    m = s.map()
+    Hd = np.zeros((s.nb_packed, s.nb_packed))
+    for t in range(s.nterms):
+        one = [s.one_d(d, lv[t, d]) for d in range(D)]
+        # first mode fastest: kron(mode1, mode0)
+        K = lambda A0, A1: np.kron(A1, A0)
+        Bt = K(one[0]["B"], one[1]["B"]); Wt = K(one[0]["W"], one[1]["W"])
+        I0, I1 = np.eye(one[0]["nq"]), np.eye(one[1]["nq"])
+        Dm = [K(one[0]["D1"], I1), K(I0, one[1]["D1"])]
+        qt = q[onq[t]:onq[t + 1]]
+        op = np.diag(V[onq[t]:onq[t + 1]]).copy()
+        for i in range(D):
+            for j in range(D):
+                g = G0[j, i] + qt @ Ga[j, i, :]
+                op += -0.5 * Dm[i] @ np.diag(g) @ Dm[j]
+        Ht = Wt @ op @ Bt
+        rows = m[onb[t]:onb[t + 1]]
+        Hd[np.ix_(rows, rows)] += w[t] * Ht
+    assert np.abs(H - Hd).max() < 1e-11 * np.abs(Hd).max()
+
+
+def _hh_levels(oracle_mod, D, L):
+    s = oracle_mod.SG4(D, L, L, nbA=1, nbB=2, nqA=1, nqB=2)
+    V = hh_potential(s.grid())
+    H = s.oppsi(oracle_mod.Op(1, V=V, F2=-0.5 * np.eye(D)), np.eye(s.nb_packed))
+    return np.linalg.eigvalsh(0.5 * (H + H.T))
+
+
+@pytest.mark.parametrize("name,D,L", [("6D_Davidson_S0", 6, 3), ("21D_Davidson_S0", 21, 2)])
+def test_henon_heiles_reference_levels(oracle_mod, name, D, L):
+    """The reference's Davidson levels are Ritz values converged to residual
+    5e-4 au: they lie above the exact levels of the same discrete operator by at
+    most ~r^2/gap.  A wrong term, weight, map or derivative matrix moves the
+    levels by far more than that."""
+    bench = np.array(json.load(open(os.path.join(GOLDEN, "hh_benchmarks.json")))[name])
+    e = _hh_levels(oracle_mod, D, L)[:len(bench)]
+    diff = bench - e
+    assert diff.min() > -1e-9
+    assert diff.max() < 2e-7
+
+
+def test_dp_matches_dense(oracle_mod):
+    """Direct product, D=2: the DP type-10 operator equals the dense form."""
+    nb, nq = [4, 5], [6, 7]
+    d = oracle_mod.DP(nb, nq)
+    q = d.grid()
+    G0 = np.array([[1.2, 0.2], [0.2, 0.8]])
+    V = 0.5 * (q ** 2).sum(1) + 0.05 * q[:, 0] ** 3
+    H = d.oppsi(oracle_mod.Op(10, V=V, G0=G0), np.eye(d.nb_tot))
+    one = []
+    for k in range(2):
+        x, w = oracle_mod.herm_quadrature(nq[k])
+        B, B1, B2 = oracle_mod.herm_basis(nb[k], x)
+        _, D1, _ = oracle_mod.herm_dngg(B, B1, B2)
+        one.append((B, np.diag(w) @ B, D1))
+    # last mode fastest: kron(mode0, mode1)
+    Bt = np.kron(one[0][0], one[1][0]); Wt = np.kron(one[0][1], one[1][1]).T
+    Dm = [np.kron(one[0][2], np.eye(nq[1])), np.kron(np.eye(nq[0]), one[1][2])]
+    op = np.diag(V) - 0.5 * sum(G0[j, i] * Dm[i] @ Dm[j] for i in range(2) for j in range(2))
+    assert np.abs(H - Wt @ op @ Bt).max() < 1e-11 * np.abs(H).max()

@@ -1,0 +1,81 @@
+"""Host-side basis construction of the engine and the C ABI surface (no GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "evr_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(evr_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(evr):
+    lib = ctypes.CDLL(evr._lib.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the python binding declares exactly the header's functions
+    assert sorted(evr._lib.declared_symbols()) == names
+
+
+@pytest.mark.parametrize("D,L,LB,law", [(6, 5, 5, (1, 2, 2, 2)), (3, 4, 2, (1, 2, 2, 2)),
+                                        (6, 3, 3, (1, 2, 1, 2)), (4, 3, 3, (1, 1, 1, 1))])
+def test_sg4_tables_match_oracle(evr, oracle_mod, D, L, LB, law):
+    b = evr.Basis.sg4(D, L, LB, *law)
+    s = oracle_mod.SG4(D, L, LB, *law)
+    lv, w, _, _ = s.terms()
+    assert b.nterms == s.nterms and b.nb_packed == s.nb_packed
+    assert (b.term_levels() == lv).all()
+    assert (b.weights() == w).all()
+    assert (b.map() == s.map()).all()
+    assert np.abs(b.grid() - s.grid()).max() < 4e-16 * 10
+
+
+def test_1d_matrices_match_oracle(evr, oracle_mod):
+    """The engine's 1D matrices (independent methods: Sturm bisection nodes,
+    Cholesky pseudo-inverse) agree with the oracle's reference-style ones."""
+    b = evr.Basis.sg4(2, 8)
+    s = oracle_mod.SG4(2, 8)
+    d = b.ptr.contents
+    nlev = d.nlev
+    offq = offB = offD = 0
+    for l in range(nlev):
+        nb, nq = d.nb[l], d.nq[l]
+        o = s.one_d(0, l)
+        x = np.ctypeslib.as_array(d.x, shape=(offq + nq,))[offq:]
+        B = np.ctypeslib.as_array(d.B, shape=(offB + nb * nq,))[offB:].reshape(nb, nq).T
+        W = np.ctypeslib.as_array(d.W, shape=(offB + nb * nq,))[offB:].reshape(nq, nb).T
+        D1 = np.ctypeslib.as_array(d.D1, shape=(offD + nq * nq,))[offD:].reshape(nq, nq).T
+        D2 = np.ctypeslib.as_array(d.D2, shape=(offD + nq * nq,))[offD:].reshape(nq, nq).T
+        assert np.abs(x - o["x"]).max() < 1e-15 * 10
+        assert np.abs(B - o["B"]).max() < 1e-14
+        assert np.abs(W - o["W"]).max() < 1e-14
+        assert np.abs(D1 - o["D1"]).max() < 1e-11 * max(1, np.abs(o["D1"]).max())
+        assert np.abs(D2 - o["D2"]).max() < 1e-10 * max(1, np.abs(o["D2"]).max())
+        offq += nq; offB += nb * nq; offD += nq * nq
+
+
+def test_dp_desc(evr, oracle_mod):
+    b = evr.Basis.dp([12, 12, 12], [14, 14, 14])
+    d = oracle_mod.DP([12, 12, 12], [14, 14, 14])
+    assert b.nb_packed == d.nb_tot == 1728 and b.sum_nq == d.nq_tot == 2744
+    assert np.abs(b.grid() - d.grid()).max() < 4e-15
+    m = b.map()
+    assert sorted(m) == list(range(1728))
+    assert m[1] == 144  # term-local first mode fastest -> DP last mode fastest
+
+
+def test_bad_arguments_raise(evr):
+    with pytest.raises(RuntimeError):
+        evr.Basis.sg4(0, 3)
+    with pytest.raises(RuntimeError):
+        evr.Basis.sg4(3, 2, 4)
+    with pytest.raises(RuntimeError):
+        evr.Basis.dp([5], [4])
