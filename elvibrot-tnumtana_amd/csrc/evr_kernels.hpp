@@ -8,6 +8,7 @@ namespace evr {
 constexpr int kMaxDev = 12;   // modes supported by the term kernel (5-bit digits in a u64)
 constexpr int kMaxN = 20;     // largest nb or nq of a 1D slot
 constexpr int kKeoFactors = 7;
+constexpr int kDp = 16;       // padded stride of the KEO coefficient strips
 
 // Per Smolyak term, as the kernel needs it.
 struct TermDev {
@@ -19,6 +20,16 @@ struct TermDev {
   int lds_mats;               // doubles of 1D matrices staged in LDS
   double w;                   // Smolyak weight (SGT2:754-854)
   int slot[kMaxDev];          // 1D slot of each mode
+  int step0;                  // first of the 2m slab contraction steps (StepDev table)
+};
+
+// One contraction step inside a slab: out[i1 + n1*(q + nout*i3)] =
+// sum_k M[q + nout*k] in[i1 + n1*(k + nin*i3)], M at pool offset `mat`.
+struct StepDev {
+  int n1, n3, nin, nout;
+  float rn1;                  // 1/(n1*NVW) is formed on the device
+  int pad;
+  int64_t mat;
 };
 
 // Per 1D slot (mode, level): offsets into the matrix pools.
@@ -42,6 +53,7 @@ struct TermParams {
   const int *order;           // term ids for this launch (-1 = padding)
   const TermDev *terms;
   const SlotDev *slots;
+  const StepDev *steps;
   const T *bw;                // pool of B and W matrices
   const T *keo;               // pool of basis-space KEO factors
   const int *map;             // term-local -> packed
@@ -49,6 +61,7 @@ struct TermParams {
   const double *coefC;        // D x D: c[i*D+j]
   const double *coefA;        // D^3: a[(i*D+j)*D+k]
   const double *coefF1;       // D
+  const double *keoc;         // pre-zeroed coefficient strips (see plan)
   const T *X;                 // packed input, [nb_packed][nvec]
   T *R;                       // term results, [sum nb_t][nvec]
   T *G;                       // grid output (B2G), [sum nq_t]

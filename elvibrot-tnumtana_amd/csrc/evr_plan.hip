@@ -60,9 +60,11 @@ struct DevBuf {
 
 // work class of the term kernel: threads per workgroup x elements per thread
 struct KClass {
-  int nt, ept;
+  int nt, ept, nvw;
+  int max_nbt() const { return nt * ept / nvw; }
 };
-constexpr KClass kClasses[] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {512, 8}, {512, 16}};
+constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {1024, 1, 4}, {1024, 1, 2},
+                               {1024, 1, 1}, {1024, 2, 1}, {1024, 4, 1}, {1024, 8, 1}};
 constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
 constexpr size_t kLdsLimit = 160 * 1024 - 1024;
 
@@ -86,11 +88,13 @@ struct evr_plan {
   // device tables
   DevBuf<evr::TermDev> d_terms;
   DevBuf<evr::SlotDev> d_slots;
+  std::vector<evr::StepDev> steps;
+  DevBuf<evr::StepDev> d_steps;
   DevBuf<double> d_bw64, d_keo64, d_x;
   DevBuf<float> d_bw32, d_keo32, d_V32;
   DevBuf<int> d_map, d_csr_ptr, d_csr_idx;
   DevBuf<double> d_V;
-  DevBuf<double> d_coefC, d_coefA, d_coefF1;
+  DevBuf<double> d_coefC, d_coefA, d_coefF1, d_keoc;
   DevBuf<int> d_order[kNumClasses];
   int64_t nunits[kNumClasses] = {};
   size_t lds_bytes64[kNumClasses] = {}, lds_bytes32[kNumClasses] = {};
@@ -109,30 +113,33 @@ struct evr_plan {
 
 namespace {
 
-template <typename T, int NT, int EPT>
-void set_lds(size_t bytes)
+template <typename T, int NT, int EPT, int NVW>
+hipError_t launch_one(const evr::TermParams<T> &prm, int64_t units, int nvec, size_t lds,
+                      hipStream_t s)
 {
-  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_term<T, NT, EPT>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_term<T, NT, EPT, NVW>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const int64_t nblk = units * ((nvec + NVW - 1) / NVW);
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL((evr::k_term<T, NT, EPT, NVW>), dim3((unsigned)nblk), dim3(NT), lds, s, prm);
+  return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_class(int c, const evr::TermParams<T> &prm, int64_t units, int nvec,
                         size_t lds, hipStream_t s)
 {
-  const int64_t nblk = units * nvec;
-  if (nblk <= 0) return hipSuccess;
-  dim3 grid((unsigned)nblk);
   switch (c) {
-    case 0: set_lds<T, 256, 1>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 1>), grid, dim3(256), lds, s, prm); break;
-    case 1: set_lds<T, 256, 2>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 2>), grid, dim3(256), lds, s, prm); break;
-    case 2: set_lds<T, 256, 4>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 4>), grid, dim3(256), lds, s, prm); break;
-    case 3: set_lds<T, 256, 8>(lds); hipLaunchKernelGGL((evr::k_term<T, 256, 8>), grid, dim3(256), lds, s, prm); break;
-    case 4: set_lds<T, 512, 8>(lds); hipLaunchKernelGGL((evr::k_term<T, 512, 8>), grid, dim3(512), lds, s, prm); break;
-    case 5: set_lds<T, 512, 16>(lds); hipLaunchKernelGGL((evr::k_term<T, 512, 16>), grid, dim3(512), lds, s, prm); break;
+    case 0: return launch_one<T, 256, 1, 4>(prm, units, nvec, lds, s);
+    case 1: return launch_one<T, 512, 1, 4>(prm, units, nvec, lds, s);
+    case 2: return launch_one<T, 1024, 1, 4>(prm, units, nvec, lds, s);
+    case 3: return launch_one<T, 1024, 1, 2>(prm, units, nvec, lds, s);
+    case 4: return launch_one<T, 1024, 1, 1>(prm, units, nvec, lds, s);
+    case 5: return launch_one<T, 1024, 2, 1>(prm, units, nvec, lds, s);
+    case 6: return launch_one<T, 1024, 4, 1>(prm, units, nvec, lds, s);
+    case 7: return launch_one<T, 1024, 8, 1>(prm, units, nvec, lds, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <typename T>
@@ -149,10 +156,12 @@ evr::TermParams<T> base_params(evr_plan *p, int mode, int nvec)
   prm.nvec = nvec;
   prm.terms = p->d_terms.ptr;
   prm.slots = p->d_slots.ptr;
+  prm.steps = p->d_steps.ptr;
   prm.map = p->d_map.ptr;
   prm.coefC = p->d_coefC.ptr;
   prm.coefA = p->d_coefA.ptr;
   prm.coefF1 = p->d_coefF1.ptr;
+  prm.keoc = p->d_keoc.ptr;
   return prm;
 }
 
@@ -474,19 +483,22 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     td.w = bd->weights[t];
     td.lds_mats = mats + (mats & 1);  // keep the data arrays 16-byte aligned
     int c = 0;
-    while (c < kNumClasses && (int64_t)kClasses[c].nt * kClasses[c].ept < nbt) ++c;
+    while (c < kNumClasses && kClasses[c].max_nbt() < nbt) ++c;
     if (c == kNumClasses) return fail(EVR_ERR_SIZE, "term basis larger than 8192 functions");
     cls[t] = c;
+    const int nvw = kClasses[c].nvw;
+    const size_t keo_bufs = kClasses[c].ept <= 2 ? 3 : 1;  // LDS buffers of the KEO path
     // largest slab (modes 0..m-1) that fits the LDS budget (double words)
     const size_t budget = kLdsLimit / 8;
-    const size_t base = td.lds_mats + nbt;
+    const size_t base = td.lds_mats + (size_t)nbt * nvw;
     int best = -1;
     int64_t nbA = 1, QA = 1;
     for (int m = 1; m <= D; ++m) {
       nbA *= nbd[m - 1];
       QA *= nqd[m - 1];
       const int64_t nbB = nbt / nbA;
-      const size_t work = std::max<size_t>(p->hasKeo ? nbt : 0, 2 * QA + 2 * nbB);
+      const size_t work = std::max<size_t>(p->hasKeo ? keo_bufs * nbt * nvw : 0,
+                                           2 * (size_t)QA * nvw + 2 * nbB);
       if (base + work <= budget) best = m;
     }
     if (best < 1) return fail(EVR_ERR_SIZE, "term does not fit in LDS");
@@ -496,7 +508,28 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     for (int d = 0; d < best; ++d) { td.nbA *= nbd[d]; td.QA *= nqd[d]; }
     td.nbB = (int)(nbt / td.nbA);
     td.QB = (int)(nqt / td.QA);
-    const size_t work = std::max<size_t>(p->hasKeo ? nbt : 0, 2 * (size_t)td.QA + 2 * td.nbB);
+    // slab contraction steps: B->G over modes 0..m-1, then G->B
+    td.step0 = (int)p->steps.size();
+    for (int d = 0; d < best; ++d) {
+      evr::StepDev sd{};
+      int n1 = 1, n3 = 1;
+      for (int e2 = 0; e2 < d; ++e2) n1 *= nqd[e2];
+      for (int e2 = d + 1; e2 < best; ++e2) n3 *= nbd[e2];
+      sd.n1 = n1; sd.n3 = n3; sd.nin = nbd[d]; sd.nout = nqd[d];
+      sd.mat = p->slots[td.slot[d]].offB;
+      p->steps.push_back(sd);
+    }
+    for (int d = 0; d < best; ++d) {
+      evr::StepDev sd{};
+      int n1 = 1, n3 = 1;
+      for (int e2 = 0; e2 < d; ++e2) n1 *= nbd[e2];
+      for (int e2 = d + 1; e2 < best; ++e2) n3 *= nqd[e2];
+      sd.n1 = n1; sd.n3 = n3; sd.nin = nqd[d]; sd.nout = nbd[d];
+      sd.mat = p->slots[td.slot[d]].offW;
+      p->steps.push_back(sd);
+    }
+    const size_t work = std::max<size_t>(p->hasKeo ? keo_bufs * nbt * nvw : 0,
+                                         2 * (size_t)td.QA * nvw + 2 * td.nbB);
     const size_t words = base + work;
     p->lds_bytes64[c] = std::max(p->lds_bytes64[c], words * 8);
     p->lds_bytes32[c] = std::max(p->lds_bytes32[c], words * 4);
@@ -558,6 +591,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   std::vector<float> bw32(bw.begin(), bw.end()), keo32(keo.begin(), keo.end());
   HIP_TRY(p->d_terms.upload(p->terms.data(), p->terms.size(), p->stream));
   HIP_TRY(p->d_slots.upload(p->slots.data(), p->slots.size(), p->stream));
+  HIP_TRY(p->d_steps.upload(p->steps.data(), p->steps.size(), p->stream));
   HIP_TRY(p->d_bw64.upload(bw.data(), bw.size(), p->stream));
   HIP_TRY(p->d_keo64.upload(keo.data(), keo.size(), p->stream));
   HIP_TRY(p->d_bw32.upload(bw32.data(), bw32.size(), p->stream));
@@ -569,6 +603,32 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   HIP_TRY(p->d_coefC.upload(cC.data(), cC.size(), p->stream));
   if (!cA.empty()) HIP_TRY(p->d_coefA.upload(cA.data(), cA.size(), p->stream));
   if (!cF1.empty()) HIP_TRY(p->d_coefF1.upload(cF1.data(), cF1.size(), p->stream));
+  {
+    // pre-zeroed strips of the basis-space KEO coefficients (kernel layout):
+    //  A[(j*Dp+k)*Dp+i] = a_ijk for i,j,k distinct; A[(j*Dp+j)*Dp+i] = c_ij, i != j
+    //  S[j*Dp+i] = a_ijj (i != j); N[j*Dp+i] = a_iji (j != i); O[k*Dp+i] = a_iik (k != i)
+    //  Cd[i] = c_ii; U[i] = a_iii; F[i] = F1_i
+    const int Dp = evr::kDp;
+    std::vector<double> kc((size_t)Dp * Dp * Dp + 3 * Dp * Dp + 3 * Dp, 0.0);
+    double *kA = kc.data(), *kS = kA + Dp * Dp * Dp, *kN = kS + Dp * Dp, *kO = kN + Dp * Dp,
+           *kCd = kO + Dp * Dp, *kU = kCd + Dp, *kF = kU + Dp;
+    auto a = [&](int i, int j, int k) { return cA.empty() ? 0.0 : cA[(i * D + j) * D + k]; };
+    for (int i = 0; i < D; ++i) {
+      kCd[i] = cC[i * D + i];
+      kU[i] = a(i, i, i);
+      kF[i] = cF1.empty() ? 0.0 : cF1[i];
+      for (int j = 0; j < D; ++j) {
+        if (j == i) continue;
+        kA[(j * Dp + j) * Dp + i] = cC[i * D + j];
+        kS[j * Dp + i] = a(i, j, j);
+        kN[j * Dp + i] = a(i, j, i);
+        kO[j * Dp + i] = a(i, i, j);
+        for (int k = 0; k < D; ++k)
+          if (k != i && k != j) kA[(j * Dp + k) * Dp + i] = a(i, j, k);
+      }
+    }
+    HIP_TRY(p->d_keoc.upload(kc.data(), kc.size(), p->stream));
+  }
   HIP_TRY(p->d_V.alloc(onq));
   HIP_TRY(p->d_X.alloc((size_t)p->nb_packed * max_nvec));
   HIP_TRY(p->d_R.alloc((size_t)onb * max_nvec));
@@ -589,10 +649,10 @@ int evr_plan_destroy(evr_plan *p)
 {
   if (!p) return EVR_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
-  p->d_terms.release(); p->d_slots.release(); p->d_bw64.release(); p->d_keo64.release();
+  p->d_terms.release(); p->d_slots.release(); p->d_steps.release(); p->d_bw64.release(); p->d_keo64.release();
   p->d_x.release(); p->d_bw32.release(); p->d_keo32.release(); p->d_V32.release();
   p->d_map.release(); p->d_csr_ptr.release(); p->d_csr_idx.release(); p->d_V.release();
-  p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release();
+  p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release(); p->d_keoc.release();
   for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
   p->d_X.release(); p->d_R.release(); p->d_psi.release(); p->d_hpsi.release(); p->d_G.release();
   for (auto &e : p->ev)
@@ -660,7 +720,7 @@ static void recount_flops(evr_plan *p)
   for (int c = 0; c < kNumClasses; ++c) p->flops_class[c] = 0;
   for (const evr::TermDev &td : p->terms) {
     int c = 0;
-    while ((int64_t)kClasses[c].nt * kClasses[c].ept < td.nbt) ++c;
+    while (kClasses[c].max_nbt() < td.nbt) ++c;
     p->flops_class[c] += term_flops(p, td);
   }
 }
@@ -683,7 +743,8 @@ int evr_plan_kernel_stats(evr_plan *p, int max, char *names, double *ms, double 
   for (int c = 0; c < kNumClasses && k < max; ++c) {
     if (p->nunits[c] == 0) continue;
     if (names)
-      snprintf(names + 64 * k, 64, "k_term<%d,%d>", kClasses[c].nt, kClasses[c].ept);
+      snprintf(names + 64 * k, 64, "k_term<%d,%d,%d>", kClasses[c].nt, kClasses[c].ept,
+               kClasses[c].nvw);
     if (ms) ms[k] = p->ms_acc[c];
     if (flops) flops[k] = p->flops_acc[c];
     if (launches) launches[k] = p->launches[c];

@@ -29,36 +29,120 @@ __device__ __forceinline__ T apply_mode(const T *__restrict__ M, const T *__rest
   return s;
 }
 
-// out[i1 + n1*(q + nout*i3)] = sum_k M[q + nout*k] in[i1 + n1*(k + nin*i3)]
-template <typename T, int NT>
-__device__ __forceinline__ void contract_lds(const T *__restrict__ in, T *__restrict__ out,
+// sum_b row[nbd*b] * col[b*st]: one row of a mode factor against a pencil.
+// nbd is wave-uniform; the common sizes are unrolled.
+template <typename T>
+__device__ __forceinline__ T apply_k(const T *__restrict__ row, const T *__restrict__ col, int nbd,
+                                     int st)
+{
+  if (nbd == 1) return row[0] * col[0];
+  if (nbd == 3) return fma(row[0], col[0], fma(row[3], col[st], row[6] * col[2 * st]));
+  if (nbd == 5)
+    return fma(row[0], col[0],
+               fma(row[5], col[st], fma(row[10], col[2 * st], fma(row[15], col[3 * st], row[20] * col[4 * st]))));
+  T s = T(0);
+  for (int b = 0; b < nbd; ++b) s = fma(row[nbd * b], col[b * st], s);
+  return s;
+}
+
+// Integer division of x < 2^24 by a wave-uniform d via the float reciprocal
+// (exact after one correction step).
+__device__ __forceinline__ int fdiv(int x, int d, float rd)
+{
+  int q = (int)((float)x * rd);
+  int r = x - q * d;
+  if (r < 0) --q;
+  else if (r >= d) ++q;
+  return q;
+}
+
+// out[(i1 + n1*(q + nout*i3))*NVW + v] = sum_k M[q + nout*k] in[(i1 + n1*(k + nin*i3))*NVW + v]
+// M is read from global memory at wave-uniform addresses (scalar loads, the
+// coefficients live in SGPRs); sizes known at compile time are fully unrolled
+// and each thread carries one pencil for the NVW vectors of the tile.
+template <typename T, int NT, int NVW, int NIN, int NOUT>
+__device__ __forceinline__ void contract_fix(const T *__restrict__ in, T *__restrict__ out,
+                                             const T *__restrict__ M, int n1, int n3)
+{
+  const int npen = n1 * n3;
+  const float rn1 = 1.0f / (float)n1;
+  T m[NIN * NOUT];
+#pragma unroll
+  for (int i = 0; i < NIN * NOUT; ++i) m[i] = M[i];
+  for (int pp = threadIdx.x; pp < npen; pp += NT) {
+    const int i3 = fdiv(pp, n1, rn1);
+    const int i1 = pp - i3 * n1;
+    const T *src = in + (i1 + n1 * NIN * i3) * NVW;
+    T *dst = out + (i1 + n1 * NOUT * i3) * NVW;
+    T v[NIN][NVW];
+#pragma unroll
+    for (int k = 0; k < NIN; ++k)
+#pragma unroll
+      for (int c = 0; c < NVW; ++c) v[k][c] = src[k * n1 * NVW + c];
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q)
+#pragma unroll
+      for (int c = 0; c < NVW; ++c) {
+        T s = m[q] * v[0][c];
+#pragma unroll
+        for (int k = 1; k < NIN; ++k) s = fma(m[q + NOUT * k], v[k][c], s);
+        dst[q * n1 * NVW + c] = s;
+      }
+  }
+}
+
+template <typename T, int NT, int NVW>
+__device__ __forceinline__ void contract_gen(const T *__restrict__ in, T *__restrict__ out,
                                              const T *__restrict__ M, int n1, int nin, int nout,
                                              int n3)
 {
-  const int npen = n1 * n3;
+  const int npen = n1 * n3 * NVW;
+  const int n1v = n1 * NVW;
+  const float rn1 = 1.0f / (float)n1v;
   for (int pp = threadIdx.x; pp < npen; pp += NT) {
-    const int i3 = pp / n1;
-    const int i1 = pp - i3 * n1;
+    const int i3 = fdiv(pp, n1v, rn1);
+    const int i1 = pp - i3 * n1v;  // (i1, v) combined
     T vin[kMaxN];
 #pragma unroll
     for (int k = 0; k < kMaxN; ++k)
-      if (k < nin) vin[k] = in[i1 + n1 * (k + nin * i3)];
+      if (k < nin) vin[k] = in[i1 + n1v * (k + nin * i3)];
     for (int q = 0; q < nout; ++q) {
       T s = T(0);
 #pragma unroll
       for (int k = 0; k < kMaxN; ++k)
-        if (k < nin) s += M[q + nout * k] * vin[k];
-      out[i1 + n1 * (q + nout * i3)] = s;
+        if (k < nin) s = fma(M[q + nout * k], vin[k], s);
+      out[i1 + n1v * (q + nout * i3)] = s;
     }
   }
 }
 
+template <typename T, int NT, int NVW>
+__device__ __forceinline__ void contract(const T *__restrict__ in, T *__restrict__ out,
+                                         const T *__restrict__ M, int n1, int nin, int nout, int n3)
+{
+#define EVR_CONTRACT_CASE(A, B) \
+  if (nin == A && nout == B) { contract_fix<T, NT, NVW, A, B>(in, out, M, n1, n3); return; }
+  EVR_CONTRACT_CASE(1, 2)
+  EVR_CONTRACT_CASE(2, 1)
+  EVR_CONTRACT_CASE(3, 4)
+  EVR_CONTRACT_CASE(4, 3)
+  EVR_CONTRACT_CASE(5, 6)
+  EVR_CONTRACT_CASE(6, 5)
+  EVR_CONTRACT_CASE(1, 1)
+  EVR_CONTRACT_CASE(3, 3)
+  EVR_CONTRACT_CASE(5, 5)
+#undef EVR_CONTRACT_CASE
+  contract_gen<T, NT, NVW>(in, out, M, n1, nin, nout, n3);
+}
+
 template <int EPT>
 struct IGroup {
-  static constexpr int value = EPT >= 16 ? 1 : (EPT >= 8 ? 2 : (EPT >= 4 ? 4 : (EPT >= 2 ? 6 : 12)));
+  static constexpr int value = EPT >= 8 ? 1 : (EPT >= 4 ? 3 : (EPT >= 2 ? 6 : 12));
 };
 
-template <typename T, int NT, int EPT>
+// One workgroup applies one Smolyak term to a tile of NVW vectors.  LDS
+// arrays are [element][vector] (vector fastest); a "unit" u = e*NVW + v.
+template <typename T, int NT, int EPT, int NVW>
 __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -66,14 +150,15 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   __shared__ int s_nb[kMaxDev], s_nq[kMaxDev], s_offB[kMaxDev], s_offW[kMaxDev],
       s_offK[kMaxDev], s_st[kMaxDev];
 
-  // XCD-aware decode: consecutive blocks go round-robin over the 8 XCDs;
-  // all vectors of one term land on the same XCD (shared V slab in L2) and
-  // the cost-sorted terms are dealt to the XCDs in turn.
+  // XCD-aware decode: consecutive blocks go round-robin over the 8 XCDs; all
+  // vector tiles of one term land on the same XCD (shared V slab and
+  // matrices in its L2) and the cost-sorted terms are dealt to the XCDs in turn.
+  const int nvt = (p.nvec + NVW - 1) / NVW;
   const int64_t b = blockIdx.x;
   const int xcd = (int)(b & 7);
   const int64_t slot = b >> 3;
-  const int v = (int)(slot % p.nvec);
-  const int64_t tr = (slot / p.nvec) * 8 + xcd;
+  const int vt = (int)(slot % nvt);
+  const int64_t tr = (slot / nvt) * 8 + xcd;
   if (tr >= p.nunits) return;
   const int t = p.order[tr];
   if (t < 0) return;
@@ -81,14 +166,17 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   const int D = p.D;
   const int tid = threadIdx.x;
   const int nbt = td.nbt;
+  const int nunit = nbt * NVW;
+  const int v0 = vt * NVW;
+  const int64_t nvec = p.nvec;
 
   if (tid == 0) {
-    int off = 0, st = 1;
+    int off = 0, st = NVW;
     for (int d = 0; d < D; ++d) {
       const SlotDev &s = p.slots[td.slot[d]];
       s_nb[d] = s.nb;
       s_nq[d] = s.nq;
-      s_st[d] = st;
+      s_st[d] = st;  // unit stride of mode d
       st *= s.nb;
       s_offB[d] = off;
       off += s.nq * s.nb;
@@ -113,13 +201,13 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     }
   }
   T *sx = smem + td.lds_mats;
-  T *work = sx + nbt;
-  const int64_t nvec = p.nvec;
+  T *work = sx + nunit;
   if (p.mode != kModeG2B) {
     const int *map = p.map + td.off_nb;
-    for (int e = tid; e < nbt; e += NT) {
+    for (int u = tid; u < nunit; u += NT) {
+      const int e = u / NVW, c = u - e * NVW;
       const int mp = map[e];
-      sx[e] = mp >= 0 ? p.X[(int64_t)mp * nvec + v] : T(0);
+      sx[u] = (mp >= 0 && v0 + c < nvec) ? p.X[(int64_t)mp * nvec + v0 + c] : T(0);
     }
   }
   __syncthreads();
@@ -130,9 +218,9 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   for (int r = 0; r < EPT; ++r) {
     yown[r] = T(0);
     dig[r] = 0;
-    int e = tid + r * NT;
-    if (e < nbt) {
-      int rem = e;
+    const int u = tid + r * NT;
+    if (u < nunit) {
+      int rem = u / NVW;
       for (int d = 0; d < D; ++d) {
         const int nbd = s_nb[d];
         const int q = rem / nbd;
@@ -143,6 +231,140 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   }
 
   // ---------------------------------------------------------------- KEO --
+  // Accumulators M_i, N_i, O_i (final factors P_i, R_i, T_i or T2_i) of a
+  // group of IG modes live in registers; the coefficient strips are
+  // pre-zeroed on the host (no branches, no index tests) and read with
+  // wave-uniform scalar loads.
+  if constexpr (EPT <= 2) {
+    if (p.mode == kModeHpsi && p.hasKeo) {
+      constexpr int IG = EPT == 1 ? 12 : 5;
+      T *zb = work;
+      T *zb2 = work + nunit;
+      T *zb3 = work + 2 * nunit;
+      const double *kA = p.keoc;                    // [(j*Dp + k)*Dp + i]
+      const double *kS = kA + kDp * kDp * kDp;      // [j*Dp + i]
+      const double *kN = kS + kDp * kDp;            // [j*Dp + i]
+      const double *kO = kN + kDp * kDp;            // [k*Dp + i]
+      const double *kCd = kO + kDp * kDp;           // [i]
+      const double *kU = kCd + kDp;                 // [i]
+      const double *kF = kU + kDp;                  // [i]
+      int base[EPT][kMaxDev];
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        const int u = tid + r * NT;
+        for (int d = 0; d < kMaxDev; ++d)
+          base[r][d] = d < D ? u - (int)((dig[r] >> (5 * d)) & 31) * s_st[d] : 0;
+      }
+      const int fdiag = p.diagT2 ? 6 : 2;
+      for (int i0 = 0; i0 < D; i0 += IG) {
+        T accM[IG][EPT], accN[IG][EPT], accO[IG][EPT];
+#pragma unroll
+        for (int ii = 0; ii < IG; ++ii)
+#pragma unroll
+          for (int r = 0; r < EPT; ++r) accM[ii][r] = accN[ii][r] = accO[ii][r] = T(0);
+        for (int j = 0; j < D; ++j) {
+          const int nbj = s_nb[j], stj = s_st[j], nnj = nbj * nbj;
+          const T *Kj = smem + s_offK[j];
+#pragma unroll
+          for (int r = 0; r < EPT; ++r) {
+            const int u = tid + r * NT;
+            if (u < nunit) {
+              const int bj = (int)((dig[r] >> (5 * j)) & 31);
+              zb[u] = apply_k(Kj + bj, sx + base[r][j], nbj, stj);  // Z_j = P_j x
+            }
+          }
+          __syncthreads();
+          {
+            T cz[IG], cs[IG], cn[IG], co[IG];
+#pragma unroll
+            for (int ii = 0; ii < IG; ++ii) {
+              cz[ii] = T(kA[(j * kDp + j) * kDp + i0 + ii]);
+              cs[ii] = T(kS[j * kDp + i0 + ii]);
+              cn[ii] = T(kN[j * kDp + i0 + ii]);
+              co[ii] = T(kO[j * kDp + i0 + ii]);
+            }
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) {
+              const int u = tid + r * NT;
+              if (u < nunit) {
+                const T zj = zb[u];
+#pragma unroll
+                for (int ii = 0; ii < IG; ++ii) accM[ii][r] = fma(cz[ii], zj, accM[ii][r]);
+                if (p.hasA) {
+                  const int bj = (int)((dig[r] >> (5 * j)) & 31);
+                  const T sxv = apply_k(Kj + 4 * nnj + bj, sx + base[r][j], nbj, stj);  // S_j x
+                  const T qxv = apply_k(Kj + nnj + bj, sx + base[r][j], nbj, stj);      // Q_j x
+#pragma unroll
+                  for (int ii = 0; ii < IG; ++ii) {
+                    accM[ii][r] = fma(cs[ii], sxv, accM[ii][r]);
+                    accN[ii][r] = fma(cn[ii], zj, accN[ii][r]);
+                    accO[ii][r] = fma(co[ii], qxv, accO[ii][r]);
+                  }
+                }
+              }
+            }
+          }
+          if (p.hasA) {
+            for (int k = 0; k < D; ++k) {
+              if (k == j) continue;
+              const int nbk = s_nb[k], stk = s_st[k];
+              const T *Qk = smem + s_offK[k] + nbk * nbk;
+              T ca[IG];
+#pragma unroll
+              for (int ii = 0; ii < IG; ++ii) ca[ii] = T(kA[(j * kDp + k) * kDp + i0 + ii]);
+#pragma unroll
+              for (int r = 0; r < EPT; ++r) {
+                const int u = tid + r * NT;
+                if (u < nunit) {
+                  const int bk = (int)((dig[r] >> (5 * k)) & 31);
+                  const T qz = apply_k(Qk + bk, zb + base[r][k], nbk, stk);  // Q_k Z_j
+#pragma unroll
+                  for (int ii = 0; ii < IG; ++ii) accM[ii][r] = fma(ca[ii], qz, accM[ii][r]);
+                }
+              }
+            }
+          }
+          __syncthreads();
+        }
+        // y += P_i M_i + R_i N_i + Tdiag_i (O_i + c_ii x) + a_iii U_i x + F1_i P_i x
+#pragma unroll
+        for (int ii = 0; ii < IG; ++ii) {
+          const int i = i0 + ii;
+          if (i >= D) break;
+          const T cd = T(kCd[i]), cu = T(kU[i]), cf = T(kF[i]);
+#pragma unroll
+          for (int r = 0; r < EPT; ++r) {
+            const int u = tid + r * NT;
+            if (u < nunit) {
+              zb[u] = accM[ii][r];
+              zb2[u] = accN[ii][r];
+              zb3[u] = fma(cd, sx[u], accO[ii][r]);
+            }
+          }
+          __syncthreads();
+          const int nbi = s_nb[i], sti = s_st[i], nn = nbi * nbi;
+          const T *Ki = smem + s_offK[i];
+#pragma unroll
+          for (int r = 0; r < EPT; ++r) {
+            const int u = tid + r * NT;
+            if (u < nunit) {
+              const int bi = (int)((dig[r] >> (5 * i)) & 31);
+              const int bo = base[r][i];
+              T yv = apply_k(Ki + bi, zb + bo, nbi, sti);                        // P_i M_i
+              yv += apply_k(Ki + fdiag * nn + bi, zb3 + bo, nbi, sti);            // T_i O_i
+              if (p.hasA) {
+                yv += apply_k(Ki + 3 * nn + bi, zb2 + bo, nbi, sti);              // R_i N_i
+                yv = fma(cu, apply_k(Ki + 5 * nn + bi, sx + bo, nbi, sti), yv);   // U_i x
+              }
+              if (p.hasF1) yv = fma(cf, apply_k(Ki + bi, sx + bo, nbi, sti), yv); // F1_i P_i x
+              yown[r] += yv;
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  } else {
   if (p.mode == kModeHpsi && p.hasKeo) {
     constexpr int IG = IGroup<EPT>::value;
     T *zb = work;
@@ -159,23 +381,23 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         const T *Kj = smem + s_offK[j];
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int e = tid + r * NT;
-          if (e < nbt) {
+          const int u = tid + r * NT;
+          if (u < nunit) {
             const int bj = (int)((dig[r] >> (5 * j)) & 31);
-            zb[e] = apply_mode(Kj + 0 * nbj * nbj, sx, e, bj, nbj, stj);  // P_j x
+            zb[u] = apply_mode(Kj + 0 * nbj * nbj, sx, u, bj, nbj, stj);  // P_j x
           }
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int e = tid + r * NT;
-          if (e < nbt) {
+          const int u = tid + r * NT;
+          if (u < nunit) {
             const int bj = (int)((dig[r] >> (5 * j)) & 31);
-            const T zj = zb[e];
+            const T zj = zb[u];
             T sxv = T(0), qxv = T(0);
             if (p.hasA) {
-              sxv = apply_mode(Kj + 4 * nbj * nbj, sx, e, bj, nbj, stj);  // S_j x
-              qxv = apply_mode(Kj + 1 * nbj * nbj, sx, e, bj, nbj, stj);  // Q_j x
+              sxv = apply_mode(Kj + 4 * nbj * nbj, sx, u, bj, nbj, stj);  // S_j x
+              qxv = apply_mode(Kj + 1 * nbj * nbj, sx, u, bj, nbj, stj);  // Q_j x
             }
 #pragma unroll
             for (int ii = 0; ii < IG; ++ii) {
@@ -194,7 +416,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
                 if (k == j) continue;
                 const int bk = (int)((dig[r] >> (5 * k)) & 31);
                 const int nbk = s_nb[k];
-                const T qz = apply_mode(smem + s_offK[k] + 1 * nbk * nbk, zb, e, bk, nbk, s_st[k]);
+                const T qz = apply_mode(smem + s_offK[k] + 1 * nbk * nbk, zb, u, bk, nbk, s_st[k]);
 #pragma unroll
                 for (int ii = 0; ii < IG; ++ii) {
                   const int i = i0 + ii;
@@ -216,48 +438,48 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         const int nn = nbi * nbi;
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int e = tid + r * NT;
-          if (e < nbt) zb[e] = accM[ii][r];
+          const int u = tid + r * NT;
+          if (u < nunit) zb[u] = accM[ii][r];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int e = tid + r * NT;
-          if (e < nbt)
-            yown[r] += apply_mode(Ki + 0 * nn, zb, e, (int)((dig[r] >> (5 * i)) & 31), nbi, sti);
+          const int u = tid + r * NT;
+          if (u < nunit)
+            yown[r] += apply_mode(Ki + 0 * nn, zb, u, (int)((dig[r] >> (5 * i)) & 31), nbi, sti);
         }
         __syncthreads();
         if (p.hasA) {
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
-            const int e = tid + r * NT;
-            if (e < nbt) zb[e] = accN[ii][r];
+            const int u = tid + r * NT;
+            if (u < nunit) zb[u] = accN[ii][r];
           }
           __syncthreads();
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
-            const int e = tid + r * NT;
-            if (e < nbt)
-              yown[r] += apply_mode(Ki + 3 * nn, zb, e, (int)((dig[r] >> (5 * i)) & 31), nbi, sti);
+            const int u = tid + r * NT;
+            if (u < nunit)
+              yown[r] += apply_mode(Ki + 3 * nn, zb, u, (int)((dig[r] >> (5 * i)) & 31), nbi, sti);
           }
           __syncthreads();
         }
         const T cii = T(cC[i * D + i]);
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int e = tid + r * NT;
-          if (e < nbt) zb[e] = accO[ii][r] + cii * sx[e];
+          const int u = tid + r * NT;
+          if (u < nunit) zb[u] = accO[ii][r] + cii * sx[u];
         }
         __syncthreads();
         const int fdiag = p.diagT2 ? 6 : 2;
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int e = tid + r * NT;
-          if (e < nbt) {
+          const int u = tid + r * NT;
+          if (u < nunit) {
             const int bi = (int)((dig[r] >> (5 * i)) & 31);
-            yown[r] += apply_mode(Ki + fdiag * nn, zb, e, bi, nbi, sti);
+            yown[r] += apply_mode(Ki + fdiag * nn, zb, u, bi, nbi, sti);
             if (p.hasA)
-              yown[r] += T(cA[(i * D + i) * D + i]) * apply_mode(Ki + 5 * nn, sx, e, bi, nbi, sti);
+              yown[r] += T(cA[(i * D + i) * D + i]) * apply_mode(Ki + 5 * nn, sx, u, bi, nbi, sti);
           }
         }
         __syncthreads();
@@ -266,16 +488,18 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     if (p.hasF1) {
 #pragma unroll
       for (int r = 0; r < EPT; ++r) {
-        const int e = tid + r * NT;
-        if (e < nbt)
+        const int u = tid + r * NT;
+        if (u < nunit)
           for (int i = 0; i < D; ++i) {
             const int nbi = s_nb[i];
             yown[r] += T(p.coefF1[i]) *
-                       apply_mode(smem + s_offK[i], sx, e, (int)((dig[r] >> (5 * i)) & 31), nbi,
+                       apply_mode(smem + s_offK[i], sx, u, (int)((dig[r] >> (5 * i)) & 31), nbi,
                                   s_st[i]);
           }
       }
     }
+  }
+
   }
 
   // ------------------------------------------------------------- V part --
@@ -283,19 +507,52 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   if (vpart) {
     const int m = td.m, QA = td.QA, nbA = td.nbA, nbB = td.nbB;
     T *slab0 = work;
-    T *slab1 = work + QA;
-    T *crow = work + 2 * QA;
+    T *slab1 = work + QA * NVW;
+    T *crow = work + 2 * QA * NVW;
     T *crowW = crow + nbB;
+    __shared__ int s_qd[kMaxDev];  // digits of the current outer grid point
+    if (tid < kMaxDev) s_qd[tid] = 0;
+    // per-unit offsets of the outer accumulation (no division inside leaves)
+    int aoff[EPT], boff[EPT];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      const int u = tid + r * NT;
+      const int e = u / NVW;
+      const int bB = e / nbA;
+      boff[r] = bB;
+      aoff[r] = u - bB * nbA * NVW;
+    }
+    // outer basis digits of the crow entry this thread computes (bB = tid)
+    uint64_t bdig = 0;
+    {
+      int rb = tid;
+      for (int d = m; d < D; ++d) {
+        const int nbd = s_nb[d];
+        const int q = rb / nbd;
+        bdig |= (uint64_t)(rb - q * nbd) << (5 * d);
+        rb = q;
+      }
+    }
+    const StepDev *stp = p.steps + td.step0;
+    __syncthreads();
     for (int qB = 0; qB < td.QB; ++qB) {
       for (int bB = tid; bB < nbB; bB += NT) {
+        uint64_t bd64 = bdig;
+        if (bB != tid) {  // more outer functions than threads: digits on the fly
+          bd64 = 0;
+          int rb = bB;
+          for (int d = m; d < D; ++d) {
+            const int nbd = s_nb[d];
+            const int q = rb / nbd;
+            bd64 |= (uint64_t)(rb - q * nbd) << (5 * d);
+            rb = q;
+          }
+        }
         T c = T(1), cw = T(1);
-        int rq = qB, rb = bB;
         for (int d = m; d < D; ++d) {
           const int nqd = s_nq[d], nbd = s_nb[d];
-          const int qd = rq % nqd;
-          rq /= nqd;
-          const int bd = rb % nbd;
-          rb /= nbd;
+          const int qd = s_qd[d];
+          const int bd = (int)((bd64 >> (5 * d)) & 31);
           c *= smem[s_offB[d] + qd + nqd * bd];
           cw *= smem[s_offW[d] + bd + nbd * qd];
         }
@@ -306,47 +563,55 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       T *cur = slab0, *nxt = slab1;
       const int64_t gbase = td.off_nq + (int64_t)QA * qB;
       if (p.mode != kModeG2B) {
-        for (int a = tid; a < nbA; a += NT) {
-          T s = T(0);
-          for (int bB = 0; bB < nbB; ++bB) s += crow[bB] * sx[a + nbA * bB];
-          cur[a] = s;
+        const int na = nbA * NVW;
+        for (int a2 = tid; a2 < na; a2 += NT) {
+          const T *xs = sx + a2;
+          T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+          int bB = 0;
+          for (; bB + 4 <= nbB; bB += 4) {
+            s0 = fma(crow[bB], xs[na * bB], s0);
+            s1 = fma(crow[bB + 1], xs[na * (bB + 1)], s1);
+            s2 = fma(crow[bB + 2], xs[na * (bB + 2)], s2);
+            s3 = fma(crow[bB + 3], xs[na * (bB + 3)], s3);
+          }
+          for (; bB < nbB; ++bB) s0 = fma(crow[bB], xs[na * bB], s0);
+          cur[a2] = (s0 + s1) + (s2 + s3);
         }
         __syncthreads();
-        int n1 = 1, n3 = nbA;
         for (int d = 0; d < m; ++d) {
-          const int nin = s_nb[d], nout = s_nq[d];
-          n3 /= nin;
-          contract_lds<T, NT>(cur, nxt, smem + s_offB[d], n1, nin, nout, n3);
-          n1 *= nout;
+          const StepDev st = stp[d];
+          contract<T, NT, NVW>(cur, nxt, p.bw + st.mat, st.n1, st.nin, st.nout, st.n3);
           T *tmp = cur; cur = nxt; nxt = tmp;
           __syncthreads();
         }
         if (p.mode == kModeB2G) {
-          for (int i = tid; i < QA; i += NT) p.G[gbase + i] = cur[i];
-          __syncthreads();
-          continue;
+          for (int i = tid; i < QA; i += NT) p.G[gbase + i] = cur[i * NVW];
+        } else {
+          for (int i2 = tid; i2 < QA * NVW; i2 += NT) cur[i2] *= p.V[gbase + i2 / NVW];
         }
-        for (int i = tid; i < QA; i += NT) cur[i] *= p.V[gbase + i];
         __syncthreads();
       } else {
-        for (int i = tid; i < QA; i += NT) cur[i] = p.Gin[gbase + i];
+        for (int i2 = tid; i2 < QA * NVW; i2 += NT) cur[i2] = p.Gin[gbase + i2 / NVW];
         __syncthreads();
       }
-      int n1 = 1, n3 = QA;
-      for (int d = 0; d < m; ++d) {
-        const int nin = s_nq[d], nout = s_nb[d];
-        n3 /= nin;
-        contract_lds<T, NT>(cur, nxt, smem + s_offW[d], n1, nin, nout, n3);
-        n1 *= nout;
-        T *tmp = cur; cur = nxt; nxt = tmp;
-        __syncthreads();
-      }
+      if (p.mode != kModeB2G) {
+        for (int d = 0; d < m; ++d) {
+          const StepDev st = stp[m + d];
+          contract<T, NT, NVW>(cur, nxt, p.bw + st.mat, st.n1, st.nin, st.nout, st.n3);
+          T *tmp = cur; cur = nxt; nxt = tmp;
+          __syncthreads();
+        }
 #pragma unroll
-      for (int r = 0; r < EPT; ++r) {
-        const int e = tid + r * NT;
-        if (e < nbt) {
-          const int bB = e / nbA;
-          yown[r] += crowW[bB] * cur[e - bB * nbA];
+        for (int r = 0; r < EPT; ++r) {
+          const int u = tid + r * NT;
+          if (u < nunit) yown[r] = fma(crowW[boff[r]], cur[aoff[r]], yown[r]);
+        }
+      }
+      // next outer grid point (digits of modes m..D-1, mode m fastest)
+      if (tid == 0) {
+        for (int d = m; d < D; ++d) {
+          if (++s_qd[d] < s_nq[d]) break;
+          s_qd[d] = 0;
         }
       }
       __syncthreads();
@@ -358,8 +623,11 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   if (p.mode == kModeG2B && fabs(td.w) < 1e-6) w = T(0);  // BGSG4:1004
 #pragma unroll
   for (int r = 0; r < EPT; ++r) {
-    const int e = tid + r * NT;
-    if (e < nbt) p.R[(td.off_nb + e) * nvec + v] = w * yown[r];
+    const int u = tid + r * NT;
+    if (u < nunit) {
+      const int e = u / NVW, c = u - e * NVW;
+      if (v0 + c < nvec) p.R[(td.off_nb + e) * nvec + v0 + c] = w * yown[r];
+    }
   }
 }
 
