@@ -1,0 +1,481 @@
+// Eigen-solver driver over the H|psi> engine: block Davidson with FP64
+// Rayleigh-Ritz, two-pass block orthogonalisation and optional FP32 H|psi>
+// refined in FP64 (see include/evr_hip.h, evr_davidson).
+//
+// Follows the reference's sub_propagation_Davidson
+// (Source_ElVibRot/sub_propagation/sub_module_Davidson.f90:54-748): one new
+// vector per unconverged wanted state and iteration (residual_max_nb,
+// DAV:1290-1588), convergence on |dE| < conv_ene and ||r|| < conv_resi
+// (DAV:407-446), collapse of the subspace on the current Ritz vectors when it
+// is full (DAV:516-606), NewVec_type 2 (Davidson correction r/(E - Hdiag),
+// DAV:1392-1413) when a diagonal is given, plain residuals (type 4) otherwise.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/evr_hip.h"
+#include "evr_internal.hpp"
+
+namespace {
+
+using evr::set_error;
+
+#define HTRY(expr)                                                                           \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return set_error(EVR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define BTRY(expr)                                                                           \
+  do {                                                                                       \
+    rocblas_status s_ = (expr);                                                              \
+    if (s_ != rocblas_status_success)                                                        \
+      return set_error(EVR_ERR_HIP, std::string(#expr) + ": rocblas status " + std::to_string((int)s_)); \
+  } while (0)
+#define RTRY(expr)                                                                           \
+  do {                                                                                       \
+    int r_ = (expr);                                                                         \
+    if (r_ != EVR_OK) return r_;                                                             \
+  } while (0)
+
+struct DBuf {
+  double *p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t k)
+  {
+    n = k;
+    return k ? hipMalloc((void **)&p, sizeof(double) * k) : hipSuccess;
+  }
+  ~DBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Blas {
+  rocblas_handle h = nullptr;
+  ~Blas()
+  {
+    if (h) rocblas_destroy_handle(h);
+  }
+};
+
+unsigned nblocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+__global__ void k_symmetrize(int m, int lda, double *S)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < (int64_t)m * m;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(g % m), j = (int)(g / m);
+    if (i < j) {
+      const double v = 0.5 * (S[i + (int64_t)lda * j] + S[j + (int64_t)lda * i]);
+      S[i + (int64_t)lda * j] = v;
+      S[j + (int64_t)lda * i] = v;
+    }
+  }
+}
+
+// R[:, c] = XH[:, sel[c]] - theta[sel[c]] XV[:, sel[c]]
+__global__ void k_residual(int64_t nb, int nc, const int *sel, const double *theta, const double *XH,
+                           const double *XV, double *R)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nb * nc;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(g / nb);
+    const int64_t i = g - (int64_t)c * nb;
+    const int j = sel[c];
+    R[g] = XH[i + nb * j] - theta[j] * XV[i + nb * j];
+  }
+}
+
+// squared column norms, one block per column, fixed reduction order
+__global__ __launch_bounds__(256) void k_colnorm2(int64_t nb, const double *A, double *out)
+{
+  __shared__ double red[256];
+  const double *a = A + nb * blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < nb; i += 256) s = fma(a[i], a[i], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Davidson correction t = r / (theta - diag), guarded near zero
+__global__ void k_precond(int64_t nb, int nc, const double *shift, const double *diag, double *T)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nb * nc;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(g / nb);
+    const int64_t i = g - (int64_t)c * nb;
+    double den = shift[c] - diag[i];
+    const double floor_ = 1e-8 * (fabs(shift[c]) + 1.0);
+    if (fabs(den) < floor_) den = den < 0 ? -floor_ : floor_;
+    T[g] /= den;
+  }
+}
+
+__global__ void k_unit_cols(int64_t nb, int nc, const int64_t *idx, double *V)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nb * nc;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(g / nb);
+    const int64_t i = g - (int64_t)c * nb;
+    V[g] = (i == idx[c]) ? 1.0 : 0.0;
+  }
+}
+
+// cyclic Jacobi on a small symmetric matrix (host), eigenvalues ascending
+void jacobi_sym(int n, std::vector<double> A, std::vector<double> &ev, std::vector<double> &V)
+{
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[i + (size_t)n * i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < n; ++p) {
+      diag += A[p + (size_t)n * p] * A[p + (size_t)n * p];
+      for (int q = p + 1; q < n; ++q) off += A[p + (size_t)n * q] * A[p + (size_t)n * q];
+    }
+    if (off <= 1e-30 * diag) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p + (size_t)n * q];
+        if (apq == 0.0) continue;
+        const double th = (A[q + (size_t)n * q] - A[p + (size_t)n * p]) / (2.0 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = A[k + (size_t)n * p], akq = A[k + (size_t)n * q];
+          A[k + (size_t)n * p] = c * akp - s * akq;
+          A[k + (size_t)n * q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[p + (size_t)n * k], aqk = A[q + (size_t)n * k];
+          A[p + (size_t)n * k] = c * apk - s * aqk;
+          A[q + (size_t)n * k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[k + (size_t)n * p], vkq = V[k + (size_t)n * q];
+          V[k + (size_t)n * p] = c * vkp - s * vkq;
+          V[k + (size_t)n * q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return A[a + (size_t)n * a] < A[b + (size_t)n * b]; });
+  ev.resize(n);
+  std::vector<double> W((size_t)n * n);
+  for (int k = 0; k < n; ++k) {
+    ev[k] = A[ord[k] + (size_t)n * ord[k]];
+    for (int i = 0; i < n; ++i) W[i + (size_t)n * k] = V[i + (size_t)n * ord[k]];
+  }
+  V.swap(W);
+}
+
+struct Dav {
+  evr_plan *plan = nullptr;
+  hipStream_t st = nullptr;
+  rocblas_handle bh = nullptr;
+  int64_t nb = 0;
+  int max_nvec = 0;
+  int64_t hpsi64 = 0, hpsi32 = 0;
+
+  // HX = H X for nc columns (chunks of the plan's max_nvec)
+  int apply(const double *X, double *HX, int nc, int flags)
+  {
+    for (int c0 = 0; c0 < nc; c0 += max_nvec) {
+      const int k = std::min(max_nvec, nc - c0);
+      RTRY(evr_oppsi(plan, k, X + nb * c0, HX + nb * c0, flags | EVR_DEVICE_PTRS));
+      if (flags & EVR_FP32) hpsi32 += k; else hpsi64 += k;
+    }
+    return EVR_OK;
+  }
+
+  // C (m x n) = A^T B with A nb x m, B nb x n
+  int gemm_tn(int m, int n, const double *A, const double *B, double *C, int ldc)
+  {
+    const double one = 1.0, zero = 0.0;
+    BTRY(rocblas_dgemm(bh, rocblas_operation_transpose, rocblas_operation_none, m, n, (int)nb, &one, A,
+                       (int)nb, B, (int)nb, &zero, C, ldc));
+    return EVR_OK;
+  }
+  // C (nb x n) = alpha A (nb x m) B (m x n) + beta C
+  int gemm_nn(int m, int n, double alpha, const double *A, const double *B, int ldb, double beta,
+              double *C)
+  {
+    BTRY(rocblas_dgemm(bh, rocblas_operation_none, rocblas_operation_none, (int)nb, n, m, &alpha, A,
+                       (int)nb, B, ldb, &beta, C, (int)nb));
+    return EVR_OK;
+  }
+
+  // Orthonormalise the nc columns of T against V[:, :m] and among themselves
+  // (classical Gram-Schmidt twice, then SVQB twice dropping dependent
+  // directions).  Returns the number of columns kept (compacted to the front).
+  int orthonormalise(const double *V, int m, double *T, int nc, double *tmpC, double *tmpT,
+                     int &kept)
+  {
+    for (int pass = 0; pass < 2 && m > 0; ++pass) {
+      RTRY(gemm_tn(m, nc, V, T, tmpC, m));
+      RTRY(gemm_nn(m, nc, -1.0, V, tmpC, m, 1.0, T));
+    }
+    kept = nc;
+    for (int pass = 0; pass < 2 && kept > 0; ++pass) {
+      RTRY(gemm_tn(kept, kept, T, T, tmpC, kept));
+      std::vector<double> G((size_t)kept * kept), ev, U;
+      HTRY(hipMemcpyAsync(G.data(), tmpC, sizeof(double) * kept * kept, hipMemcpyDeviceToHost, st));
+      HTRY(hipStreamSynchronize(st));
+      // SVQB: scale to unit diagonal, keep the well-conditioned directions
+      std::vector<double> sc(kept);
+      for (int i = 0; i < kept; ++i) {
+        const double g = G[i + (size_t)kept * i];
+        sc[i] = g > 1e-300 ? 1.0 / std::sqrt(g) : 0.0;
+      }
+      for (int i = 0; i < kept; ++i)
+        for (int j = 0; j < kept; ++j) G[i + (size_t)kept * j] *= sc[i] * sc[j];
+      jacobi_sym(kept, G, ev, U);
+      std::vector<double> M;
+      int nk = 0;
+      for (int k = kept - 1; k >= 0; --k) {
+        if (ev[k] < 1e-10) continue;
+        for (int i = 0; i < kept; ++i) M.push_back(sc[i] * U[i + (size_t)kept * k] / std::sqrt(ev[k]));
+        ++nk;
+      }
+      if (nk == 0) { kept = 0; break; }
+      HTRY(hipMemcpyAsync(tmpC, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice, st));
+      RTRY(gemm_nn(kept, nk, 1.0, T, tmpC, kept, 0.0, tmpT));
+      HTRY(hipMemcpyAsync(T, tmpT, sizeof(double) * nb * nk, hipMemcpyDeviceToDevice, st));
+      kept = nk;
+    }
+    return EVR_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *evals, double *evecs,
+                            evr_davidson_info *info)
+{
+  if (!plan || !o || !evals) return set_error(EVR_ERR_ARG, "NULL argument");
+  const auto t_start = std::chrono::steady_clock::now();
+  int64_t sz[7];
+  RTRY(evr_plan_sizes(plan, sz));
+  Dav d;
+  d.plan = plan;
+  d.nb = sz[0];
+  d.st = (hipStream_t)evr_plan_stream(plan);
+  const int nev = o->nev;
+  const int blk = o->block > 0 ? o->block : std::min(nev, 16);
+  if (nev < 1 || nev > d.nb) return set_error(EVR_ERR_ARG, "nev out of range");
+  int mmax = o->max_sub > 0 ? o->max_sub : std::max(3 * nev, nev + 4 * blk);
+  mmax = (int)std::min<int64_t>(mmax, d.nb);
+  if (mmax < nev + 1 && mmax < d.nb) return set_error(EVR_ERR_ARG, "max_sub must exceed nev");
+  const int max_it = o->max_it > 0 ? o->max_it : 200;
+  d.max_nvec = (int)o->max_nvec_hint;
+  if (d.max_nvec <= 0) d.max_nvec = blk;
+  Blas blas;
+  BTRY(rocblas_create_handle(&blas.h));
+  BTRY(rocblas_set_stream(blas.h, d.st));
+  BTRY(rocblas_set_pointer_mode(blas.h, rocblas_pointer_mode_host));
+  d.bh = blas.h;
+
+  const int64_t nb = d.nb;
+  const int kmax = std::min(mmax, nev + blk);
+  DBuf V, HV, XV, XH, T, HT, S, theta, E, tmpC, tmpT, nrm, dsel, dshift, diag, didx;
+  int *dinfo = nullptr;
+  HTRY(V.alloc(nb * mmax));
+  HTRY(HV.alloc(nb * mmax));
+  HTRY(XV.alloc(nb * kmax));
+  HTRY(XH.alloc(nb * kmax));
+  HTRY(T.alloc(nb * blk));
+  HTRY(HT.alloc(nb * blk));
+  HTRY(tmpT.alloc(nb * std::max(blk, 1)));
+  HTRY(S.alloc((size_t)mmax * mmax));
+  HTRY(theta.alloc(mmax));
+  HTRY(E.alloc(mmax));
+  HTRY(tmpC.alloc((size_t)mmax * std::max(blk, kmax)));
+  HTRY(nrm.alloc(blk));
+  HTRY(dsel.alloc(blk));
+  HTRY(dshift.alloc(blk));
+  HTRY(hipMalloc((void **)&dinfo, sizeof(int)));
+  struct IntFree { int *p; ~IntFree() { if (p) (void)hipFree(p); } } dinfo_free{dinfo};
+
+  // preconditioner diagonal
+  const bool use_diag = o->precond != 0;
+  std::vector<double> hdiag;
+  if (use_diag || !o->guess) {
+    HTRY(diag.alloc(nb));
+    if (o->diag) {
+      HTRY(hipMemcpyAsync(diag.p, o->diag, sizeof(double) * nb, hipMemcpyHostToDevice, d.st));
+      hdiag.assign(o->diag, o->diag + nb);
+    } else {
+      RTRY(evr_plan_diagonal(plan, diag.p, EVR_DEVICE_PTRS));
+      hdiag.resize(nb);
+      HTRY(hipMemcpyAsync(hdiag.data(), diag.p, sizeof(double) * nb, hipMemcpyDeviceToHost, d.st));
+      HTRY(hipStreamSynchronize(d.st));
+    }
+  }
+
+  // initial subspace: given vectors, else unit vectors on the lowest diagonal entries
+  int m = 0;
+  int k0 = o->guess ? std::min<int>(o->nguess, mmax) : std::min(mmax, std::max(nev, blk));
+  if (o->guess) {
+    HTRY(hipMemcpyAsync(V.p, o->guess, sizeof(double) * nb * k0, hipMemcpyHostToDevice, d.st));
+  } else {
+    std::vector<int64_t> idx(nb);
+    for (int64_t i = 0; i < nb; ++i) idx[i] = i;
+    std::partial_sort(idx.begin(), idx.begin() + k0, idx.end(),
+                      [&](int64_t a, int64_t b) { return hdiag[a] < hdiag[b] || (hdiag[a] == hdiag[b] && a < b); });
+    DBuf didx2;
+    HTRY(didx2.alloc(k0));
+    HTRY(hipMemcpyAsync(didx2.p, idx.data(), sizeof(int64_t) * k0, hipMemcpyHostToDevice, d.st));
+    hipLaunchKernelGGL(k_unit_cols, dim3(nblocks(nb * k0)), dim3(256), 0, d.st, nb, k0,
+                       (const int64_t *)didx2.p, V.p);
+    HTRY(hipGetLastError());
+    HTRY(hipStreamSynchronize(d.st));
+  }
+  {
+    // orthonormalise the start vectors in chunks of blk
+    int done = 0;
+    for (int c0 = 0; c0 < k0; c0 += blk) {
+      const int nc = std::min(blk, k0 - c0);
+      HTRY(hipMemcpyAsync(T.p, V.p + nb * c0, sizeof(double) * nb * nc, hipMemcpyDeviceToDevice, d.st));
+      int kept = 0;
+      RTRY(d.orthonormalise(V.p, done, T.p, nc, tmpC.p, tmpT.p, kept));
+      HTRY(hipMemcpyAsync(V.p + nb * done, T.p, sizeof(double) * nb * kept, hipMemcpyDeviceToDevice, d.st));
+      done += kept;
+    }
+    m = done;
+  }
+  int flags = (o->precision == EVR_FP32) ? EVR_FP32 : EVR_FP64;
+  bool refined = flags == EVR_FP64;
+  RTRY(d.apply(V.p, HV.p, m, flags));
+
+  std::vector<double> th(mmax), th_old(nev, 1e300), rn(nev, 1e300);
+  std::vector<int> conv(nev, 0);
+  int it = 0, nconv = 0;
+  double maxres = 0.0;
+  bool done = false;
+  for (it = 1; it <= max_it; ++it) {
+    // Rayleigh-Ritz in FP64
+    RTRY(d.gemm_tn(m, m, V.p, HV.p, S.p, m));
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)m * m)), dim3(256), 0, d.st, m, m, S.p);
+    HTRY(hipGetLastError());
+    BTRY(rocsolver_dsyevd(d.bh, rocblas_evect_original, rocblas_fill_upper, m, S.p, m, theta.p, E.p,
+                          dinfo));
+    HTRY(hipMemcpyAsync(th.data(), theta.p, sizeof(double) * m, hipMemcpyDeviceToHost, d.st));
+    int hinfo = 0;
+    HTRY(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, d.st));
+    HTRY(hipStreamSynchronize(d.st));
+    if (hinfo != 0) return set_error(EVR_ERR_STATE, "subspace eigensolver failed");
+    const int kr = std::min(m, kmax);
+    const int ne = std::min(nev, m);
+    RTRY(d.gemm_nn(m, kr, 1.0, V.p, S.p, m, 0.0, XV.p));
+    RTRY(d.gemm_nn(m, kr, 1.0, HV.p, S.p, m, 0.0, XH.p));
+    // residuals of the wanted states, in chunks of blk
+    maxres = 0.0;
+    for (int j0 = 0; j0 < ne; j0 += blk) {
+      const int nc = std::min(blk, ne - j0);
+      std::vector<int> sel(nc);
+      for (int c = 0; c < nc; ++c) sel[c] = j0 + c;
+      HTRY(hipMemcpyAsync(dsel.p, sel.data(), sizeof(int) * nc, hipMemcpyHostToDevice, d.st));
+      hipLaunchKernelGGL(k_residual, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc,
+                         (const int *)dsel.p, theta.p, XH.p, XV.p, T.p);
+      hipLaunchKernelGGL(k_colnorm2, dim3(nc), dim3(256), 0, d.st, nb, T.p, nrm.p);
+      HTRY(hipGetLastError());
+      std::vector<double> r2(nc);
+      HTRY(hipMemcpyAsync(r2.data(), nrm.p, sizeof(double) * nc, hipMemcpyDeviceToHost, d.st));
+      HTRY(hipStreamSynchronize(d.st));
+      for (int c = 0; c < nc; ++c) {
+        rn[j0 + c] = std::sqrt(r2[c]);
+        maxres = std::max(maxres, rn[j0 + c]);
+      }
+    }
+    nconv = 0;
+    for (int j = 0; j < ne; ++j) {
+      conv[j] = (rn[j] < o->conv_resi && std::fabs(th[j] - th_old[j]) < o->conv_ene) ? 1 : 0;
+      nconv += conv[j];
+      th_old[j] = th[j];
+    }
+    if (o->verbose) {
+      fprintf(stderr, "davidson it %d m %d conv %d/%d max|r| %.3e E0 %.10f %s\n", it, m, nconv, nev,
+              maxres, th[0], refined ? "fp64" : "fp32");
+      if (o->verbose > 1)
+        for (int j = 0; j < ne; ++j) fprintf(stderr, "   %3d %.10f %.3e\n", j, th[j], rn[j]);
+    }
+    if (!refined && (maxres < o->refine_resi || nconv == ne)) {
+      // FP64 refinement: rebuild H V in FP64, continue with FP64 H|psi>
+      flags = EVR_FP64;
+      refined = true;
+      RTRY(d.apply(V.p, HV.p, m, flags));
+      for (int j = 0; j < nev; ++j) th_old[j] = 1e300;
+      continue;
+    }
+    if (nconv == ne && ne == nev && refined) { done = true; break; }
+    // new vectors: one per unconverged wanted state (DAV:1290-1588); when there
+    // are more than the block holds, the states with the largest residuals
+    std::vector<int> sel;
+    for (int j = 0; j < ne; ++j)
+      if (!conv[j]) sel.push_back(j);
+    if ((int)sel.size() > blk) {
+      std::stable_sort(sel.begin(), sel.end(), [&](int a, int b) { return rn[a] > rn[b]; });
+      sel.resize(blk);
+      std::sort(sel.begin(), sel.end());
+    }
+    if (sel.empty())  // converged in FP32 before the switch: force the refinement
+      for (int j = 0; j < ne && (int)sel.size() < blk; ++j) sel.push_back(j);
+    int nc = (int)sel.size();
+    if (m + nc > mmax) {
+      // collapse on the current Ritz vectors (XV, XH hold kr of them)
+      const int keep = std::min(kr, mmax - nc);
+      HTRY(hipMemcpyAsync(V.p, XV.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
+      HTRY(hipMemcpyAsync(HV.p, XH.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
+      m = keep;
+    }
+    HTRY(hipMemcpyAsync(dsel.p, sel.data(), sizeof(int) * nc, hipMemcpyHostToDevice, d.st));
+    hipLaunchKernelGGL(k_residual, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc,
+                       (const int *)dsel.p, theta.p, XH.p, XV.p, T.p);
+    HTRY(hipGetLastError());
+    if (use_diag) {
+      std::vector<double> sh(nc);
+      for (int c = 0; c < nc; ++c) sh[c] = th[sel[c]];
+      HTRY(hipMemcpyAsync(dshift.p, sh.data(), sizeof(double) * nc, hipMemcpyHostToDevice, d.st));
+      hipLaunchKernelGGL(k_precond, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc, dshift.p,
+                         diag.p, T.p);
+      HTRY(hipGetLastError());
+    }
+    int kept = 0;
+    RTRY(d.orthonormalise(V.p, m, T.p, nc, tmpC.p, tmpT.p, kept));
+    if (o->verbose) fprintf(stderr, "  new vectors %d kept %d (m %d)\n", nc, kept, m);
+    if (kept == 0) break;  // the subspace cannot grow (DAV:1534 drops tiny vectors)
+    RTRY(d.apply(T.p, HT.p, kept, flags));
+    HTRY(hipMemcpyAsync(V.p + nb * m, T.p, sizeof(double) * nb * kept, hipMemcpyDeviceToDevice, d.st));
+    HTRY(hipMemcpyAsync(HV.p + nb * m, HT.p, sizeof(double) * nb * kept, hipMemcpyDeviceToDevice, d.st));
+    m += kept;
+  }
+  // results: Ritz values and vectors of the last Rayleigh-Ritz step
+  for (int j = 0; j < nev; ++j) evals[j] = j < m ? th[j] : NAN;
+  if (evecs)
+    HTRY(hipMemcpyAsync(evecs, XV.p, sizeof(double) * nb * std::min(nev, m), hipMemcpyDeviceToHost, d.st));
+  HTRY(hipStreamSynchronize(d.st));
+  if (info) {
+    info->iterations = std::min(it, max_it);
+    info->converged = done ? 1 : 0;
+    info->nconv = nconv;
+    info->hpsi_fp64 = d.hpsi64;
+    info->hpsi_fp32 = d.hpsi32;
+    info->max_resi = maxres;
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return EVR_OK;
+}

@@ -48,6 +48,7 @@ struct TermParams {
   int D;
   int mode;
   int hasV, hasKeo, hasA, hasF1, diagT2;
+  int diagKeo;                // G2B mode: add the kinetic diagonal (plan diagonal)
   int nvec;                   // vectors in this call (blockIdx decode)
   int64_t nunits;             // padded count of terms in this launch (multiple of 8)
   const int *order;           // term ids for this launch (-1 = padding)

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/evr_hip.h"
+#include "evr_internal.hpp"
 #include "evr_kernels.hpp"
 #include "evr_setup.hpp"
 #include "evr_term.hip"
@@ -70,6 +71,8 @@ constexpr size_t kLdsLimit = 160 * 1024 - 1024;
 
 }  // namespace
 
+int evr::set_error(int code, const std::string &msg) { return fail(code, msg); }
+
 struct evr_desc_owner {
   evr_basis_desc desc;
   evr::BasisTables tab;
@@ -90,7 +93,7 @@ struct evr_plan {
   DevBuf<evr::SlotDev> d_slots;
   std::vector<evr::StepDev> steps;
   DevBuf<evr::StepDev> d_steps;
-  DevBuf<double> d_bw64, d_keo64, d_x;
+  DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag;
   DevBuf<float> d_bw32, d_keo32, d_V32;
   DevBuf<int> d_map, d_csr_ptr, d_csr_idx;
   DevBuf<double> d_V;
@@ -206,11 +209,12 @@ double term_flops(const evr_plan *p, const evr::TermDev &td)
 // Apply every term class for `nvec` vectors held in X ([nb_packed][nvec]).
 template <typename T>
 int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *Gin, const T *bw,
-              const T *keo, const T *V)
+              const T *keo, const T *V, int diagKeo = 0)
 {
   for (int c = 0; c < kNumClasses; ++c) {
     if (p->nunits[c] == 0) continue;
     evr::TermParams<T> prm = base_params<T>(p, mode, nvec);
+    prm.diagKeo = diagKeo;
     prm.nunits = p->nunits[c];
     prm.order = p->d_order[c].ptr;
     prm.bw = bw;
@@ -386,6 +390,20 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   if ((op->type == 10 && !op->G0) || (op->type == 1 && !op->F2))
     return fail(EVR_ERR_ARG, "type 10 needs G0, type 1 needs F2");
   if (max_nvec < 1) return fail(EVR_ERR_ARG, "max_nvec must be >= 1");
+  if (op->hermitian && op->type == 10) {
+    const int D0 = bd->D;
+    for (int i = 0; i < D0; ++i)
+      for (int j = 0; j < D0; ++j) {
+        if (op->G0[i + D0 * j] != op->G0[j + D0 * i])
+          return fail(EVR_ERR_ARG, "hermitian form needs a symmetric G0");
+        if (op->Ga)
+          for (int k = 0; k < D0; ++k) {
+            const double a = op->Ga[i + D0 * (j + D0 * k)];
+            if (a != op->Ga[j + D0 * (i + D0 * k)] || a != op->Ga[k + D0 * (j + D0 * i)])
+              return fail(EVR_ERR_ARG, "hermitian form needs a fully symmetric Ga");
+          }
+      }
+  }
   std::unique_ptr<evr_plan> p(new evr_plan());
   const int D = bd->D, nlev = bd->nlev;
   p->D = D;
@@ -400,7 +418,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   const int ns = D * nlev;
   std::vector<int64_t> ox, oB, oD;
   slot_offsets(bd, ox, oB, oD);
-  std::vector<double> bw, keo, xs;
+  std::vector<double> bw, keo, xs, bwdiag;
   p->slots.resize(ns);
   for (int s = 0; s < ns; ++s) {
     const int nb = bd->nb[s], nq = bd->nq[s];
@@ -413,6 +431,11 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     bw.insert(bw.end(), bd->B + oB[s], bd->B + oB[s + 1]);
     sd.offW = (int64_t)bw.size();
     bw.insert(bw.end(), bd->W + oB[s], bd->W + oB[s + 1]);
+    // pool for the plan diagonal: B, and W o B^T in place of W
+    bwdiag.insert(bwdiag.end(), bd->B + oB[s], bd->B + oB[s + 1]);
+    for (int q = 0; q < nq; ++q)
+      for (int b = 0; b < nb; ++b)
+        bwdiag.push_back(bd->W[oB[s] + b + (int64_t)nb * q] * bd->B[oB[s] + q + (int64_t)nq * b]);
     sd.offX = (int64_t)xs.size();
     xs.insert(xs.end(), bd->x + ox[s], bd->x + ox[s + 1]);
     evr::Slot1D s1;
@@ -425,6 +448,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     s1.D2.assign(bd->D2 + oD[s], bd->D2 + oD[s + 1]);
     std::vector<double> f;
     evr::keo_factors(s1, f);
+    if (op->hermitian) evr::symmetrize_factors(nb, f);
     sd.offK = (int64_t)keo.size();
     keo.insert(keo.end(), f.begin(), f.end());
   }
@@ -448,7 +472,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   } else if (op->type == 1) {
     for (int i = 0; i < D; ++i)
       for (int j = 0; j < D; ++j) cC[i * D + j] = op->F2[i + D * j];
-    if (op->F1) {
+    if (op->F1 && !op->hermitian) {  // the symmetric part of F1_i d_i is zero
       p->hasF1 = true;
       cF1.assign(op->F1, op->F1 + D);
     }
@@ -593,6 +617,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   HIP_TRY(p->d_slots.upload(p->slots.data(), p->slots.size(), p->stream));
   HIP_TRY(p->d_steps.upload(p->steps.data(), p->steps.size(), p->stream));
   HIP_TRY(p->d_bw64.upload(bw.data(), bw.size(), p->stream));
+  HIP_TRY(p->d_bwdiag.upload(bwdiag.data(), bwdiag.size(), p->stream));
   HIP_TRY(p->d_keo64.upload(keo.data(), keo.size(), p->stream));
   HIP_TRY(p->d_bw32.upload(bw32.data(), bw32.size(), p->stream));
   HIP_TRY(p->d_keo32.upload(keo32.data(), keo32.size(), p->stream));
@@ -650,7 +675,7 @@ int evr_plan_destroy(evr_plan *p)
   if (!p) return EVR_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   p->d_terms.release(); p->d_slots.release(); p->d_steps.release(); p->d_bw64.release(); p->d_keo64.release();
-  p->d_x.release(); p->d_bw32.release(); p->d_keo32.release(); p->d_V32.release();
+  p->d_x.release(); p->d_bwdiag.release(); p->d_bw32.release(); p->d_keo32.release(); p->d_V32.release();
   p->d_map.release(); p->d_csr_ptr.release(); p->d_csr_idx.release(); p->d_V.release();
   p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release(); p->d_keoc.release();
   for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
@@ -832,6 +857,30 @@ int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG)
   HIP_TRY(hipStreamSynchronize(p->stream));
   if (p->dp_order)
     for (int64_t i = 0; i < p->sum_nq; ++i) psiG[p->dp_perm[i]] = tmp[i];
+  return EVR_OK;
+}
+
+int evr_plan_diagonal(evr_plan *p, double *diag, int flags)
+{
+  if (!p || !diag) return fail(EVR_ERR_ARG, "NULL argument");
+  // V part: GridTOBasis of V with W o B^T as the grid->basis matrices
+  const double *Gin = p->hasV ? p->d_V.ptr : nullptr;
+  if (!Gin) {
+    if (!p->d_G.ptr) HIP_TRY(p->d_G.alloc(p->sum_nq));
+    HIP_TRY(hipMemsetAsync(p->d_G.ptr, 0, sizeof(double) * p->sum_nq, p->stream));
+    Gin = p->d_G.ptr;
+  }
+  int rc = run_terms<double>(p, evr::kModeG2B, 1, nullptr, p->d_R.ptr, nullptr, Gin,
+                             p->d_bwdiag.ptr, p->d_keo64.ptr, p->d_V.ptr, 1);
+  if (rc) return rc;
+  double *dst = (flags & EVR_DEVICE_PTRS) ? diag : p->d_hpsi.ptr;
+  hipLaunchKernelGGL(evr::k_reduce<double>, dim3(blocks_for(p->nb_packed)), dim3(256), 0, p->stream,
+                     p->d_R.ptr, p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, 1, dst);
+  HIP_TRY(hipGetLastError());
+  if (!(flags & EVR_DEVICE_PTRS))
+    HIP_TRY(hipMemcpyAsync(diag, dst, sizeof(double) * p->nb_packed, hipMemcpyDeviceToHost,
+                           p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
   return EVR_OK;
 }
 

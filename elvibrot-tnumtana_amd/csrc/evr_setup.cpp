@@ -186,6 +186,35 @@ void keo_factors(const Slot1D &s, std::vector<double> &out)
       }
 }
 
+void symmetrize_factors(int nb, std::vector<double> &f)
+{
+  const size_t nn = (size_t)nb * nb;
+  auto M = [&](int fac, int r, int c) -> double & { return f[fac * nn + r + (size_t)nb * c]; };
+  std::vector<double> RS(nn);
+  for (int r = 0; r < nb; ++r)
+    for (int c = 0; c < nb; ++c) RS[r + (size_t)nb * c] = M(kR, r, c) + M(kS, r, c);
+  for (int r = 0; r < nb; ++r)
+    for (int c = r; c < nb; ++c) {
+      auto sym = [&](int fac) {
+        double v = 0.5 * (M(fac, r, c) + M(fac, c, r));
+        M(fac, r, c) = M(fac, c, r) = v;
+      };
+      auto anti = [&](int fac) {
+        double v = 0.5 * (M(fac, r, c) - M(fac, c, r));
+        M(fac, r, c) = v;
+        M(fac, c, r) = -v;
+      };
+      anti(kP);
+      sym(kQ);
+      sym(kT);
+      sym(kU);
+      sym(kT2);
+      double a = 0.25 * (RS[r + (size_t)nb * c] - RS[c + (size_t)nb * r]);
+      M(kR, r, c) = M(kS, r, c) = a;
+      M(kR, c, r) = M(kS, c, r) = -a;
+    }
+}
+
 static void append_slot(BasisTables &t, const Slot1D &s)
 {
   t.x.insert(t.x.end(), s.x.begin(), s.x.end());

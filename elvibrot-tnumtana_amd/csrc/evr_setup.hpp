@@ -34,6 +34,11 @@ void hermite_slot(int nb, int nq, Slot1D &s);
 //   Qg = diag-free grid coordinate factor W diag(x) B (same as Q).
 enum KeoFactor { kP = 0, kQ, kT, kR, kS, kU, kT2, kNumFactors };
 void keo_factors(const Slot1D &s, std::vector<double> &out /* kNumFactors*nb*nb */);
+// Factors of the symmetric part (K + K^T)/2 of the kinetic operator:
+// P -> antisym(P), Q -> sym(Q), T -> sym(T), U -> sym(U), T2 -> sym(T2),
+// R = S -> antisym(R + S)/2 (the k=i and k=j terms of a symmetric a_ijk pair
+// up as (R + S)_i x P_j).
+void symmetrize_factors(int nb, std::vector<double> &f);
 
 // Fully built basis (owned arrays behind evr_basis_desc).
 struct BasisTables {

@@ -619,8 +619,24 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   }
 
   if (p.mode == kModeB2G) return;
+  if (p.mode == kModeG2B && p.diagKeo && p.hasKeo) {
+    // diagonal of the kinetic part: sum_i c_ii Tdiag_i[b_i,b_i] + a_iii U_i[b_i,b_i]
+    // (the other factor products vanish on the diagonal: P and Q have a zero diagonal)
+    const double *kCd = p.keoc + kDp * kDp * kDp + 3 * kDp * kDp, *kU = kCd + kDp;
+    const int fdiag = p.diagT2 ? 6 : 2;
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      const int u = tid + r * NT;
+      if (u < nunit)
+        for (int i = 0; i < D; ++i) {
+          const int nbi = s_nb[i], bi = (int)((dig[r] >> (5 * i)) & 31), nn = nbi * nbi;
+          const T *Ki = smem + s_offK[i];
+          yown[r] += T(kCd[i]) * Ki[fdiag * nn + bi + nbi * bi] + T(kU[i]) * Ki[5 * nn + bi + nbi * bi];
+        }
+    }
+  }
   T w = T(td.w);
-  if (p.mode == kModeG2B && fabs(td.w) < 1e-6) w = T(0);  // BGSG4:1004
+  if (p.mode == kModeG2B && !p.diagKeo && fabs(td.w) < 1e-6) w = T(0);  // BGSG4:1004
 #pragma unroll
   for (int r = 0; r < EPT; ++r) {
     const int u = tid + r * NT;

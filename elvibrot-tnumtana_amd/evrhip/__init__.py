@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import hamiltonian  # noqa: F401
-from ._lib import BasisDesc, OpDesc, check, load
+from ._lib import BasisDesc, DavidsonInfo, DavidsonOpts, OpDesc, check, load
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -80,11 +80,12 @@ class Basis:
 class Plan:
     """Device plan of one operator on one basis (para_Op + basis on the GPU)."""
 
-    def __init__(self, basis, type=10, V=None, G0=None, Ga=None, F2=None, F1=None, max_nvec=1):
+    def __init__(self, basis, type=10, V=None, G0=None, Ga=None, F2=None, F1=None, max_nvec=1,
+                 hermitian=False):
         self.basis = basis
         f = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, float).ravel(order="F"))
         self._keep = [f(V), f(G0), f(Ga), f(F2), f(F1)]
-        op = OpDesc(type, *[_d(a) for a in self._keep])
+        op = OpDesc(type, *[_d(a) for a in self._keep], 1 if hermitian else 0)
         h = ctypes.c_void_p()
         check(load().evr_plan_create(basis.ptr, ctypes.byref(op), max_nvec, ctypes.byref(h)))
         self.h = h
@@ -138,6 +139,29 @@ class Plan:
             nm = names.raw[64 * k:64 * (k + 1)].split(b"\0")[0].decode()
             out.append((nm, ms[k], fl[k], int(la[k])))
         return out
+
+    def diagonal(self):
+        """Diagonal of H in the packed basis."""
+        d = np.zeros(self.nb_packed)
+        check(load().evr_plan_diagonal(self.h, _d(d), 0))
+        return d
+
+    def davidson(self, nev, block=16, max_sub=0, max_it=200, conv_resi=1e-8, conv_ene=1e-9,
+                 precond=True, diag=None, guess=None, precision=FP64, refine_resi=1e-3,
+                 want_vectors=False, verbose=False):
+        """Lowest nev levels by block Davidson (sub_propagation_Davidson)."""
+        keep = []
+        dg = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
+        gs = None if guess is None else np.asfortranarray(guess, dtype=np.float64)
+        keep += [dg, gs]
+        o = DavidsonOpts(nev, block, max_sub, max_it, conv_resi, conv_ene, 1 if precond else 0,
+                         _d(dg), _d(gs), 0 if gs is None else gs.shape[1], precision, refine_resi,
+                         min(block, self.max_nvec), int(verbose))
+        ev = np.zeros(nev)
+        vec = np.zeros((self.nb_packed, nev), order="F") if want_vectors else None
+        info = DavidsonInfo()
+        check(load().evr_davidson(self.h, ctypes.byref(o), _d(ev), _d(vec), ctypes.byref(info)))
+        return ev, vec, info
 
     def BasisRep_TO_GridRep(self, psiB):
         psiB = np.ascontiguousarray(psiB, dtype=np.float64)

@@ -21,7 +21,21 @@ class BasisDesc(ctypes.Structure):
 
 class OpDesc(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("V", _dp), ("G0", _dp), ("Ga", _dp), ("F2", _dp),
-                ("F1", _dp)]
+                ("F1", _dp), ("hermitian", ctypes.c_int)]
+
+
+class DavidsonOpts(ctypes.Structure):
+    _fields_ = [("nev", ctypes.c_int), ("block", ctypes.c_int), ("max_sub", ctypes.c_int),
+                ("max_it", ctypes.c_int), ("conv_resi", ctypes.c_double), ("conv_ene", ctypes.c_double),
+                ("precond", ctypes.c_int), ("diag", _dp), ("guess", _dp), ("nguess", ctypes.c_int),
+                ("precision", ctypes.c_int), ("refine_resi", ctypes.c_double),
+                ("max_nvec_hint", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class DavidsonInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("nconv", ctypes.c_int),
+                ("hpsi_fp64", ctypes.c_int64), ("hpsi_fp32", ctypes.c_int64),
+                ("max_resi", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
 # name: (restype, argtypes)
@@ -44,6 +58,9 @@ _SIGS = {
                                              ctypes.POINTER(ctypes.c_int)]),
     "evr_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "evr_basis_to_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "evr_plan_diagonal": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
+    "evr_davidson": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DavidsonOpts), _dp, _dp,
+                                    ctypes.POINTER(DavidsonInfo)]),
     "evr_grid_to_basis": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
 }
 

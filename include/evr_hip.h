@@ -98,6 +98,15 @@ typedef struct evr_op_desc {
   const double *Ga;          /* D x D x D (i + D*(j + D*k)) or NULL              */
   const double *F2;          /* D x D (type 1)                                   */
   const double *F1;          /* D (type 1) or NULL                               */
+  int hermitian;             /* 0: the reference operator H as is (OpPsi).
+                                1: its symmetric part (H + H^T)/2, evaluated
+                                exactly in the same pass: the V part is
+                                symmetric (W = B^T diag(w)), P and Q are exactly
+                                (anti)symmetric under the quadrature, and the
+                                mode factors T, U and R+S are replaced by their
+                                (anti)symmetric parts.  Needed by the symmetric
+                                eigensolvers: with dnRGG%d1 = B' pinv(B) the
+                                type-10 operator is not symmetric.             */
 } evr_op_desc;
 
 typedef struct evr_plan evr_plan;
@@ -157,6 +166,13 @@ int evr_plan_kernel_stats(evr_plan *p, int max, char *names, double *ms, double 
  * optionally | EVR_DEVICE_PTRS. */
 int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags);
 
+/* Diagonal of H in the packed basis (nb_packed values): V part exact
+ * (sum_t w_t W^(x) diag(V) B^(x) diagonal, one grid->basis pass of V with
+ * W o B^T), kinetic part from the diagonals of the per-mode factors.  Used as
+ * the Davidson preconditioner (NewVec_type 2, DAV:1392-1413).  flags:
+ * EVR_DEVICE_PTRS if diag is a device pointer. */
+int evr_plan_diagonal(evr_plan *p, double *diag, int flags);
+
 /* BasisTOGrid: sub_PsiBasisRep_TO_GridRep (PSIBG:60) -> RecRvecB_TO_RVecG
  * (BG:605), SG4 branch BSmolyakRep_TO_GSmolyakRep (BGSG4:2216).  psiG is the
  * concatenated term grid (sum nq_t).  One vector, host pointers. */
@@ -164,6 +180,47 @@ int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG);
 /* GridTOBasis: sub_PsiGridRep_TO_BasisRep (PSIBG:178) -> RecRVecG_TO_RvecB
  * (BG:59), SG4 branch with the weighted Smolyak sum (BGSG4:951-1028). */
 int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB);
+
+/* ---- eigen-solver --------------------------------------------------------- */
+/* Block Davidson for the nev lowest levels: sub_propagation_Davidson
+ * (DAV:54-748) with sub_MakeHPsi (DAV:880-1024), Rayleigh-Ritz in FP64
+ * (DAV:1028-1143, here a device symmetric eigensolver), residuals
+ * (DAV:1214-1288), new vectors (DAV:1290-1588: NewVec_type 2 when precond=1,
+ * type 4 otherwise), collapse (DAV:516-606).  Symmetric H expected (type 1, or
+ * a plan created with hermitian=1).  precision EVR_FP32: FP32 H|psi> until the
+ * largest residual drops below refine_resi, then H V is rebuilt in FP64 and the
+ * iterations finish in FP64 (the FP64 refinement). */
+typedef struct evr_davidson_opts {
+  int nev;                 /* levels wanted                                     */
+  int block;               /* new vectors per iteration (<= plan max_nvec)      */
+  int max_sub;             /* subspace size before a collapse (0: auto)         */
+  int max_it;              /* iterations (0: 200)                               */
+  double conv_resi;        /* ||H x - E x|| threshold (energy units)            */
+  double conv_ene;         /* |E(it) - E(it-1)| threshold                       */
+  int precond;             /* 1: Davidson correction r/(E - diag)               */
+  const double *diag;      /* host diagonal (nb_packed) or NULL: evr_plan_diagonal */
+  const double *guess;     /* host start vectors nb_packed x nguess, or NULL:
+                              unit vectors on the lowest diagonal entries       */
+  int nguess;
+  int precision;           /* EVR_FP64 or EVR_FP32                              */
+  double refine_resi;      /* FP32 -> FP64 switch threshold                     */
+  int max_nvec_hint;       /* vectors per evr_oppsi call (0: block)             */
+  int verbose;
+} evr_davidson_opts;
+
+typedef struct evr_davidson_info {
+  int iterations;
+  int converged;           /* 1 when all nev levels met both criteria in FP64   */
+  int nconv;
+  int64_t hpsi_fp64;       /* vector H|psi> applications in FP64                */
+  int64_t hpsi_fp32;       /* ... in FP32                                       */
+  double max_resi;
+  double seconds;
+} evr_davidson_info;
+
+/* evals: nev values (ascending); evecs: host nb_packed x nev or NULL. */
+int evr_davidson(evr_plan *p, const evr_davidson_opts *opts, double *evals, double *evecs,
+                 evr_davidson_info *info);
 
 #ifdef __cplusplus
 }
