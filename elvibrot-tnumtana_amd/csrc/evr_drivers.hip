@@ -252,7 +252,7 @@ struct Dav {
         ++nk;
       }
       if (nk == 0) { kept = 0; break; }
-      HTRY(hipMemcpyAsync(tmpC, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice, st));
+      HTRY(evr::upload_now(tmpC, M.data(), sizeof(double) * M.size(), st));
       RTRY(gemm_nn(kept, nk, 1.0, T, tmpC, kept, 0.0, tmpT));
       HTRY(hipMemcpyAsync(T, tmpT, sizeof(double) * nb * nk, hipMemcpyDeviceToDevice, st));
       kept = nk;
@@ -338,7 +338,7 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
                       [&](int64_t a, int64_t b) { return hdiag[a] < hdiag[b] || (hdiag[a] == hdiag[b] && a < b); });
     DBuf didx2;
     HTRY(didx2.alloc(k0));
-    HTRY(hipMemcpyAsync(didx2.p, idx.data(), sizeof(int64_t) * k0, hipMemcpyHostToDevice, d.st));
+    HTRY(evr::upload_now(didx2.p, idx.data(), sizeof(int64_t) * k0, d.st));
     hipLaunchKernelGGL(k_unit_cols, dim3(nblocks(nb * k0)), dim3(256), 0, d.st, nb, k0,
                        (const int64_t *)didx2.p, V.p);
     HTRY(hipGetLastError());
@@ -388,7 +388,7 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
       const int nc = std::min(blk, ne - j0);
       std::vector<int> sel(nc);
       for (int c = 0; c < nc; ++c) sel[c] = j0 + c;
-      HTRY(hipMemcpyAsync(dsel.p, sel.data(), sizeof(int) * nc, hipMemcpyHostToDevice, d.st));
+      HTRY(evr::upload_now(dsel.p, sel.data(), sizeof(int) * nc, d.st));
       hipLaunchKernelGGL(k_residual, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc,
                          (const int *)dsel.p, theta.p, XH.p, XV.p, T.p);
       hipLaunchKernelGGL(k_colnorm2, dim3(nc), dim3(256), 0, d.st, nb, T.p, nrm.p);
@@ -442,14 +442,14 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
       HTRY(hipMemcpyAsync(HV.p, XH.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
       m = keep;
     }
-    HTRY(hipMemcpyAsync(dsel.p, sel.data(), sizeof(int) * nc, hipMemcpyHostToDevice, d.st));
+    HTRY(evr::upload_now(dsel.p, sel.data(), sizeof(int) * nc, d.st));
     hipLaunchKernelGGL(k_residual, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc,
                        (const int *)dsel.p, theta.p, XH.p, XV.p, T.p);
     HTRY(hipGetLastError());
     if (use_diag) {
       std::vector<double> sh(nc);
       for (int c = 0; c < nc; ++c) sh[c] = th[sel[c]];
-      HTRY(hipMemcpyAsync(dshift.p, sh.data(), sizeof(double) * nc, hipMemcpyHostToDevice, d.st));
+      HTRY(evr::upload_now(dshift.p, sh.data(), sizeof(double) * nc, d.st));
       hipLaunchKernelGGL(k_precond, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc, dshift.p,
                          diag.p, T.p);
       HTRY(hipGetLastError());

@@ -740,6 +740,8 @@ int evr_plan_sizes(const evr_plan *p, int64_t *out)
 
 void *evr_plan_stream(evr_plan *p) { return p ? (void *)p->stream : nullptr; }
 
+int evr_plan_max_nvec(const evr_plan *p) { return p ? p->max_nvec : fail(EVR_ERR_ARG, "NULL plan"); }
+
 static void recount_flops(evr_plan *p)
 {
   for (int c = 0; c < kNumClasses; ++c) p->flops_class[c] = 0;

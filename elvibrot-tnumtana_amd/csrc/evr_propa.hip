@@ -1,0 +1,680 @@
+// Time propagation drivers over the H|psi> engine: Chebyshev (march_cheby)
+// and short iterative Lanczos (march_SIL) for a batch of complex wavepackets,
+// with an optional field coupling H_n = H - E_n mu held constant over a step.
+//
+// Reference: Source_ElVibRot/sub_propagation/sub_module_propa_march.f90
+// (MARCH): march_cheby MARCH:4142-4433 (coefficients cof MARCH:5734-5755,
+// sub_scaledOpPsi OPPSI:2823-2866, Hmax retry MARCH:4348-4373), march_SIL
+// MARCH:2899-3100 with UPsi_spec MARCH:3101-3172.  A complex psi is carried
+// as two real columns (Re, Im), as the reference does for its real operator
+// kernels (OPPSI:392-408); the operator is real symmetric, so H acts on both
+// columns independently and one evr_oppsi call covers a whole batch.
+//
+// Device layout: every wavepacket batch is nb x (2 nwp) column-major; the
+// Krylov space of the SIL driver is one allocation of (npoly + 1) such blocks.
+// The per-order vector work is one grid-stride kernel (recurrence, field term,
+// series accumulation); reductions run one workgroup per (vector, wavepacket)
+// in a fixed order, so results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "evr_propa.hpp"
+
+namespace evr {
+namespace propa {
+
+#define HTRY(expr)                                                                           \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(EVR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+#define RTRY(expr)                                                                           \
+  do {                                                                                       \
+    int r_ = (expr);                                                                         \
+    if (r_ != EVR_OK) return r_;                                                             \
+  } while (0)
+
+static unsigned nblocks(int64_t n)
+{
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+}
+
+// ---- batched complex kernels on (Re, Im) column pairs ------------------------
+
+// One Chebyshev order for every wavepacket w (field E_w, shift e0, scale esc):
+//   T = (HW - E_w MW - e0 W2) / esc
+//   first order (W1 == nullptr):  W3 = -i T
+//   later orders:                 W3 = W1 - 2i T
+//   PSI += c W3
+__global__ void k_cheb_order(int64_t nb, int nwp, const double *E, double e0, double esc,
+                             const double *W1, const double *W2, const double *HW,
+                             const double *MW, double *W3, double *PSI, double c)
+{
+  const double inv = 1.0 / esc;
+  const double f = W1 ? 2.0 : 1.0;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nb * nwp;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int w = (int)(g / nb);
+    const int64_t r = g + nb * w, m = r + nb;  // row i of column 2w, then of column 2w+1
+    double tr = HW[r] - e0 * W2[r], ti = HW[m] - e0 * W2[m];
+    if (MW) {
+      tr -= E[w] * MW[r];
+      ti -= E[w] * MW[m];
+    }
+    tr *= inv;
+    ti *= inv;
+    double xr = f * ti, xi = -f * tr;
+    if (W1) {
+      xr += W1[r];
+      xi += W1[m];
+    }
+    W3[r] = xr;
+    W3[m] = xi;
+    PSI[r] += c * xr;
+    PSI[m] += c * xi;
+  }
+}
+
+// Y_w = beta_w Y_w + sum_j coef[j][w] Q_j,w  (complex coefficients; Q_j = Q + j*qs)
+__global__ void k_zcomb(int64_t nb, int nwp, int nk, const double *Q, int64_t qs,
+                        const double *coef, const double *beta, double *Y)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nb * nwp;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int w = (int)(g / nb);
+    const int64_t r = g + nb * w, m = r + nb;
+    double yr = 0.0, yi = 0.0;
+    if (beta) {
+      const double br = beta[2 * w], bi = beta[2 * w + 1];
+      const double ur = Y[r], ui = Y[m];
+      yr = br * ur - bi * ui;
+      yi = br * ui + bi * ur;
+    }
+    for (int j = 0; j < nk; ++j) {
+      const double cr = coef[2 * (j * nwp + w)], ci = coef[2 * (j * nwp + w) + 1];
+      const double qr = Q[j * qs + r], qi = Q[j * qs + m];
+      yr += cr * qr - ci * qi;
+      yi += cr * qi + ci * qr;
+    }
+    Y[r] = yr;
+    Y[m] = yi;
+  }
+}
+
+// HW_w -= E_w MW_w
+__global__ void k_field(int64_t nb, int nwp, const double *E, const double *MW, double *HW)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < 2 * nb * nwp;
+       g += (int64_t)gridDim.x * blockDim.x)
+    HW[g] -= E[g / (2 * nb)] * MW[g];
+}
+
+// out[j][w] = <X_j,w | Y_w> (complex), X_j = X + j*xs; grid (nwp, nk)
+__global__ __launch_bounds__(256) void k_zdots(int64_t nb, int nwp, const double *X, int64_t xs,
+                                               const double *Y, double *out)
+{
+  __shared__ double rr[256], ri[256];
+  const int w = blockIdx.x, j = blockIdx.y;
+  const double *xr = X + j * xs + nb * (2 * w), *xi = xr + nb;
+  const double *yr = Y + nb * (2 * w), *yi = yr + nb;
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = threadIdx.x; i < nb; i += 256) {
+    sr = fma(xr[i], yr[i], fma(xi[i], yi[i], sr));
+    si = fma(xr[i], yi[i], fma(-xi[i], yr[i], si));
+  }
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      rr[threadIdx.x] += rr[threadIdx.x + s];
+      ri[threadIdx.x] += ri[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * (j * nwp + w)] = rr[0];
+    out[2 * (j * nwp + w) + 1] = ri[0];
+  }
+}
+
+// real versions for one column: out[j] = <X_j|y>, X_j = X + j*xs; grid (nk)
+__global__ __launch_bounds__(256) void k_rdots(int64_t nb, const double *X, int64_t xs,
+                                               const double *y, double *out)
+{
+  __shared__ double rr[256];
+  const double *x = X + blockIdx.x * xs;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < nb; i += 256) s = fma(x[i], y[i], s);
+  rr[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) rr[threadIdx.x] += rr[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = rr[0];
+}
+
+// y = b y + sum_j c_j X_j
+__global__ void k_rcomb(int64_t nb, int nk, const double *X, int64_t xs, const double *c, double b,
+                        double *y)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nb;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = b == 0.0 ? 0.0 : b * y[i];
+    for (int j = 0; j < nk; ++j) v = fma(c[j], X[j * xs + i], v);
+    y[i] = v;
+  }
+}
+
+// ---- host numerics ---------------------------------------------------------
+
+std::vector<double> bessel_j(int n, double r)
+{
+  std::vector<double> out(n + 1, 0.0);
+  if (r == 0.0) {
+    out[0] = 1.0;
+    return out;
+  }
+  // J_{k-1} = (2k/r) J_k - J_{k+1}, started far above max(n, r) and normalised
+  // with J_0 + 2 sum_k J_2k = 1 (Abramowitz & Stegun 9.1.27, 9.1.46).
+  const double ar = std::fabs(r);
+  const int top = std::max(n, (int)ar);
+  const int start = 2 * ((top + 30 + (int)std::sqrt(60.0 * (top + 1))) / 2);
+  std::vector<long double> j(start + 1, 0.0L);
+  long double jp = 0.0L, jc = 1e-30L;
+  for (int k = start; k >= 0; --k) {
+    j[k] = jc;
+    if (k == 0) break;
+    const long double jm = (2.0L * k / (long double)ar) * jc - jp;
+    jp = jc;
+    jc = jm;
+    if (std::fabs((double)jc) > 1e200) {
+      for (int q = k; q <= start; ++q) j[q] *= 1e-200L;
+      jp *= 1e-200L;
+      jc *= 1e-200L;
+    }
+  }
+  long double norm = j[0];
+  for (int k = 2; k <= start; k += 2) norm += 2.0L * j[k];
+  for (int k = 0; k <= n; ++k) {
+    long double v = j[k] / norm;
+    if (r < 0.0 && (k & 1)) v = -v;  // J_k(-r) = (-1)^k J_k(r)
+    out[k] = (double)v;
+  }
+  return out;
+}
+
+void sym_eig(int n, std::vector<double> A, std::vector<double> &ev, std::vector<double> &V)
+{
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[i + (size_t)n * i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (int p = 0; p < n; ++p) {
+      dg += A[p + (size_t)n * p] * A[p + (size_t)n * p];
+      for (int q = p + 1; q < n; ++q) off += A[p + (size_t)n * q] * A[p + (size_t)n * q];
+    }
+    if (off <= 1e-34 * (dg + 1e-300)) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p + (size_t)n * q];
+        if (apq == 0.0) continue;
+        const double th = (A[q + (size_t)n * q] - A[p + (size_t)n * p]) / (2.0 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = A[k + (size_t)n * p], akq = A[k + (size_t)n * q];
+          A[k + (size_t)n * p] = c * akp - s * akq;
+          A[k + (size_t)n * q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[p + (size_t)n * k], aqk = A[q + (size_t)n * k];
+          A[p + (size_t)n * k] = c * apk - s * aqk;
+          A[q + (size_t)n * k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[k + (size_t)n * p], vkq = V[k + (size_t)n * q];
+          V[k + (size_t)n * p] = c * vkp - s * vkq;
+          V[k + (size_t)n * q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  ev.resize(n);
+  for (int i = 0; i < n; ++i) ev[i] = A[i + (size_t)n * i];
+}
+
+std::vector<cplx> expm_tridiag_e1(int k, const double *alpha, const double *beta, double dt)
+{
+  std::vector<double> T((size_t)k * k, 0.0), ev, V;
+  for (int i = 0; i < k; ++i) {
+    T[i + (size_t)k * i] = alpha[i];
+    if (i + 1 < k) T[i + (size_t)k * (i + 1)] = T[(i + 1) + (size_t)k * i] = beta[i + 1];
+  }
+  sym_eig(k, T, ev, V);
+  std::vector<cplx> u(k, 0.0);
+  for (int e = 0; e < k; ++e) {
+    const cplx f = std::exp(cplx(0.0, -ev[e] * dt)) * V[(size_t)k * e];
+    for (int i = 0; i < k; ++i) u[i] += f * V[i + (size_t)k * e];
+  }
+  return u;
+}
+
+int lanczos_bounds(evr_plan *p, int steps, double *lo, double *hi)
+{
+  int64_t sz[7];
+  RTRY(evr_plan_sizes(p, sz));
+  const int64_t nb = sz[0];
+  hipStream_t st = (hipStream_t)evr_plan_stream(p);
+  DBuf Q, w, red, cf;
+  HTRY(Q.alloc(nb * (steps + 1)));
+  HTRY(w.alloc(nb));
+  HTRY(red.alloc(steps + 1));
+  HTRY(cf.alloc(steps + 1));
+  std::vector<double> h(nb);
+  uint64_t s = 0x9E3779B97F4A7C15ull;
+  double nrm = 0.0;
+  for (int64_t i = 0; i < nb; ++i) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    h[i] = (double)(s >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    nrm += h[i] * h[i];
+  }
+  for (auto &v : h) v /= std::sqrt(nrm);
+  HTRY(upload_now(Q.p, h.data(), sizeof(double) * nb, st));
+  std::vector<double> al, be(1, 0.0);
+  for (int k = 0; k < steps; ++k) {
+    RTRY(evr_oppsi(p, 1, Q.p + nb * k, w.p, EVR_FP64 | EVR_DEVICE_PTRS));
+    // c_j = <q_j|H q_k>, j <= k; w -= sum_j c_j q_j (Lanczos with full
+    // re-orthogonalisation: c_k is alpha_k, c_{k-1} is beta_k; a single pass
+    // loses orthogonality within a few tens of steps)
+    hipLaunchKernelGGL(k_rdots, dim3(k + 1), dim3(256), 0, st, nb, Q.p, nb, w.p, red.p);
+    HTRY(hipGetLastError());
+    std::vector<double> c(k + 1);
+    HTRY(hipMemcpyAsync(c.data(), red.p, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, st));
+    HTRY(hipStreamSynchronize(st));
+    al.push_back(c[k]);
+    for (int pass = 0; pass < 2; ++pass) {  // classical Gram-Schmidt, twice
+      if (pass == 1) {
+        hipLaunchKernelGGL(k_rdots, dim3(k + 1), dim3(256), 0, st, nb, Q.p, nb, w.p, red.p);
+        HTRY(hipGetLastError());
+        HTRY(hipMemcpyAsync(c.data(), red.p, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, st));
+        HTRY(hipStreamSynchronize(st));
+      }
+      for (auto &v : c) v = -v;
+      HTRY(upload_now(cf.p, c.data(), sizeof(double) * (k + 1), st));
+      hipLaunchKernelGGL(k_rcomb, dim3(nblocks(nb)), dim3(256), 0, st, nb, k + 1, Q.p, nb, cf.p, 1.0,
+                         w.p);
+      HTRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rdots, dim3(1), dim3(256), 0, st, nb, w.p, 0, w.p, red.p);
+    HTRY(hipGetLastError());
+    double b2 = 0.0;
+    HTRY(hipMemcpyAsync(&b2, red.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HTRY(hipStreamSynchronize(st));
+    const double b = std::sqrt(std::max(b2, 0.0));
+    if (k + 1 == steps || b <= 1e-13 * (std::fabs(al.back()) + 1e-300)) break;
+    be.push_back(b);
+    const double ib = 1.0 / b;
+    HTRY(upload_now(cf.p, &ib, sizeof(double), st));
+    hipLaunchKernelGGL(k_rcomb, dim3(nblocks(nb)), dim3(256), 0, st, nb, 1, w.p, 0, cf.p, 0.0,
+                       Q.p + nb * (k + 1));
+    HTRY(hipGetLastError());
+  }
+  HTRY(hipStreamSynchronize(st));
+  const int k = (int)al.size();
+  std::vector<double> T((size_t)k * k, 0.0), ev, V;
+  for (int i = 0; i < k; ++i) {
+    T[i + (size_t)k * i] = al[i];
+    if (i + 1 < k) T[i + (size_t)k * (i + 1)] = T[(i + 1) + (size_t)k * i] = be[i + 1];
+  }
+  sym_eig(k, T, ev, V);
+  *lo = *std::min_element(ev.begin(), ev.end());
+  *hi = *std::max_element(ev.begin(), ev.end());
+  return EVR_OK;
+}
+
+// ---- batch context ---------------------------------------------------------
+
+int Batch::init(evr_plan *H_, evr_plan *mu_, int nwp_, int flags_)
+{
+  H = H_;
+  mu = mu_;
+  nwp = nwp_;
+  flags = flags_;
+  int64_t sz[7];
+  RTRY(evr_plan_sizes(H, sz));
+  nb = sz[0];
+  st = (hipStream_t)evr_plan_stream(H);
+  if (mu) {
+    int64_t s2[7];
+    RTRY(evr_plan_sizes(mu, s2));
+    if (s2[0] != nb) return set_error(EVR_ERR_ARG, "coupling plan has another basis size");
+  }
+  HTRY(dred.alloc(2 * (int64_t)nwp * 64));
+  HTRY(dcoef.alloc(2 * (int64_t)nwp * 65));
+  HTRY(dE.alloc(nwp));
+  return EVR_OK;
+}
+
+int Batch::apply(evr_plan *p, const double *X, double *Y, int fl)
+{
+  const int ncol = 2 * nwp;
+  const int mv = std::max(1, evr_plan_max_nvec(p));
+  // evr_oppsi runs on the stream of plan p: order it after this batch's work
+  hipStream_t ps = (hipStream_t)evr_plan_stream(p);
+  if (ps != st) HTRY(hipStreamSynchronize(st));
+  for (int c0 = 0; c0 < ncol; c0 += mv) {
+    const int k = std::min(mv, ncol - c0);
+    RTRY(evr_oppsi(p, k, X + nb * c0, Y + nb * c0, fl | EVR_DEVICE_PTRS));
+  }
+  if (ps != st) HTRY(hipStreamSynchronize(ps));
+  if (p == H) hpsi += ncol;
+  return EVR_OK;
+}
+
+int Batch::apply_field(const double *X, double *HX, double *MX, const double *dEp)
+{
+  RTRY(apply(H, X, HX, flags));
+  if (mu && dEp) {
+    if (!MX) return set_error(EVR_ERR_STATE, "no buffer for the coupling term");
+    RTRY(apply(mu, X, MX, flags));
+  }
+  return EVR_OK;
+}
+
+int Batch::dots(const double *X, int64_t xs, int nk, const double *Y, std::vector<cplx> &out)
+{
+  if (nk < 1 || nk > 64) return set_error(EVR_ERR_ARG, "dot products: 1..64 vectors");
+  hipLaunchKernelGGL(k_zdots, dim3(nwp, nk), dim3(256), 0, st, nb, nwp, X, xs, Y, dred.p);
+  HTRY(hipGetLastError());
+  std::vector<double> h(2 * (size_t)nwp * nk);
+  HTRY(hipMemcpyAsync(h.data(), dred.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  HTRY(hipStreamSynchronize(st));
+  out.resize((size_t)nwp * nk);
+  for (size_t i = 0; i < out.size(); ++i) out[i] = cplx(h[2 * i], h[2 * i + 1]);
+  return EVR_OK;
+}
+
+int Batch::comb(const double *Q, int64_t qs, int nk, const std::vector<cplx> &coef,
+                const std::vector<cplx> *beta, double *Y)
+{
+  if (nk < 0 || nk > 64) return set_error(EVR_ERR_ARG, "linear combination: 0..64 vectors");
+  std::vector<double> h(2 * (size_t)nwp * (nk + 1));
+  for (size_t i = 0; i < (size_t)nwp * nk; ++i) {
+    h[2 * i] = coef[i].real();
+    h[2 * i + 1] = coef[i].imag();
+  }
+  const size_t bo = 2 * (size_t)nwp * nk;
+  if (beta)
+    for (int w = 0; w < nwp; ++w) {
+      h[bo + 2 * w] = (*beta)[w].real();
+      h[bo + 2 * w + 1] = (*beta)[w].imag();
+    }
+  HTRY(upload_now(dcoef.p, h.data(), sizeof(double) * h.size(), st));
+  hipLaunchKernelGGL(k_zcomb, dim3(nblocks(nb * nwp)), dim3(256), 0, st, nb, nwp, nk, Q, qs,
+                     dcoef.p, beta ? dcoef.p + bo : nullptr, Y);
+  HTRY(hipGetLastError());
+  return EVR_OK;
+}
+
+int Batch::scale(const std::vector<cplx> &a, double *Y)
+{
+  return comb(Y, 0, 0, std::vector<cplx>(), &a, Y);
+}
+
+int Batch::set_field(const double *E)
+{
+  HTRY(upload_now(dE.p, E, sizeof(double) * nwp, st));
+  return EVR_OK;
+}
+
+int launch_cheb_order(const Batch &P, const double *dE, double e0, double esc, const double *W1,
+                      const double *W2, const double *HW, const double *MW, double *W3, double *PSI,
+                      double c)
+{
+  hipLaunchKernelGGL(k_cheb_order, dim3(nblocks(P.nb * P.nwp)), dim3(256), 0, P.st, P.nb, P.nwp, dE,
+                     e0, esc, W1, W2, HW, MW, W3, PSI, c);
+  HTRY(hipGetLastError());
+  return EVR_OK;
+}
+
+int launch_field(const Batch &P, const double *dE, const double *MW, double *HW)
+{
+  hipLaunchKernelGGL(k_field, dim3(nblocks(2 * P.nb * P.nwp)), dim3(256), 0, P.st, P.nb, P.nwp, dE,
+                     MW, HW);
+  HTRY(hipGetLastError());
+  return EVR_OK;
+}
+
+}  // namespace propa
+}  // namespace evr
+
+using namespace evr::propa;
+using evr::set_error;
+
+extern "C" int evr_propagate(evr_plan *H, const evr_propa_opts *o, int nwp, double *psi,
+                             double *autocorr, evr_propa_info *info)
+{
+  if (!H || !o || !psi || nwp < 1) return set_error(EVR_ERR_ARG, "bad argument");
+  if (o->method != EVR_PROPA_CHEBY && o->method != EVR_PROPA_SIL)
+    return set_error(EVR_ERR_ARG, "method must be EVR_PROPA_CHEBY or EVR_PROPA_SIL");
+  if (o->method == EVR_PROPA_CHEBY && o->npoly < 3)
+    return set_error(EVR_ERR_ARG, "Chebyshev needs npoly >= 3");
+  if (o->method == EVR_PROPA_SIL && (o->npoly < 2 || o->npoly > 63))
+    return set_error(EVR_ERR_ARG, "SIL needs 2 <= npoly <= 63");
+  if (o->nsteps < 0 || !(o->dt > 0.0)) return set_error(EVR_ERR_ARG, "bad dt or nsteps");
+  if ((o->field != nullptr) != (o->mu != nullptr))
+    return set_error(EVR_ERR_ARG, "field values and the coupling plan go together");
+  const auto t_start = std::chrono::steady_clock::now();
+  Batch P;
+  RTRY(P.init(H, o->mu, nwp, o->precision == EVR_FP32 ? EVR_FP32 : EVR_FP64));
+  const int64_t nb = P.nb, nblk = nb * 2 * nwp;
+  const hipMemcpyKind h2d = o->device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  DBuf PSI, HW, MW, PSI0;
+  HTRY(PSI.alloc(nblk));
+  HTRY(HW.alloc(nblk));
+  if (o->mu) HTRY(MW.alloc(nblk));
+  HTRY(hipMemcpyAsync(PSI.p, psi, sizeof(double) * nblk, h2d, P.st));
+  if (autocorr) {
+    HTRY(PSI0.alloc(nblk));
+    HTRY(hipMemcpyAsync(PSI0.p, o->psi0 ? o->psi0 : psi, sizeof(double) * nblk, h2d, P.st));
+  }
+  HTRY(hipStreamSynchronize(P.st));
+  double fmax = 0.0;
+  if (o->field)
+    for (int64_t k = 0; k < (int64_t)o->nsteps * nwp; ++k) fmax = std::max(fmax, std::fabs(o->field[k]));
+  double hmin = o->Hmin, hmax = o->Hmax;
+  int retries = 0, max_order = 0;
+  DBuf W1, W2, W3, SAVE, Q;
+  if (o->method == EVR_PROPA_CHEBY) {
+    if (!(hmin < hmax)) {
+      RTRY(lanczos_bounds(H, 60, &hmin, &hmax));
+      const double pad = 0.02 * (hmax - hmin);
+      hmin -= pad;
+      hmax += pad;
+    }
+    if (o->mu && fmax > 0.0) {
+      double mlo, mhi;
+      RTRY(lanczos_bounds(o->mu, 60, &mlo, &mhi));
+      const double mb = 1.05 * std::max(std::fabs(mlo), std::fabs(mhi));
+      hmin -= fmax * mb;
+      hmax += fmax * mb;
+    }
+    HTRY(W1.alloc(nblk));
+    HTRY(W2.alloc(nblk));
+    HTRY(W3.alloc(nblk));
+    HTRY(SAVE.alloc(nblk));
+  } else {
+    HTRY(Q.alloc(nblk * (o->npoly + 1)));
+  }
+  std::vector<double> E(nwp, 0.0);
+  for (int step = 0; step < o->nsteps; ++step) {
+    if (o->field) {
+      for (int w = 0; w < nwp; ++w) E[w] = o->field[(int64_t)step * nwp + w];
+      RTRY(P.set_field(E.data()));
+    }
+    const double *dEp = o->field ? P.dE.p : nullptr;
+    double *MWp = o->field ? MW.p : nullptr;
+    if (o->method == EVR_PROPA_CHEBY) {
+      // ---- march_cheby (MARCH:4142-4433)
+      HTRY(hipMemcpyAsync(SAVE.p, PSI.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, P.st));
+      bool ok = false;
+      for (int attempt = 0; attempt < 10 && !ok; ++attempt) {
+        const double dE_ = hmax - hmin, E0 = hmin + 0.5 * dE_, esc = 0.5 * dE_;
+        const std::vector<double> J = bessel_j(o->npoly - 1, esc * o->dt);
+        // order 0: w0 = psi, psi = J_0 psi
+        double *w0 = W1.p, *w1 = W2.p, *w2 = W3.p;
+        HTRY(hipMemcpyAsync(w0, SAVE.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, P.st));
+        HTRY(hipMemcpyAsync(PSI.p, SAVE.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, P.st));
+        RTRY(P.scale(std::vector<cplx>(nwp, cplx(J[0], 0.0)), PSI.p));
+        // order 1: w1 = -i Hs w0, psi += 2 J_1 w1
+        RTRY(P.apply_field(w0, HW.p, MWp, dEp));
+        RTRY(launch_cheb_order(P, dEp, E0, esc, nullptr, w0, HW.p, MWp, w1, PSI.p, 2.0 * J[1]));
+        bool blow = false, conv = false;
+        int order = 1;
+        for (order = 2; order < o->npoly; ++order) {
+          RTRY(P.apply_field(w1, HW.p, MWp, dEp));
+          RTRY(launch_cheb_order(P, dEp, E0, esc, w0, w1, HW.p, MWp, w2, PSI.p, 2.0 * J[order]));
+          double *t = w0;
+          w0 = w1;
+          w1 = w2;
+          w2 = t;
+          // exit test |c_k| <w_k|w_k> (MARCH:4306-4335), over every wavepacket
+          std::vector<cplx> nn;
+          RTRY(P.dots(w1, 0, 1, w1, nn));
+          double crit = 0.0;
+          for (int w = 0; w < nwp; ++w)
+            crit = std::max(crit, std::fabs(nn[w].real() * 2.0 * J[order]));
+          if (!(crit <= 1e15)) {
+            blow = true;
+            break;
+          }
+          if (crit < o->poly_tol) {
+            conv = true;
+            break;
+          }
+        }
+        if (blow || !conv) {
+          hmax += 0.1 * (hmax - hmin);
+          ++retries;
+          continue;
+        }
+        max_order = std::max(max_order, order);
+        RTRY(P.scale(std::vector<cplx>(nwp, std::exp(cplx(0.0, -E0 * o->dt))), PSI.p));
+        ok = true;
+      }
+      if (!ok)
+        return set_error(EVR_ERR_STATE, "Chebyshev expansion did not converge at step " +
+                                            std::to_string(step) + " (Hmin " + std::to_string(hmin) +
+                                            ", Hmax " + std::to_string(hmax) +
+                                            "): reduce dt or raise npoly");
+    } else {
+      // ---- march_SIL (MARCH:2899-3100)
+      const int kmax = o->npoly;
+      std::vector<cplx> nn;
+      RTRY(P.dots(PSI.p, 0, 1, PSI.p, nn));
+      std::vector<double> nu(nwp);
+      std::vector<cplx> a(nwp), zero(nwp, 0.0), one(nwp, 1.0);
+      for (int w = 0; w < nwp; ++w) {
+        nu[w] = std::sqrt(std::max(nn[w].real(), 0.0));
+        a[w] = nu[w] > 0.0 ? 1.0 / nu[w] : 0.0;
+      }
+      RTRY(P.comb(PSI.p, 0, 1, a, &zero, Q.p));
+      std::vector<std::vector<double>> al(nwp), be(nwp, std::vector<double>(1, 0.0));
+      std::vector<double> e0(nwp, 0.0);
+      std::vector<std::vector<cplx>> U(nwp);
+      int n = 0;
+      for (int k = 0; k < kmax; ++k) {
+        double *qk = Q.p + nblk * k;
+        RTRY(P.apply_field(qk, HW.p, MWp, dEp));
+        if (MWp) RTRY(launch_field(P, dEp, MWp, HW.p));
+        // w = H q_k - beta_k q_{k-1} (the E0 q_k shift is folded into the projection)
+        if (k > 0) {
+          std::vector<cplx> c(nwp);
+          for (int w = 0; w < nwp; ++w) c[w] = -be[w][k];
+          RTRY(P.comb(Q.p + nblk * (k - 1), 0, 1, c, &one, HW.p));
+        }
+        // c_j = <q_j|w>, j <= k: c_k is alpha_k + E0, the others re-orthogonalise
+        std::vector<cplx> c;
+        RTRY(P.dots(Q.p, nblk, k + 1, HW.p, c));
+        if (k == 0)
+          for (int w = 0; w < nwp; ++w) e0[w] = c[w].real();  // E0 = <psi|H|psi>
+        for (int w = 0; w < nwp; ++w) al[w].push_back(c[(size_t)k * nwp + w].real() - e0[w]);
+        for (int pass = 0; pass < 2; ++pass) {  // classical Gram-Schmidt, twice
+          if (pass == 1) RTRY(P.dots(Q.p, nblk, k + 1, HW.p, c));
+          for (auto &v : c) v = -v;
+          RTRY(P.comb(Q.p, nblk, k + 1, c, &one, HW.p));
+        }
+        bool done = true;
+        for (int w = 0; w < nwp; ++w) {
+          U[w] = expm_tridiag_e1(k + 1, al[w].data(), be[w].data(), o->dt);
+          if (std::abs(U[w][k]) >= o->poly_tol) done = false;
+        }
+        n = k + 1;
+        if (done || k + 1 == kmax) break;
+        std::vector<cplx> bb;
+        RTRY(P.dots(HW.p, 0, 1, HW.p, bb));
+        std::vector<cplx> ib(nwp);
+        for (int w = 0; w < nwp; ++w) {
+          const double b = std::sqrt(std::max(bb[w].real(), 0.0));
+          be[w].push_back(b);
+          ib[w] = b > 0.0 ? 1.0 / b : 0.0;
+        }
+        RTRY(P.comb(HW.p, 0, 1, ib, &zero, Q.p + nblk * (k + 1)));
+      }
+      max_order = std::max(max_order, n);
+      // psi = nu exp(-i E0 dt) sum_k u_k q_k
+      std::vector<cplx> coef((size_t)n * nwp);
+      for (int k = 0; k < n; ++k)
+        for (int w = 0; w < nwp; ++w)
+          coef[(size_t)k * nwp + w] = nu[w] * U[w][k] * std::exp(cplx(0.0, -e0[w] * o->dt));
+      RTRY(P.comb(Q.p, nblk, n, coef, nullptr, PSI.p));
+    }
+    if (autocorr) {
+      std::vector<cplx> ac;
+      RTRY(P.dots(PSI0.p, 0, 1, PSI.p, ac));
+      for (int w = 0; w < nwp; ++w) {
+        autocorr[2 * ((int64_t)step * nwp + w)] = ac[w].real();
+        autocorr[2 * ((int64_t)step * nwp + w) + 1] = ac[w].imag();
+      }
+    }
+  }
+  HTRY(hipMemcpyAsync(psi, PSI.p, sizeof(double) * nblk,
+                      o->device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, P.st));
+  HTRY(hipStreamSynchronize(P.st));
+  if (info) {
+    info->hpsi = P.hpsi;
+    info->max_order = max_order;
+    info->retries = retries;
+    info->Hmin = hmin;
+    info->Hmax = hmax;
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return EVR_OK;
+}
+
+extern "C" int evr_plan_bounds(evr_plan *p, int steps, double *Hmin, double *Hmax)
+{
+  if (!p || !Hmin || !Hmax || steps < 2) return set_error(EVR_ERR_ARG, "bad argument");
+  int64_t sz[7];
+  RTRY(evr_plan_sizes(p, sz));
+  return lanczos_bounds(p, (int)std::min<int64_t>(steps, sz[0]), Hmin, Hmax);
+}
+
+extern "C" int evr_bessel_j(int n, double r, double *out)
+{
+  if (n < 0 || !out) return set_error(EVR_ERR_ARG, "bad argument");
+  const std::vector<double> j = bessel_j(n, r);
+  std::copy(j.begin(), j.end(), out);
+  return EVR_OK;
+}

@@ -1,0 +1,78 @@
+// Shared pieces of the propagation drivers and the control gradient: the
+// batch context (a plan, an optional coupling plan, nwp complex wavepackets
+// stored as (Re, Im) column pairs) and the host numerics they share.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/evr_hip.h"
+#include "evr_internal.hpp"
+
+namespace evr {
+namespace propa {
+
+using cplx = std::complex<double>;
+
+struct DBuf {
+  double *p = nullptr;
+  hipError_t alloc(int64_t k)
+  {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    return k > 0 ? hipMalloc((void **)&p, sizeof(double) * (size_t)k) : hipSuccess;
+  }
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// J_0(r) .. J_n(r) (Miller backward recurrence in long double).
+std::vector<double> bessel_j(int n, double r);
+// Eigenpairs of a small dense symmetric matrix (cyclic Jacobi; V column-major).
+void sym_eig(int n, std::vector<double> A, std::vector<double> &ev, std::vector<double> &V);
+// exp(-i T dt) e_1 for the k x k tridiagonal T (diagonal alpha[0..k-1],
+// off-diagonal beta[1..k-1]).
+std::vector<cplx> expm_tridiag_e1(int k, const double *alpha, const double *beta, double dt);
+// Extreme eigenvalues of a symmetric plan from `steps` Lanczos steps.
+int lanczos_bounds(evr_plan *p, int steps, double *lo, double *hi);
+
+// nwp complex wavepackets on one plan (and an optional coupling plan).
+struct Batch {
+  evr_plan *H = nullptr, *mu = nullptr;
+  hipStream_t st = nullptr;
+  int64_t nb = 0;
+  int nwp = 0, flags = 0;
+  int64_t hpsi = 0;
+  DBuf dred, dcoef, dE;
+
+  int init(evr_plan *H, evr_plan *mu, int nwp, int flags);
+  // Y = p X on all 2 nwp columns (chunks of the plan's max_nvec)
+  int apply(evr_plan *p, const double *X, double *Y, int flags);
+  // HX = H X and, when a field is set, MX = mu X
+  int apply_field(const double *X, double *HX, double *MX, const double *dE);
+  // out[j*nwp + w] = <X_j,w | Y_w>, X_j = X + j*xs, j < nk <= 64
+  int dots(const double *X, int64_t xs, int nk, const double *Y, std::vector<cplx> &out);
+  // Y_w = beta_w Y_w + sum_j coef[j*nwp + w] Q_j,w (beta NULL: Y not read)
+  int comb(const double *Q, int64_t qs, int nk, const std::vector<cplx> &coef,
+           const std::vector<cplx> *beta, double *Y);
+  int scale(const std::vector<cplx> &a, double *Y);
+  int set_field(const double *E);
+};
+
+// One Chebyshev order on the batch (see k_cheb_order) and HW -= E mu W.
+int launch_cheb_order(const Batch &P, const double *dE, double e0, double esc, const double *W1,
+                      const double *W2, const double *HW, const double *MW, double *W3, double *PSI,
+                      double c);
+int launch_field(const Batch &P, const double *dE, const double *MW, double *HW);
+
+}  // namespace propa
+}  // namespace evr

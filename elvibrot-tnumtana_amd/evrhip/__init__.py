@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import hamiltonian  # noqa: F401
-from ._lib import BasisDesc, DavidsonInfo, DavidsonOpts, OpDesc, check, load
+from ._lib import BasisDesc, DavidsonInfo, DavidsonOpts, OpDesc, PropaInfo, PropaOpts, check, load
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -20,6 +20,30 @@ _ip = ctypes.POINTER(ctypes.c_int)
 FP64 = 0
 FP32 = 1
 DEVICE_PTRS = 2
+CHEBY = 1
+SIL = 2
+
+
+def _to_pairs(psi):
+    """Complex (nb, nwp) -> real nb x 2nwp column-major, columns (Re, Im) per wavepacket."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    if psi.ndim == 1:
+        psi = psi[:, None]
+    r = np.empty((psi.shape[0], 2 * psi.shape[1]), order="F")
+    r[:, 0::2] = psi.real
+    r[:, 1::2] = psi.imag
+    return r
+
+
+def _from_pairs(r):
+    return r[:, 0::2] + 1j * r[:, 1::2]
+
+
+def bessel_j(n, r):
+    """J_0(r) .. J_n(r) as computed for the Chebyshev coefficients."""
+    out = np.zeros(n + 1)
+    check(load().evr_bessel_j(n, float(r), _d(out)))
+    return out
 
 
 def _d(a):
@@ -162,6 +186,36 @@ class Plan:
         info = DavidsonInfo()
         check(load().evr_davidson(self.h, ctypes.byref(o), _d(ev), _d(vec), ctypes.byref(info)))
         return ev, vec, info
+
+    def bounds(self, steps=60):
+        """(Hmin, Hmax) estimated by Lanczos."""
+        lo, hi = ctypes.c_double(), ctypes.c_double()
+        check(load().evr_plan_bounds(self.h, steps, ctypes.byref(lo), ctypes.byref(hi)))
+        return lo.value, hi.value
+
+    def propagate(self, psi, dt, nsteps, method=CHEBY, npoly=200, poly_tol=1e-12, Hmin=0.0, Hmax=0.0,
+                  field=None, mu=None, psi0=None, precision=FP64, autocorr=False):
+        """psi(t + nsteps dt) for complex psi (nb_packed,) or (nb_packed, nwp): march_cheby / march_SIL.
+
+        field: (nsteps, nwp) values of the field coupling -E mu (mu a Plan on the
+        same basis), constant over each step.  Returns (psi, autocorr or None, info)."""
+        r = _to_pairs(psi)
+        nwp = r.shape[1] // 2
+        fl = None
+        if field is not None:
+            fl = np.ascontiguousarray(np.asarray(field, dtype=np.float64).reshape(nsteps, nwp))
+        p0 = None if psi0 is None else _to_pairs(psi0)
+        o = PropaOpts(method, dt, nsteps, npoly, poly_tol, Hmin, Hmax, _d(fl),
+                      None if mu is None else mu.h,
+                      None if p0 is None else p0.ctypes.data_as(ctypes.c_void_p), precision, 0)
+        ac = np.zeros((nsteps, nwp, 2)) if autocorr else None
+        info = PropaInfo()
+        check(load().evr_propagate(self.h, ctypes.byref(o), nwp, r.ctypes.data_as(ctypes.c_void_p), _d(ac),
+                                   ctypes.byref(info)))
+        out = _from_pairs(r)
+        if np.ndim(psi) == 1:
+            out = out[:, 0]
+        return out, (None if ac is None else ac[..., 0] + 1j * ac[..., 1]), info
 
     def BasisRep_TO_GridRep(self, psiB):
         psiB = np.ascontiguousarray(psiB, dtype=np.float64)

@@ -38,6 +38,18 @@ class DavidsonInfo(ctypes.Structure):
                 ("max_resi", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
+class PropaOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("dt", ctypes.c_double), ("nsteps", ctypes.c_int),
+                ("npoly", ctypes.c_int), ("poly_tol", ctypes.c_double), ("Hmin", ctypes.c_double),
+                ("Hmax", ctypes.c_double), ("field", _dp), ("mu", ctypes.c_void_p), ("psi0", ctypes.c_void_p),
+                ("precision", ctypes.c_int), ("device_ptrs", ctypes.c_int)]
+
+
+class PropaInfo(ctypes.Structure):
+    _fields_ = [("hpsi", ctypes.c_int64), ("max_order", ctypes.c_int), ("retries", ctypes.c_int),
+                ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
 # name: (restype, argtypes)
 _SIGS = {
     "evr_last_error": (ctypes.c_char_p, []),
@@ -62,6 +74,11 @@ _SIGS = {
     "evr_davidson": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DavidsonOpts), _dp, _dp,
                                     ctypes.POINTER(DavidsonInfo)]),
     "evr_grid_to_basis": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "evr_plan_max_nvec": (ctypes.c_int, [ctypes.c_void_p]),
+    "evr_propagate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PropaOpts), ctypes.c_int, ctypes.c_void_p,
+                                     _dp, ctypes.POINTER(PropaInfo)]),
+    "evr_plan_bounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "evr_bessel_j": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _dp]),
 }
 
 _LIB = None

@@ -149,6 +149,8 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
 int evr_plan_sizes(const evr_plan *p, int64_t *out);
 /* hipStream_t of the plan (as void*), for callers that pass device pointers. */
 void *evr_plan_stream(evr_plan *p);
+/* Largest nvec one evr_oppsi call accepts (the max_nvec given at creation). */
+int evr_plan_max_nvec(const evr_plan *p);
 /* Profiling (no counterpart in the reference, whose analogue is the
  * nb_mult_BTOG/GTOB counters, SYS:120): when on, every term-kernel launch is
  * bracketed by HIP events on the plan stream; evr_plan_kernel_stats returns,
@@ -221,6 +223,71 @@ typedef struct evr_davidson_info {
 /* evals: nev values (ascending); evecs: host nb_packed x nev or NULL. */
 int evr_davidson(evr_plan *p, const evr_davidson_opts *opts, double *evals, double *evecs,
                  evr_davidson_info *info);
+
+/* ---- time propagation ------------------------------------------------------ */
+/* psi(t + dt) for a batch of nwp complex wavepackets: march_cheby
+ * (MARCH:4142-4433) and march_SIL (MARCH:2899-3100).  psi is nb_packed x
+ * (2 nwp) column-major, columns (Re psi_w, Im psi_w) for w = 0..nwp-1, and is
+ * overwritten with psi(nsteps dt).
+ *
+ * Chebyshev: H_s = (H - E0)/Esc with E0 = Hmin + dE/2, Esc = dE/2, dE =
+ * Hmax - Hmin (MARCH:4211-4231); coefficients c_0 = J_0(r), c_k = 2 J_k(r),
+ * r = Esc dt (cof, MARCH:5734-5755); orders 0..npoly-1 at most, stopping at
+ * the first order k >= 2 with |c_k| <w_k|w_k> < poly_tol for every wavepacket
+ * (MARCH:4290-4335); on failure Hmax += dE/10 and the step is redone, at most
+ * 10 times (MARCH:4348-4373); finally psi *= exp(-i E0 dt).  Hmin >= Hmax in
+ * the options asks for the interval from a 60-step Lanczos estimate of the
+ * extreme eigenvalues, widened by 2% of the range (the reference obtains it
+ * from sub_Hmax, HMAX:43-328).
+ * SIL: Krylov space of at most npoly vectors from psi_w, E0_w = <psi|H|psi>,
+ * stopping when the last Krylov component of exp(-i T dt) e_1 is below
+ * poly_tol (MARCH:2957-2990, UPsi_spec MARCH:3101-3172).  The Krylov vectors
+ * are re-orthogonalised against all previous ones (classical Gram-Schmidt,
+ * two passes); the reference keeps the three-term recurrence only.
+ * Field: with field != NULL the step n (0-based) uses H - field[n*nwp + w] mu
+ * for wavepacket w, mu being a plan on the same basis (a dipole surface as a
+ * type-0 operator); the field is constant over a step (for Chebyshev the
+ * spectral interval is widened by max|field| * |mu|_max).
+ * The operator must be symmetric (type 1, or a plan created hermitian=1). */
+#define EVR_PROPA_CHEBY 1
+#define EVR_PROPA_SIL 2
+typedef struct evr_propa_opts {
+  int method;              /* EVR_PROPA_CHEBY or EVR_PROPA_SIL                  */
+  double dt;               /* WPdeltaT                                          */
+  int nsteps;
+  int npoly;               /* max Chebyshev terms / Krylov dimension            */
+  double poly_tol;
+  double Hmin, Hmax;       /* spectral interval (Hmin >= Hmax: estimated)       */
+  const double *field;     /* nsteps x nwp values (host) or NULL                */
+  evr_plan *mu;            /* coupling operator when field != NULL              */
+  const double *psi0;      /* autocorrelation reference, same layout as psi;
+                              NULL: the initial psi                             */
+  int precision;           /* EVR_FP64 (EVR_FP32: H|psi> in FP32)               */
+  int device_ptrs;         /* psi / psi0 are device pointers                    */
+} evr_propa_opts;
+
+typedef struct evr_propa_info {
+  int64_t hpsi;            /* real-column H|psi> applications                   */
+  int max_order;           /* largest Chebyshev order / Krylov dimension used   */
+  int retries;             /* Hmax enlargements                                 */
+  double Hmin, Hmax;       /* interval used (Chebyshev)                         */
+  double seconds;
+} evr_propa_info;
+
+/* autocorr: host nsteps x nwp complex values <psi0_w|psi_w(t_n)> (Re, Im
+ * pairs, Calc_AutoCorr) or NULL. */
+int evr_propagate(evr_plan *H, const evr_propa_opts *opts, int nwp, double *psi,
+                  double *autocorr, evr_propa_info *info);
+
+/* Extreme eigenvalues of a symmetric plan from `steps` Lanczos steps with full
+ * re-orthogonalisation (min(steps, nb_packed) used), the estimate behind the
+ * Chebyshev interval; the reference's sub_Hmax (HMAX:43-328) takes Hmin from
+ * the grid potential and Hmax from a power iteration. */
+int evr_plan_bounds(evr_plan *p, int steps, double *Hmin, double *Hmax);
+
+/* J_0(r) .. J_n(r) (out: n + 1 values), the Bessel functions behind the
+ * Chebyshev coefficients (mmbsjn, POLY:1781), by Miller's backward recurrence. */
+int evr_bessel_j(int n, double r, double *out);
 
 #ifdef __cplusplus
 }

@@ -79,3 +79,21 @@ def test_bad_arguments_raise(evr):
         evr.Basis.sg4(3, 2, 4)
     with pytest.raises(RuntimeError):
         evr.Basis.dp([5], [4])
+
+
+@pytest.mark.parametrize("r", [0.0, 0.3, 5.0, 57.3, 1234.5])
+def test_bessel_coefficients(evr, r):
+    """J_k(r) behind the Chebyshev coefficients (host code) against scipy and,
+    where scipy's own error reaches 1e-14 (large r), 40-digit mpmath values."""
+    import mpmath
+    from scipy.special import jv
+    n = int(r) + 60
+    j = evr.bessel_j(n, r)
+    assert np.abs(j - jv(np.arange(n + 1), r)).max() < 1e-12
+    mpmath.mp.dps = 40
+    for k in sorted({0, 1, 2, n // 3, n // 2, max(n - 61, 0), n - 30, n}):
+        assert abs(j[k] - float(mpmath.besselj(k, r))) < 2e-16, k
+    # J_0 + 2 sum J_2k = 1 and sum_k J_k^2 (k in Z) = 1 (tail beyond r + 30 r^(1/3) negligible)
+    j = evr.bessel_j(int(r + 30 * r ** (1 / 3) + 60), r)
+    assert abs(j[0] + 2 * j[2::2].sum() - 1.0) < 1e-14
+    assert abs(j[0] ** 2 + 2 * (j[1:] ** 2).sum() - 1.0) < 1e-14
