@@ -1,0 +1,354 @@
+// Optimal-control objective and its adjoint gradient for a batch of fields.
+//
+// Each field f drives psi_f from psi0 with H_n = H - E_f(t_n) mu, the field
+// held at its mid-step value over step n (t_n = (n + 1/2) dt), one march_cheby
+// step per dt.  J_f = |<phi1|psi_f(T)>|^2 (the fidelity of calc_fidelity,
+// Source_ElVibRot/sub_propagation/sub_control.f90:1000-1057).  The reference
+// updates its field by Krotov / Zhu-Rabitz (sub_Opt_control, CTRL:290-520);
+// the field expansion E(t) = sin^2(pi t/T) sum_k a_k sin(k pi t/T) and the
+// gradient below are this library's own.
+//
+// Gradient: exact reverse mode through the computed Chebyshev recurrences.
+// Forward, w_0 = psi_n, w_1 = -i Hs w_0, w_k = w_{k-2} - 2i Hs w_{k-1},
+// psi_{n+1} = p sum_k c_k w_k with Hs = (H - E_n mu - E0)/Esc and p =
+// exp(-i E0 dt).  With g the gradient of J with respect to psi_{n+1}
+// (dJ = Re<g, dpsi>), the costates run backwards through the same H|psi>:
+//   lam_K = c_K p* g,  lam_k = c_k p* g + f_k i Hs lam_{k+1} + lam_{k+2}
+// (f_0 = 1, f_k = 2 otherwise), and g <- lam_0 for the previous step.  Every
+// application Hs w_k inside w_{k+1} contributes Re<lam_{k+1}, s_{k+1} dHs w_k>
+// (s_1 = -i, s_k = -2i): dHs/dE_n = -mu/Esc and dHs/dV_g = P_g/Esc with
+// <x|P_g|y> = gw_g conj(B x)_g (B y)_g.  The wavepackets are stored every
+// `checkpoint` steps; a segment is recomputed forward (same orders, same
+// E0/Esc, so bit-identical) with its w_k kept, then swept backwards.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "evr_propa.hpp"
+
+namespace evr {
+namespace propa {
+namespace {
+
+#define HTRY(expr)                                                                           \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_error(EVR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+#define RTRY(expr)                                                                           \
+  do {                                                                                       \
+    int r_ = (expr);                                                                         \
+    if (r_ != EVR_OK) return r_;                                                             \
+  } while (0)
+
+unsigned nblocks(int64_t n)
+{
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+}
+
+// Costate order k for every wavepacket w:
+//   T = (HW - E_w MW - e0 L1) / esc          (Hs lam_{k+1}; skipped when L1 == NULL)
+//   OUT = (cr + i ci) G + fi * i T + L2
+__global__ void k_adj_order(int64_t nb, int nwp, const double *E, double e0, double esc, double fi,
+                            double cr, double ci, const double *G, const double *L1,
+                            const double *L2, const double *HW, const double *MW, double *OUT)
+{
+  const double inv = 1.0 / esc;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nb * nwp;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int w = (int)(g / nb);
+    const int64_t r = g + nb * w, m = r + nb;
+    double yr = cr * G[r] - ci * G[m], yi = cr * G[m] + ci * G[r];
+    if (L1) {
+      double tr = HW[r] - e0 * L1[r], ti = HW[m] - e0 * L1[m];
+      if (MW) {
+        tr -= E[w] * MW[r];
+        ti -= E[w] * MW[m];
+      }
+      yr -= fi * inv * ti;  // i T = (-ti, tr)
+      yi += fi * inv * tr;
+    }
+    if (L2) {
+      yr += L2[r];
+      yi += L2[m];
+    }
+    OUT[r] = yr;
+    OUT[m] = yi;
+  }
+}
+
+// acc[w] += Re((cr + i ci) <X_w|Y_w>); one workgroup per wavepacket, fixed order
+__global__ __launch_bounds__(256) void k_zdot_acc(int64_t nb, const double *X, const double *Y,
+                                                  double cr, double ci, double *acc)
+{
+  __shared__ double rr[256], ri[256];
+  const int w = blockIdx.x;
+  const double *xr = X + nb * (2 * w), *xi = xr + nb, *yr = Y + nb * (2 * w), *yi = yr + nb;
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = threadIdx.x; i < nb; i += 256) {
+    sr = fma(xr[i], yr[i], fma(xi[i], yi[i], sr));
+    si = fma(xr[i], yi[i], fma(-xi[i], yr[i], si));
+  }
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      rr[threadIdx.x] += rr[threadIdx.x + s];
+      ri[threadIdx.x] += ri[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[w] += cr * rr[0] - ci * ri[0];
+}
+
+// GV[g + nq*w] += gw_g Re((cr + i ci) conj(GL_w)_g (GW_w)_g), GL/GW: nq x 2nwp
+__global__ void k_vgrad_acc(int64_t nq, int nwp, const double *GL, const double *GW,
+                            const double *gw, double cr, double ci, double *GV)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nq * nwp;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int w = (int)(g / nq);
+    const int64_t i = g - nq * w, r = i + nq * (2 * w), m = r + nq;
+    const double lr = GL[r], li = GL[m], xr = GW[r], xi = GW[m];
+    const double a = lr * xr + li * xi, b = lr * xi - li * xr;
+    GV[g] += gw[i] * (cr * a - ci * b);
+  }
+}
+
+// E_f(t_n) and the basis functions of the expansion at the mid-steps
+void field_basis(const evr_control_opts *o, std::vector<double> &S /* nsteps x nk */)
+{
+  const double dt = o->T / o->nsteps;
+  S.assign((size_t)o->nsteps * o->nk, 0.0);
+  for (int n = 0; n < o->nsteps; ++n) {
+    const double t = (n + 0.5) * dt, s = std::sin(M_PI * t / o->T);
+    for (int k = 1; k <= o->nk; ++k)
+      S[(size_t)n * o->nk + (k - 1)] = s * s * std::sin(k * M_PI * t / o->T);
+  }
+}
+
+}  // namespace
+}  // namespace propa
+}  // namespace evr
+
+using namespace evr::propa;
+using evr::set_error;
+
+extern "C" int evr_control_field(const evr_control_opts *o, int nf, const double *amp, double *field)
+{
+  if (!o || !amp || !field || nf < 1 || o->nsteps < 1 || o->nk < 1 || !(o->T > 0.0))
+    return set_error(EVR_ERR_ARG, "bad argument");
+  std::vector<double> S;
+  field_basis(o, S);
+  for (int n = 0; n < o->nsteps; ++n)
+    for (int f = 0; f < nf; ++f) {
+      double e = 0.0;
+      for (int k = 0; k < o->nk; ++k) e += S[(size_t)n * o->nk + k] * amp[(size_t)f * o->nk + k];
+      field[(size_t)n * nf + f] = e;
+    }
+  return EVR_OK;
+}
+
+extern "C" int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control_opts *o, int nf,
+                                    const double *amp, const double *psi0, const double *phi1,
+                                    double *J, double *grad_a, double *grad_E, double *grad_V,
+                                    evr_control_info *info)
+{
+  if (!H || !mu || !o || !amp || !psi0 || !phi1 || !J || nf < 1)
+    return set_error(EVR_ERR_ARG, "bad argument");
+  if (o->nsteps < 1 || o->nk < 1 || !(o->T > 0.0) || o->npoly < 3)
+    return set_error(EVR_ERR_ARG, "need nsteps >= 1, nk >= 1, T > 0, npoly >= 3");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int N = o->nsteps, ck = o->checkpoint > 0 ? o->checkpoint : 32;
+  const double dt = o->T / N;
+  Batch P;
+  RTRY(P.init(H, mu, nf, EVR_FP64));
+  int64_t sz[7];
+  RTRY(evr_plan_sizes(H, sz));
+  const int64_t nb = P.nb, nblk = nb * 2 * nf, nq = sz[3];
+  const size_t bytes = sizeof(double) * nblk;
+
+  // field values at the mid-steps
+  std::vector<double> field((size_t)N * nf);
+  RTRY(evr_control_field(o, nf, amp, field.data()));
+  double fmax = 0.0;
+  for (double e : field) fmax = std::max(fmax, std::fabs(e));
+  double hmin = o->Hmin, hmax = o->Hmax;
+  if (!(hmin < hmax)) {
+    RTRY(lanczos_bounds(H, 60, &hmin, &hmax));
+    const double pad = 0.02 * (hmax - hmin);
+    hmin -= pad;
+    hmax += pad;
+  }
+  if (fmax > 0.0) {
+    double mlo, mhi;
+    RTRY(lanczos_bounds(mu, 60, &mlo, &mhi));
+    const double mb = 1.05 * std::max(std::fabs(mlo), std::fabs(mhi));
+    hmin -= fmax * mb;
+    hmax += fmax * mb;
+  }
+
+  // psi0 and phi1 replicated over the fields
+  DBuf PSI, PHI, MW, G, CK;
+  ChebBufs cw;
+  HTRY(PSI.alloc(nblk));
+  HTRY(PHI.alloc(nblk));
+  HTRY(MW.alloc(nblk));
+  HTRY(G.alloc(nblk));
+  RTRY(cw.alloc(nblk));
+  const int nck = (N + ck - 1) / ck;
+  HTRY(CK.alloc(nblk * nck));
+  for (int f = 0; f < nf; ++f) {
+    HTRY(hipMemcpyAsync(PSI.p + 2 * nb * f, psi0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, P.st));
+    HTRY(hipMemcpyAsync(PHI.p + 2 * nb * f, phi1, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, P.st));
+  }
+  HTRY(hipStreamSynchronize(P.st));
+
+  // ---- forward, storing psi every ck steps
+  std::vector<ChebRecord> rec(N);
+  int retries = 0, max_order = 0;
+  for (int n = 0; n < N; ++n) {
+    if (n % ck == 0)
+      HTRY(hipMemcpyAsync(CK.p + nblk * (n / ck), PSI.p, bytes, hipMemcpyDeviceToDevice, P.st));
+    RTRY(P.set_field(field.data() + (size_t)n * nf));
+    RTRY(cheb_step(P, cw, PSI.p, P.dE.p, MW.p, dt, o->npoly, o->poly_tol, hmin, hmax, rec[n], retries,
+                   nullptr, false));
+    max_order = std::max(max_order, rec[n].order);
+  }
+  std::vector<cplx> ov;
+  RTRY(P.dots(PHI.p, 0, 1, PSI.p, ov));
+  for (int f = 0; f < nf; ++f) J[f] = std::norm(ov[f]);
+  const auto t1 = std::chrono::steady_clock::now();
+
+  const bool want_E = grad_a || grad_E, want_V = grad_V != nullptr;
+  if (want_E || want_V) {
+    const double *gw = nullptr;
+    DBuf dEacc, GV, GL, GX, LA, LB, LC, ST;
+    HTRY(dEacc.alloc((int64_t)N * nf));
+    HTRY(hipMemsetAsync(dEacc.p, 0, sizeof(double) * N * nf, P.st));
+    if (want_V) {
+      RTRY(evr::plan_grid_weights(H, &gw));
+      HTRY(GV.alloc(nq * nf));
+      HTRY(hipMemsetAsync(GV.p, 0, sizeof(double) * nq * nf, P.st));
+      HTRY(GL.alloc(nq * 2 * nf));
+      HTRY(GX.alloc(nq * 2 * nf));
+    }
+    HTRY(LA.alloc(nblk));
+    HTRY(LB.alloc(nblk));
+    HTRY(LC.alloc(nblk));
+    // segment store: w_k of every step of one segment
+    int kseg = 0;
+    for (int n = 0; n < N; ++n) kseg = std::max(kseg, rec[n].order + 1);
+    HTRY(ST.alloc(nblk * (int64_t)kseg * std::min(ck, N)));
+    // g = dJ/dpsi(T) = 2 <phi1|psi(T)> phi1
+    {
+      std::vector<cplx> c(nf), zero(nf, 0.0);
+      for (int f = 0; f < nf; ++f) c[f] = 2.0 * ov[f];
+      RTRY(P.comb(PHI.p, 0, 1, c, &zero, G.p));
+    }
+    for (int seg = nck - 1; seg >= 0; --seg) {
+      const int s0 = seg * ck, s1 = std::min(N, s0 + ck);
+      // recompute the segment forward, keeping every w_k
+      HTRY(hipMemcpyAsync(PSI.p, CK.p + nblk * seg, bytes, hipMemcpyDeviceToDevice, P.st));
+      for (int n = s0; n < s1; ++n) {
+        RTRY(P.set_field(field.data() + (size_t)n * nf));
+        int dummy = 0;
+        RTRY(cheb_step(P, cw, PSI.p, P.dE.p, MW.p, dt, o->npoly, o->poly_tol, hmin, hmax, rec[n], dummy,
+                       ST.p + nblk * (int64_t)kseg * (n - s0), true));
+      }
+      // backwards through the segment
+      for (int n = s1 - 1; n >= s0; --n) {
+        RTRY(P.set_field(field.data() + (size_t)n * nf));
+        const ChebRecord &r = rec[n];
+        const int K = r.order;
+        const double *W = ST.p + nblk * (int64_t)kseg * (n - s0);
+        const std::vector<double> Jb = bessel_j(K, r.esc * dt);
+        const cplx pc = std::exp(cplx(0.0, r.e0 * dt));  // conj(exp(-i E0 dt))
+        auto ck_of = [&](int k) { return (k == 0 ? 1.0 : 2.0) * Jb[k]; };
+        // lam_K = c_K p* g
+        double *l1 = LA.p, *l2 = LB.p, *l0 = LC.p;  // lam_{k+1}, lam_{k+2}, lam_k
+        {
+          const cplx c = ck_of(K) * pc;
+          hipLaunchKernelGGL(k_adj_order, dim3(nblocks(nb * nf)), dim3(256), 0, P.st, nb, nf,
+                             (const double *)nullptr, 0.0, 1.0, 0.0, c.real(), c.imag(), G.p,
+                             (const double *)nullptr, (const double *)nullptr,
+                             (const double *)nullptr, (const double *)nullptr, l1);
+          HTRY(hipGetLastError());
+        }
+        bool have_l2 = false;
+        for (int k = K - 1; k >= 0; --k) {
+          // H lam_{k+1}, mu lam_{k+1}
+          RTRY(P.apply(H, l1, cw.HW.p, EVR_FP64));
+          RTRY(P.apply(mu, l1, MW.p, EVR_FP64));
+          // order k+1 used Hs w_k with factor s_{k+1} = -i (k = 0) or -2i
+          const double sk = (k == 0) ? -1.0 : -2.0;  // s = i * sk
+          // dE_n += Re((-s/esc) <mu lam|w_k>) = Re((-i sk/esc) <MW|w_k>)
+          if (want_E) {
+            hipLaunchKernelGGL(k_zdot_acc, dim3(nf), dim3(256), 0, P.st, nb, MW.p, W + nblk * k, 0.0,
+                               -sk / r.esc, dEacc.p + (int64_t)n * nf);
+            HTRY(hipGetLastError());
+          }
+          if (want_V) {
+            RTRY(evr::plan_b2g_device(H, 2 * nf, l1, GL.p));
+            RTRY(evr::plan_b2g_device(H, 2 * nf, W + nblk * k, GX.p));
+            hipLaunchKernelGGL(k_vgrad_acc, dim3(nblocks(nq * nf)), dim3(256), 0, P.st, nq, nf, GL.p,
+                               GX.p, gw, 0.0, sk / r.esc, GV.p);
+            HTRY(hipGetLastError());
+          }
+          // lam_k = c_k p* g + f_k i Hs lam_{k+1} + lam_{k+2}
+          const cplx c = ck_of(k) * pc;
+          hipLaunchKernelGGL(k_adj_order, dim3(nblocks(nb * nf)), dim3(256), 0, P.st, nb, nf,
+                             (const double *)P.dE.p, r.e0, r.esc, k == 0 ? 1.0 : 2.0, c.real(),
+                             c.imag(), G.p, (const double *)l1, have_l2 ? (const double *)l2 : nullptr,
+                             (const double *)cw.HW.p, (const double *)MW.p, l0);
+          HTRY(hipGetLastError());
+          double *t = l2;
+          l2 = l1;
+          l1 = l0;
+          l0 = t;
+          have_l2 = true;
+        }
+        // g <- lam_0 (now in l1)
+        HTRY(hipMemcpyAsync(G.p, l1, bytes, hipMemcpyDeviceToDevice, P.st));
+      }
+    }
+    std::vector<double> dE((size_t)N * nf);
+    HTRY(hipMemcpyAsync(dE.data(), dEacc.p, sizeof(double) * dE.size(), hipMemcpyDeviceToHost, P.st));
+    HTRY(hipStreamSynchronize(P.st));
+    if (grad_E) std::copy(dE.begin(), dE.end(), grad_E);
+    if (grad_a) {
+      std::vector<double> S;
+      field_basis(o, S);
+      for (int f = 0; f < nf; ++f)
+        for (int k = 0; k < o->nk; ++k) {
+          double s = 0.0;
+          for (int n = 0; n < N; ++n) s += dE[(size_t)n * nf + f] * S[(size_t)n * o->nk + k];
+          grad_a[(size_t)f * o->nk + k] = s;
+        }
+    }
+    if (want_V) {
+      std::vector<double> gv((size_t)nq * nf);
+      HTRY(hipMemcpyAsync(gv.data(), GV.p, sizeof(double) * gv.size(), hipMemcpyDeviceToHost, P.st));
+      HTRY(hipStreamSynchronize(P.st));
+      for (int f = 0; f < nf; ++f)
+        RTRY(evr::plan_grid_to_user(H, gv.data() + (size_t)nq * f, grad_V + (size_t)nq * f));
+    }
+  }
+  if (info) {
+    info->hpsi = P.hpsi;
+    info->max_order = max_order;
+    info->retries = retries;
+    info->Hmin = hmin;
+    info->Hmax = hmax;
+    info->seconds_forward = std::chrono::duration<double>(t1 - t0).count();
+    info->seconds_backward =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+  }
+  return EVR_OK;
+}

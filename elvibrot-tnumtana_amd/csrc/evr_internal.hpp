@@ -4,6 +4,8 @@
 
 #include <string>
 
+#include "../../include/evr_hip.h"
+
 namespace evr {
 // Record the message returned by evr_last_error() and return `code`.
 int set_error(int code, const std::string &msg);
@@ -16,4 +18,14 @@ inline hipError_t upload_now(void *dst, const void *src, size_t bytes, hipStream
   hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
   return e != hipSuccess ? e : hipStreamSynchronize(s);
 }
+
+// Basis -> grid for ncol device columns on the plan stream: X is nb_packed x
+// ncol, G is sum_nq x ncol (column-major, term-concatenated grid order).
+int plan_b2g_device(evr_plan *p, int ncol, const double *X, double *G);
+// Device array (sum_nq) of w_t * (quadrature weight) for every term-grid point:
+// the V part of the operator is sum_g gw_g b_g b_g^T with b_g the basis
+// functions at g, so d<x|H|y>/dV_g = gw_g (B x)_g (B y)_g.
+int plan_grid_weights(evr_plan *p, const double **gw);
+// Term-concatenated grid order -> the caller's grid order (DP: reference order).
+int plan_grid_to_user(const evr_plan *p, const double *term_grid, double *user);
 }  // namespace evr

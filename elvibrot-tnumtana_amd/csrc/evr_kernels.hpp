@@ -65,7 +65,8 @@ struct TermParams {
   const double *keoc;         // pre-zeroed coefficient strips (see plan)
   const T *X;                 // packed input, [nb_packed][nvec]
   T *R;                       // term results, [sum nb_t][nvec]
-  T *G;                       // grid output (B2G), [sum nq_t]
+  T *G;                       // grid output (B2G), column v at G + v*gld
+  int64_t gld;                // column stride of G (sum nq_t)
   const T *Gin;               // grid input (G2B), [sum nq_t]
 };
 

@@ -103,6 +103,7 @@ struct evr_plan {
   size_t lds_bytes64[kNumClasses] = {}, lds_bytes32[kNumClasses] = {};
   // work buffers
   DevBuf<double> d_X, d_R, d_psi, d_hpsi, d_G;
+  DevBuf<double> d_gw;  // w_t * quadrature weight of every term-grid point (lazy)
   bool V32_valid = false;
   // per-launch accounting: algorithmic flops per vector of each kernel class
   double flops_class[kNumClasses] = {};
@@ -165,6 +166,7 @@ evr::TermParams<T> base_params(evr_plan *p, int mode, int nvec)
   prm.coefA = p->d_coefA.ptr;
   prm.coefF1 = p->d_coefF1.ptr;
   prm.keoc = p->d_keoc.ptr;
+  prm.gld = p->sum_nq;
   return prm;
 }
 
@@ -912,3 +914,62 @@ int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------- internal (driver) helpers --
+
+int evr::plan_b2g_device(evr_plan *p, int ncol, const double *X, double *G)
+{
+  if (!p || ncol < 0 || (ncol && (!X || !G))) return fail(EVR_ERR_ARG, "bad argument");
+  for (int c0 = 0; c0 < ncol; c0 += p->max_nvec) {
+    const int k = std::min(p->max_nvec, ncol - c0);
+    const int64_t n = p->nb_packed * k;
+    hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream,
+                       X + p->nb_packed * c0, p->nb_packed, k, p->d_X.ptr);
+    HIP_TRY(hipGetLastError());
+    int rc = run_terms<double>(p, evr::kModeB2G, k, p->d_X.ptr, nullptr, G + p->sum_nq * c0, nullptr,
+                               p->d_bw64.ptr, p->d_keo64.ptr, p->d_V.ptr);
+    if (rc) return rc;
+  }
+  return EVR_OK;
+}
+
+int evr::plan_grid_weights(evr_plan *p, const double **out)
+{
+  if (!p || !out) return fail(EVR_ERR_ARG, "bad argument");
+  if (!p->d_gw.ptr) {
+    // quadrature weight of 1D point q: W[0, q] / B[q, 0] (W = B^T diag(w), B[q, 0] > 0)
+    std::vector<double> bw(p->d_bw64.n);
+    HIP_TRY(hipMemcpy(bw.data(), p->d_bw64.ptr, sizeof(double) * bw.size(), hipMemcpyDeviceToHost));
+    std::vector<std::vector<double>> wq(p->slots.size());
+    for (size_t s = 0; s < p->slots.size(); ++s) {
+      const evr::SlotDev &sd = p->slots[s];
+      wq[s].resize(sd.nq);
+      for (int q = 0; q < sd.nq; ++q) wq[s][q] = bw[sd.offW + (int64_t)sd.nb * q] / bw[sd.offB + q];
+    }
+    std::vector<double> gw(p->sum_nq);
+    for (const evr::TermDev &td : p->terms) {
+      double *g = gw.data() + td.off_nq;
+      g[0] = td.w;
+      int64_t len = 1;  // first mode fastest: extend the tensor product mode by mode
+      for (int d = 0; d < p->D; ++d) {
+        const std::vector<double> &w = wq[td.slot[d]];
+        for (int q = (int)w.size() - 1; q >= 0; --q)
+          for (int64_t i = 0; i < len; ++i) g[q * len + i] = g[i] * w[q];
+        len *= (int64_t)w.size();
+      }
+    }
+    HIP_TRY(p->d_gw.upload(gw.data(), gw.size(), p->stream));
+  }
+  *out = p->d_gw.ptr;
+  return EVR_OK;
+}
+
+int evr::plan_grid_to_user(const evr_plan *p, const double *term_grid, double *user)
+{
+  if (!p) return fail(EVR_ERR_ARG, "bad argument");
+  if (p->dp_order)
+    for (int64_t i = 0; i < p->sum_nq; ++i) user[p->dp_perm[i]] = term_grid[i];
+  else
+    std::copy(term_grid, term_grid + p->sum_nq, user);
+  return EVR_OK;
+}

@@ -456,6 +456,86 @@ int launch_field(const Batch &P, const double *dE, const double *MW, double *HW)
   return EVR_OK;
 }
 
+int ChebBufs::alloc(int64_t nblk)
+{
+  for (DBuf *b : {&W1, &W2, &W3, &SAVE, &HW}) {
+    const hipError_t e = b->alloc(nblk);
+    if (e != hipSuccess) return set_error(EVR_ERR_HIP, std::string("Chebyshev buffers: ") + hipGetErrorString(e));
+  }
+  return EVR_OK;
+}
+
+int cheb_step(Batch &P, ChebBufs &cw, double *PSI, const double *dEp, double *MWp, double dt,
+              int npoly, double poly_tol, double &hmin, double &hmax, ChebRecord &rec, int &retries,
+              double *store, bool fixed)
+{
+  const int64_t nblk = P.nb * 2 * P.nwp;
+  const size_t bytes = sizeof(double) * nblk;
+  HTRY(hipMemcpyAsync(cw.SAVE.p, PSI, bytes, hipMemcpyDeviceToDevice, P.st));
+  for (int attempt = 0; attempt < 10; ++attempt) {
+    double E0, esc;
+    if (fixed) {
+      E0 = rec.e0;
+      esc = rec.esc;
+    } else {
+      const double dE_ = hmax - hmin;
+      E0 = hmin + 0.5 * dE_;  // MARCH:4211-4231
+      esc = 0.5 * dE_;
+    }
+    const int kmax = fixed ? rec.order : npoly - 1;
+    const std::vector<double> J = bessel_j(kmax, esc * dt);
+    // w_k lives at store + k*nblk, or in three rotating buffers
+    double *w0 = store ? store : cw.W1.p, *w1 = store ? store + nblk : cw.W2.p,
+           *w2 = store ? store + 2 * nblk : cw.W3.p;
+    // order 0: w0 = psi, psi = J_0 psi
+    HTRY(hipMemcpyAsync(w0, cw.SAVE.p, bytes, hipMemcpyDeviceToDevice, P.st));
+    HTRY(hipMemcpyAsync(PSI, cw.SAVE.p, bytes, hipMemcpyDeviceToDevice, P.st));
+    RTRY(P.scale(std::vector<cplx>(P.nwp, cplx(J[0], 0.0)), PSI));
+    // order 1: w1 = -i Hs w0, psi += 2 J_1 w1
+    RTRY(P.apply_field(w0, cw.HW.p, MWp, dEp));
+    RTRY(launch_cheb_order(P, dEp, E0, esc, nullptr, w0, cw.HW.p, MWp, w1, PSI, 2.0 * J[1]));
+    bool blow = false, conv = fixed;
+    int order = 1;
+    for (order = 2; order <= kmax; ++order) {
+      if (store) w2 = store + order * nblk;
+      RTRY(P.apply_field(w1, cw.HW.p, MWp, dEp));
+      RTRY(launch_cheb_order(P, dEp, E0, esc, w0, w1, cw.HW.p, MWp, w2, PSI, 2.0 * J[order]));
+      double *t = w0;
+      w0 = w1;
+      w1 = w2;
+      w2 = t;
+      if (fixed) continue;
+      // exit test |c_k| <w_k|w_k> (MARCH:4306-4335), over every wavepacket
+      std::vector<cplx> nn;
+      RTRY(P.dots(w1, 0, 1, w1, nn));
+      double crit = 0.0;
+      for (int w = 0; w < P.nwp; ++w) crit = std::max(crit, std::fabs(nn[w].real() * 2.0 * J[order]));
+      if (!(crit <= 1e15)) {
+        blow = true;
+        break;
+      }
+      if (crit < poly_tol) {
+        conv = true;
+        break;
+      }
+    }
+    if (fixed) order = kmax;
+    if (blow || !conv) {  // MARCH:4348-4373
+      hmax += 0.1 * (hmax - hmin);
+      ++retries;
+      continue;
+    }
+    // phase exp(-i E0 dt) (MARCH:4386-4388)
+    RTRY(P.scale(std::vector<cplx>(P.nwp, std::exp(cplx(0.0, -E0 * dt))), PSI));
+    rec.e0 = E0;
+    rec.esc = esc;
+    rec.order = order;
+    return EVR_OK;
+  }
+  return set_error(EVR_ERR_STATE, "Chebyshev expansion did not converge (Hmin " + std::to_string(hmin) +
+                                      ", Hmax " + std::to_string(hmax) + "): reduce dt or raise npoly");
+}
+
 }  // namespace propa
 }  // namespace evr
 
@@ -495,7 +575,8 @@ extern "C" int evr_propagate(evr_plan *H, const evr_propa_opts *o, int nwp, doub
     for (int64_t k = 0; k < (int64_t)o->nsteps * nwp; ++k) fmax = std::max(fmax, std::fabs(o->field[k]));
   double hmin = o->Hmin, hmax = o->Hmax;
   int retries = 0, max_order = 0;
-  DBuf W1, W2, W3, SAVE, Q;
+  ChebBufs cw;
+  DBuf Q;
   if (o->method == EVR_PROPA_CHEBY) {
     if (!(hmin < hmax)) {
       RTRY(lanczos_bounds(H, 60, &hmin, &hmax));
@@ -510,10 +591,7 @@ extern "C" int evr_propagate(evr_plan *H, const evr_propa_opts *o, int nwp, doub
       hmin -= fmax * mb;
       hmax += fmax * mb;
     }
-    HTRY(W1.alloc(nblk));
-    HTRY(W2.alloc(nblk));
-    HTRY(W3.alloc(nblk));
-    HTRY(SAVE.alloc(nblk));
+    RTRY(cw.alloc(nblk));
   } else {
     HTRY(Q.alloc(nblk * (o->npoly + 1)));
   }
@@ -527,57 +605,10 @@ extern "C" int evr_propagate(evr_plan *H, const evr_propa_opts *o, int nwp, doub
     double *MWp = o->field ? MW.p : nullptr;
     if (o->method == EVR_PROPA_CHEBY) {
       // ---- march_cheby (MARCH:4142-4433)
-      HTRY(hipMemcpyAsync(SAVE.p, PSI.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, P.st));
-      bool ok = false;
-      for (int attempt = 0; attempt < 10 && !ok; ++attempt) {
-        const double dE_ = hmax - hmin, E0 = hmin + 0.5 * dE_, esc = 0.5 * dE_;
-        const std::vector<double> J = bessel_j(o->npoly - 1, esc * o->dt);
-        // order 0: w0 = psi, psi = J_0 psi
-        double *w0 = W1.p, *w1 = W2.p, *w2 = W3.p;
-        HTRY(hipMemcpyAsync(w0, SAVE.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, P.st));
-        HTRY(hipMemcpyAsync(PSI.p, SAVE.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, P.st));
-        RTRY(P.scale(std::vector<cplx>(nwp, cplx(J[0], 0.0)), PSI.p));
-        // order 1: w1 = -i Hs w0, psi += 2 J_1 w1
-        RTRY(P.apply_field(w0, HW.p, MWp, dEp));
-        RTRY(launch_cheb_order(P, dEp, E0, esc, nullptr, w0, HW.p, MWp, w1, PSI.p, 2.0 * J[1]));
-        bool blow = false, conv = false;
-        int order = 1;
-        for (order = 2; order < o->npoly; ++order) {
-          RTRY(P.apply_field(w1, HW.p, MWp, dEp));
-          RTRY(launch_cheb_order(P, dEp, E0, esc, w0, w1, HW.p, MWp, w2, PSI.p, 2.0 * J[order]));
-          double *t = w0;
-          w0 = w1;
-          w1 = w2;
-          w2 = t;
-          // exit test |c_k| <w_k|w_k> (MARCH:4306-4335), over every wavepacket
-          std::vector<cplx> nn;
-          RTRY(P.dots(w1, 0, 1, w1, nn));
-          double crit = 0.0;
-          for (int w = 0; w < nwp; ++w)
-            crit = std::max(crit, std::fabs(nn[w].real() * 2.0 * J[order]));
-          if (!(crit <= 1e15)) {
-            blow = true;
-            break;
-          }
-          if (crit < o->poly_tol) {
-            conv = true;
-            break;
-          }
-        }
-        if (blow || !conv) {
-          hmax += 0.1 * (hmax - hmin);
-          ++retries;
-          continue;
-        }
-        max_order = std::max(max_order, order);
-        RTRY(P.scale(std::vector<cplx>(nwp, std::exp(cplx(0.0, -E0 * o->dt))), PSI.p));
-        ok = true;
-      }
-      if (!ok)
-        return set_error(EVR_ERR_STATE, "Chebyshev expansion did not converge at step " +
-                                            std::to_string(step) + " (Hmin " + std::to_string(hmin) +
-                                            ", Hmax " + std::to_string(hmax) +
-                                            "): reduce dt or raise npoly");
+      ChebRecord rec;
+      RTRY(cheb_step(P, cw, PSI.p, dEp, MWp, o->dt, o->npoly, o->poly_tol, hmin, hmax, rec, retries,
+                     nullptr, false));
+      max_order = std::max(max_order, rec.order);
     } else {
       // ---- march_SIL (MARCH:2899-3100)
       const int kmax = o->npoly;

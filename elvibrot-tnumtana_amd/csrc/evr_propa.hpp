@@ -74,5 +74,26 @@ int launch_cheb_order(const Batch &P, const double *dE, double e0, double esc, c
                       double c);
 int launch_field(const Batch &P, const double *dE, const double *MW, double *HW);
 
+// Work buffers of one Chebyshev step (nblk = nb * 2 nwp each).
+struct ChebBufs {
+  DBuf W1, W2, W3, SAVE, HW;
+  int alloc(int64_t nblk);
+};
+// What a step used: shift, scale and highest order.
+struct ChebRecord {
+  double e0 = 0.0, esc = 1.0;
+  int order = 0;
+};
+// One march_cheby step on the batch, PSI (device, nb x 2nwp) in place, field
+// values dE (device, per wavepacket) with MW a buffer for mu w (both NULL
+// without a field).  Adaptive (fixed == false): orders until the exit test,
+// Hmax widened and the step redone on failure; rec receives E0, Esc, order.
+// Fixed: exactly rec.order orders with rec's E0/Esc (a recomputation).  With
+// store != NULL, w_k is kept at store + k*nblk for k = 0..order (the store
+// holds npoly blocks in adaptive mode).
+int cheb_step(Batch &P, ChebBufs &cw, double *PSI, const double *dE, double *MW, double dt,
+              int npoly, double poly_tol, double &hmin, double &hmax, ChebRecord &rec, int &retries,
+              double *store, bool fixed);
+
 }  // namespace propa
 }  // namespace evr

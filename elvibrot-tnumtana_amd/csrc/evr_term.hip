@@ -585,7 +585,10 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           __syncthreads();
         }
         if (p.mode == kModeB2G) {
-          for (int i = tid; i < QA; i += NT) p.G[gbase + i] = cur[i * NVW];
+          for (int i2 = tid; i2 < QA * NVW; i2 += NT) {
+            const int i = i2 / NVW, c = i2 - i * NVW;
+            if (v0 + c < nvec) p.G[(int64_t)(v0 + c) * p.gld + gbase + i] = cur[i2];
+          }
         } else {
           for (int i2 = tid; i2 < QA * NVW; i2 += NT) cur[i2] *= p.V[gbase + i2 / NVW];
         }

@@ -12,7 +12,8 @@ import ctypes
 import numpy as np
 
 from . import hamiltonian  # noqa: F401
-from ._lib import BasisDesc, DavidsonInfo, DavidsonOpts, OpDesc, PropaInfo, PropaOpts, check, load
+from ._lib import (BasisDesc, ControlInfo, ControlOpts, DavidsonInfo, DavidsonOpts, OpDesc, PropaInfo,
+                   PropaOpts, check, load)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -37,6 +38,16 @@ def _to_pairs(psi):
 
 def _from_pairs(r):
     return r[:, 0::2] + 1j * r[:, 1::2]
+
+
+def control_field(amp, T, nsteps):
+    """Mid-step field values (nsteps, nf) of E(t) = sin^2(pi t/T) sum_k a_k sin(k pi t/T)."""
+    amp = np.asfortranarray(np.asarray(amp, dtype=np.float64).reshape(np.shape(amp)[0], -1))
+    nk, nf = amp.shape
+    o = ControlOpts(T, nsteps, nk, 0, 3, 0.0, 0.0, 0.0)
+    out = np.zeros((nsteps, nf))
+    check(load().evr_control_field(ctypes.byref(o), nf, _d(amp), _d(out)))
+    return out
 
 
 def bessel_j(n, r):
@@ -216,6 +227,33 @@ class Plan:
         if np.ndim(psi) == 1:
             out = out[:, 0]
         return out, (None if ac is None else ac[..., 0] + 1j * ac[..., 1]), info
+
+    def control_gradient(self, mu, amp, psi0, phi1, T, nsteps, checkpoint=32, npoly=200, poly_tol=1e-14,
+                         Hmin=0.0, Hmax=0.0, want=("a",)):
+        """J_f = |<phi1|psi_f(T)>|^2 for the fields of amp (nk x nf) and its gradients.
+
+        want: any of "a" (nk x nf), "E" (nsteps x nf), "V" (sum_nq x nf).
+        Returns (J, {name: gradient}, info)."""
+        amp = np.asfortranarray(np.asarray(amp, dtype=np.float64).reshape(np.shape(amp)[0], -1))
+        nk, nf = amp.shape
+        o = ControlOpts(T, nsteps, nk, checkpoint, npoly, poly_tol, Hmin, Hmax)
+        p0 = _to_pairs(np.asarray(psi0).reshape(-1, 1))
+        p1 = _to_pairs(np.asarray(phi1).reshape(-1, 1))
+        J = np.zeros(nf)
+        ga = np.zeros((nk, nf), order="F") if "a" in want else None
+        gE = np.zeros((nf, nsteps), order="F") if "E" in want else None
+        gV = np.zeros((self.basis.sum_nq, nf), order="F") if "V" in want else None
+        info = ControlInfo()
+        check(load().evr_control_gradient(self.h, mu.h, ctypes.byref(o), nf, _d(amp), _d(p0), _d(p1), _d(J),
+                                          _d(ga), _d(gE), _d(gV), ctypes.byref(info)))
+        out = {}
+        if ga is not None:
+            out["a"] = ga
+        if gE is not None:
+            out["E"] = gE.T.copy()   # (nsteps, nf)
+        if gV is not None:
+            out["V"] = gV
+        return J, out, info
 
     def BasisRep_TO_GridRep(self, psiB):
         psiB = np.ascontiguousarray(psiB, dtype=np.float64)

@@ -50,6 +50,18 @@ class PropaInfo(ctypes.Structure):
                 ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
+class ControlOpts(ctypes.Structure):
+    _fields_ = [("T", ctypes.c_double), ("nsteps", ctypes.c_int), ("nk", ctypes.c_int),
+                ("checkpoint", ctypes.c_int), ("npoly", ctypes.c_int), ("poly_tol", ctypes.c_double),
+                ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double)]
+
+
+class ControlInfo(ctypes.Structure):
+    _fields_ = [("hpsi", ctypes.c_int64), ("max_order", ctypes.c_int), ("retries", ctypes.c_int),
+                ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double), ("seconds_forward", ctypes.c_double),
+                ("seconds_backward", ctypes.c_double)]
+
+
 # name: (restype, argtypes)
 _SIGS = {
     "evr_last_error": (ctypes.c_char_p, []),
@@ -78,6 +90,10 @@ _SIGS = {
     "evr_propagate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PropaOpts), ctypes.c_int, ctypes.c_void_p,
                                      _dp, ctypes.POINTER(PropaInfo)]),
     "evr_plan_bounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "evr_control_field": (ctypes.c_int, [ctypes.POINTER(ControlOpts), ctypes.c_int, _dp, _dp]),
+    "evr_control_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ControlOpts),
+                                            ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                            ctypes.POINTER(ControlInfo)]),
     "evr_bessel_j": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _dp]),
 }
 

@@ -279,6 +279,48 @@ typedef struct evr_propa_info {
 int evr_propagate(evr_plan *H, const evr_propa_opts *opts, int nwp, double *psi,
                   double *autocorr, evr_propa_info *info);
 
+/* ---- optimal control: objective and adjoint gradient ----------------------- */
+/* nf fields, each E_f(t) = sin^2(pi t/T) sum_{k=1..nk} a_kf sin(k pi t/T),
+ * drive psi0 with H - E_f(t) mu by march_cheby steps of dt = T/nsteps (the
+ * field of step n is its value at (n + 1/2) dt).  J_f = |<phi1|psi_f(T)>|^2,
+ * the fidelity of calc_fidelity (CTRL:1000-1057).  The gradient of J_f with
+ * respect to a_kf, to the mid-step field values and to the grid values of V
+ * (the array of evr_plan_set_potential, so per term-grid point for SG4) is the
+ * exact derivative of the computed propagation: a backward (costate) sweep
+ * through the Chebyshev recurrences with the same H|psi>, from wavepackets
+ * stored every `checkpoint` steps and recomputed.  The reference optimises
+ * its field by Krotov / Zhu-Rabitz iterations instead (sub_Opt_control,
+ * CTRL:290-520); the expansion and the gradient have no counterpart there. */
+typedef struct evr_control_opts {
+  double T;                /* final time                                        */
+  int nsteps;              /* dt = T / nsteps                                   */
+  int nk;                  /* number of field coefficients a_k                  */
+  int checkpoint;          /* steps between stored wavepackets (0: 32)          */
+  int npoly;               /* max Chebyshev terms per step                      */
+  double poly_tol;
+  double Hmin, Hmax;       /* interval of H (Hmin >= Hmax: estimated); it is
+                              widened by max|E| * |mu|_max                      */
+} evr_control_opts;
+
+typedef struct evr_control_info {
+  int64_t hpsi;            /* real-column H|psi> applications (forward + backward) */
+  int max_order;
+  int retries;
+  double Hmin, Hmax;
+  double seconds_forward, seconds_backward;
+} evr_control_info;
+
+/* Field values at the mid-steps: field[n*nf + f] (amp: nk x nf, column per field). */
+int evr_control_field(const evr_control_opts *opts, int nf, const double *amp, double *field);
+/* H: symmetric plan (hermitian=1 or type 1); mu: plan of the coupling on the
+ * same basis (a type-0 plan whose V is mu(q)).  psi0, phi1: host nb_packed x 2
+ * (Re, Im columns), shared by all fields.  Outputs (host): J (nf); grad_a
+ * (nk x nf), grad_E (nsteps x nf: dJ/dE_n), grad_V (sum_nq x nf), each NULL
+ * when not wanted (with all three NULL only the forward sweep runs). */
+int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control_opts *opts, int nf,
+                         const double *amp, const double *psi0, const double *phi1, double *J,
+                         double *grad_a, double *grad_E, double *grad_V, evr_control_info *info);
+
 /* Extreme eigenvalues of a symmetric plan from `steps` Lanczos steps with full
  * re-orthogonalisation (min(steps, nb_packed) used), the estimate behind the
  * Chebyshev interval; the reference's sub_Hmax (HMAX:43-328) takes Hmin from
