@@ -31,6 +31,8 @@
  *   DAV   Source_ElVibRot/sub_propagation/sub_module_Davidson.f90
  *   MARCH Source_ElVibRot/sub_propagation/sub_module_propa_march.f90
  *   CTRL  Source_ElVibRot/sub_propagation/sub_control.f90
+ *   HMAX  Source_ElVibRot/sub_propagation/sub_Hmax.f90
+ *   POLY  Source_Lib/sub_communf90/sub_math/sub_polyortho.f90
  */
 #ifndef EVR_HIP_H
 #define EVR_HIP_H
@@ -324,7 +326,8 @@ int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control_opts *opts
 /* Extreme eigenvalues of a symmetric plan from `steps` Lanczos steps with full
  * re-orthogonalisation (min(steps, nb_packed) used), the estimate behind the
  * Chebyshev interval; the reference's sub_Hmax (HMAX:43-328) takes Hmin from
- * the grid potential and Hmax from a power iteration. */
+ * the minimum of the grid potential (HMAX:111-125) and Hmax from the last
+ * basis function's <e_N|H|e_N> (HMAX:206-217), widened by DHmax. */
 int evr_plan_bounds(evr_plan *p, int steps, double *Hmin, double *Hmax);
 
 /* J_0(r) .. J_n(r) (out: n + 1 values), the Bessel functions behind the
