@@ -54,6 +54,33 @@ def cpu_baseline(D, L, ham, nvec_cpu, stride, threads):
                       % (nvec_cpu, stride, len(terms), s.nterms, 100 * frac, dt)}
 
 
+def copy_calibration():
+    """1 GiB device-to-device copies: their FETCH_SIZE / WRITE_SIZE per dispatch
+    give the KiB-to-bytes scale of the HBM counters for a known traffic."""
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    summary (profiles/traffic_k_term_sg4_d9l8.md), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "traffic_k_term_sg4_d9l8.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\|\s*(\S+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|", line)
+        if m and m.group(1) == kernel:
+            return float(m.group(4))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,11 +93,16 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (scale of the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the H|psi> engine runs on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import numpy as np
     import torch
@@ -132,7 +164,8 @@ def main():
         log("  %-16s %8.3f ms/launch  %7.2f TFLOP/s" % (s[0], s[1] / max(s[3], 1),
                                                        s[2] / (s[1] / 1e3) / 1e12 if s[1] else 0))
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4),
+                "traffic": measured_traffic(name),
                 "kernel": name, "avg_launch_ms": round(ms / launches, 4),
                 "all_term_kernels_tflops": round(tot_fl / (tot_ms / 1e3) / 1e12, 3)}
 
