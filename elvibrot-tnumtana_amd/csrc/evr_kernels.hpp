@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace evr {
 
@@ -21,6 +22,17 @@ struct TermDev {
   double w;                   // Smolyak weight (SGT2:754-854)
   int slot[kMaxDev];          // 1D slot of each mode
   int step0;                  // first of the 2m slab contraction steps (StepDev table)
+  // Reduced grid of the H|psi> V part: one-function modes (nb = 1) are summed
+  // into V_eff once per potential (k_fold_v), so the slab walks only the Dv
+  // modes vmode[0..Dv-1] (split fields as above, over those modes).
+  int Dv;
+  int vmode[kMaxDev];
+  int vm, vnbA, vQA, vnbB, vQB, vstep0;
+  int64_t off_vq;             // start of the term in the concatenated V_eff
+  // Kinetic part: the na modes amode[] carry non-vanishing P, Q, R factors;
+  // the other modes (one function each) enter through scalar folds only.
+  int na;
+  int amode[kMaxDev];
 };
 
 // One contraction step inside a slab: out[i1 + n1*(q + nout*i3)] =

@@ -82,6 +82,7 @@ struct evr_plan {
   int D = 0, type = 0, max_nvec = 0;
   bool dp_order = false;
   int64_t nterms = 0, nb_packed = 0, sum_nb = 0, sum_nq = 0, max_nb = 0, max_nq = 0;
+  int64_t sum_vq = 0;  // reduced grid points (V_eff) of all terms
   hipStream_t stream = nullptr;
   bool hasV = false, hasKeo = false, hasA = false, hasF1 = false, diagT2 = false;
   // host copies
@@ -94,9 +95,11 @@ struct evr_plan {
   std::vector<evr::StepDev> steps;
   DevBuf<evr::StepDev> d_steps;
   DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag;
-  DevBuf<float> d_bw32, d_keo32, d_V32;
+  DevBuf<float> d_bw32, d_keo32;
   DevBuf<int> d_map, d_csr_ptr, d_csr_idx;
   DevBuf<double> d_V;
+  DevBuf<double> d_Veff;  // V summed over the one-function modes of each term
+  DevBuf<float> d_Veff32;
   DevBuf<double> d_coefC, d_coefA, d_coefF1, d_keoc;
   DevBuf<int> d_order[kNumClasses];
   int64_t nunits[kNumClasses] = {};
@@ -179,32 +182,41 @@ double term_flops(const evr_plan *p, const evr::TermDev &td)
   for (int d = 0; d < D; ++d) { nb[d] = p->slots[td.slot[d]].nb; nq[d] = p->slots[td.slot[d]].nq; }
   double f = 0.0;
   if (p->hasV) {
-    double leaf = 2.0 * td.nbA * td.nbB + 2.0 * td.nbt + (double)td.nbB * (D - td.m) * 2.0 + td.QA;
-    for (int d = 0; d < td.m; ++d) {
+    // reduced grid: the modes with more than one function
+    int vb[evr::kMaxDev], vq[evr::kMaxDev];
+    for (int j = 0; j < td.Dv; ++j) { vb[j] = nb[td.vmode[j]]; vq[j] = nq[td.vmode[j]]; }
+    double leaf = 2.0 * td.vnbA * td.vnbB + 2.0 * td.nbt + (double)td.vnbB * (td.Dv - td.vm) * 2.0 +
+                  td.vQA;
+    for (int d = 0; d < td.vm; ++d) {
       double n1 = 1, n3 = 1, m1 = 1, m3 = 1;
-      for (int e = 0; e < d; ++e) { n1 *= nq[e]; m1 *= nb[e]; }
-      for (int e = d + 1; e < td.m; ++e) { n3 *= nb[e]; m3 *= nq[e]; }
-      leaf += 2.0 * n1 * nb[d] * nq[d] * n3 + 2.0 * m1 * nq[d] * nb[d] * m3;
+      for (int e = 0; e < d; ++e) { n1 *= vq[e]; m1 *= vb[e]; }
+      for (int e = d + 1; e < td.vm; ++e) { n3 *= vb[e]; m3 *= vq[e]; }
+      leaf += 2.0 * n1 * vb[d] * vq[d] * n3 + 2.0 * m1 * vq[d] * vb[d] * m3;
     }
-    f += leaf * td.QB;
+    f += leaf * td.vQB;
   }
   if (p->hasKeo) {
-    double e = 0.0;
-    for (int j = 0; j < D; ++j) {
-      e += 2.0 * nb[j] + 2.0 * (D - 1);
+    // modes with non-vanishing P, Q, R; the others are scalar folds
+    const int na = td.na;
+    double e = 2.0;
+    for (int ja = 0; ja < na; ++ja) {
+      const int j = td.amode[ja];
+      e += 2.0 * nb[j] + 2.0 * (na - 1);
       if (p->hasA) {
-        e += 4.0 * nb[j] + 6.0 * (D - 1);
-        for (int k = 0; k < D; ++k)
-          if (k != j) e += 2.0 * nb[k] + 2.0 * (D - 2);
+        e += 4.0 * nb[j] + 6.0 * (na - 1) + 2.0;
+        for (int ka = 0; ka < na; ++ka)
+          if (ka != ja) e += 2.0 * nb[td.amode[ka]] + 2.0 * (na - 2);
       }
     }
-    for (int i = 0; i < D; ++i) {
+    for (int ia = 0; ia < na; ++ia) {
+      const int i = td.amode[ia];
       e += 4.0 * nb[i] + 2.0;
-      if (p->hasA) e += 4.0 * nb[i] + 2.0;
+      if (p->hasA) e += 4.0 * nb[i] + 4.0;
       if (p->hasF1) e += 2.0 * nb[i] + 2.0;
     }
     f += e * td.nbt;
   }
+  (void)D;
   return f;
 }
 
@@ -247,6 +259,17 @@ unsigned blocks_for(int64_t n)
 {
   int64_t b = (n + 255) / 256;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+// V_eff = V summed over the one-function modes of every term (k_fold_v),
+// queued on the plan stream after the V it reads.
+int fold_v(evr_plan *p)
+{
+  if (p->nterms == 0) return EVR_OK;
+  hipLaunchKernelGGL(evr::k_fold_v, dim3((unsigned)p->nterms), dim3(256), 0, p->stream, p->D,
+                     p->d_terms.ptr, p->d_slots.ptr, p->d_bw64.ptr, p->d_V.ptr, p->d_Veff.ptr);
+  HIP_TRY(hipGetLastError());
+  return EVR_OK;
 }
 
 }  // namespace
@@ -421,6 +444,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   std::vector<int64_t> ox, oB, oD;
   slot_offsets(bd, ox, oB, oD);
   std::vector<double> bw, keo, xs, bwdiag;
+  std::vector<char> slot_fold(ns, 0);  // one function with vanishing P, Q, R
   p->slots.resize(ns);
   for (int s = 0; s < ns; ++s) {
     const int nb = bd->nb[s], nq = bd->nq[s];
@@ -453,6 +477,15 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     if (op->hermitian) evr::symmetrize_factors(nb, f);
     sd.offK = (int64_t)keo.size();
     keo.insert(keo.end(), f.begin(), f.end());
+    if (nb == 1) {
+      // <0|d|0>, <0|x|0> and <0|d x|0> are integrals of odd functions: zero on
+      // a symmetric rule up to rounding; such a mode folds into scalars
+      double scale = 1.0;
+      for (int k : {evr::kT, evr::kS, evr::kU, evr::kT2}) scale = std::max(scale, std::fabs(f[k]));
+      const double tiny = 1e-13 * scale;
+      slot_fold[s] = std::fabs(f[evr::kP]) <= tiny && std::fabs(f[evr::kQ]) <= tiny &&
+                     std::fabs(f[evr::kR]) <= tiny;
+    }
   }
 
   // ---- operator coefficients in the kernel's (i, j, k) convention
@@ -482,7 +515,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
 
   // ---- terms
   p->terms.resize(p->nterms);
-  int64_t onb = 0, onq = 0;
+  int64_t onb = 0, onq = 0, onvq = 0;
   std::vector<int> cls(p->nterms);
   std::vector<double> cost(p->nterms);
   for (int64_t t = 0; t < p->nterms; ++t) {
@@ -514,60 +547,92 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     cls[t] = c;
     const int nvw = kClasses[c].nvw;
     const size_t keo_bufs = kClasses[c].ept <= 2 ? 3 : 1;  // LDS buffers of the KEO path
-    // largest slab (modes 0..m-1) that fits the LDS budget (double words)
-    const size_t budget = kLdsLimit / 8;
+    // mode lists: all modes (basis <-> grid transforms), the modes with more
+    // than one function (the H|psi> V part, one-function modes folded into
+    // V_eff), and the modes whose P, Q, R do not vanish (the kinetic part)
+    int fullm[evr::kMaxDev];
+    td.Dv = 0;
+    td.na = 0;
+    int64_t qv = 1;
+    for (int d = 0; d < D; ++d) {
+      fullm[d] = d;
+      if (nbd[d] > 1) {
+        td.vmode[td.Dv++] = d;
+        qv *= nqd[d];
+      }
+      if (!slot_fold[td.slot[d]]) td.amode[td.na++] = d;
+    }
+    for (int d = td.Dv; d < evr::kMaxDev; ++d) td.vmode[d] = 0;
+    for (int d = td.na; d < evr::kMaxDev; ++d) td.amode[d] = 0;
+    td.off_vq = onvq;
+    // largest slab (first m modes of a list) that fits the LDS budget (double
+    // words); then the 2m slab contraction steps, B->G then G->B
     const size_t base = td.lds_mats + (size_t)nbt * nvw;
-    int best = -1;
-    int64_t nbA = 1, QA = 1;
-    for (int m = 1; m <= D; ++m) {
-      nbA *= nbd[m - 1];
-      QA *= nqd[m - 1];
-      const int64_t nbB = nbt / nbA;
-      const size_t work = std::max<size_t>(p->hasKeo ? keo_bufs * nbt * nvw : 0,
-                                           2 * (size_t)QA * nvw + 2 * nbB);
-      if (base + work <= budget) best = m;
-    }
-    if (best < 1) return fail(EVR_ERR_SIZE, "term does not fit in LDS");
-    td.m = best;
-    td.nbA = 1;
-    td.QA = 1;
-    for (int d = 0; d < best; ++d) { td.nbA *= nbd[d]; td.QA *= nqd[d]; }
-    td.nbB = (int)(nbt / td.nbA);
-    td.QB = (int)(nqt / td.QA);
-    // slab contraction steps: B->G over modes 0..m-1, then G->B
-    td.step0 = (int)p->steps.size();
-    for (int d = 0; d < best; ++d) {
-      evr::StepDev sd{};
-      int n1 = 1, n3 = 1;
-      for (int e2 = 0; e2 < d; ++e2) n1 *= nqd[e2];
-      for (int e2 = d + 1; e2 < best; ++e2) n3 *= nbd[e2];
-      sd.n1 = n1; sd.n3 = n3; sd.nin = nbd[d]; sd.nout = nqd[d];
-      sd.mat = p->slots[td.slot[d]].offB;
-      p->steps.push_back(sd);
-    }
-    for (int d = 0; d < best; ++d) {
-      evr::StepDev sd{};
-      int n1 = 1, n3 = 1;
-      for (int e2 = 0; e2 < d; ++e2) n1 *= nbd[e2];
-      for (int e2 = d + 1; e2 < best; ++e2) n3 *= nqd[e2];
-      sd.n1 = n1; sd.n3 = n3; sd.nin = nqd[d]; sd.nout = nbd[d];
-      sd.mat = p->slots[td.slot[d]].offW;
-      p->steps.push_back(sd);
-    }
-    const size_t work = std::max<size_t>(p->hasKeo ? keo_bufs * nbt * nvw : 0,
-                                         2 * (size_t)td.QA * nvw + 2 * td.nbB);
-    const size_t words = base + work;
+    const size_t keo_words = p->hasKeo ? keo_bufs * nbt * nvw : 0;
+    // half the LDS when the term fits in it, so two workgroups share a CU
+    const size_t half = (kLdsLimit / 2 - 1024) / 8;
+    const size_t budget =
+        base + std::max<size_t>(keo_words, 2 * (size_t)nvw + 2 * nbt) <= half ? half : kLdsLimit / 8;
+    size_t words = base + keo_words;
+    auto split = [&](const int *ml, int nm, int &m_, int &nbA_, int &QA_, int &nbB_, int &QB_,
+                     int &step0) -> bool {
+      int best = -1;
+      int64_t nbA = 1, QA = 1;
+      for (int m = 0; m <= nm; ++m) {
+        if (m > 0) {
+          nbA *= nbd[ml[m - 1]];
+          QA *= nqd[ml[m - 1]];
+        }
+        const size_t work = std::max<size_t>(keo_words, 2 * (size_t)QA * nvw + 2 * (nbt / nbA));
+        if (base + work <= budget) best = m;
+      }
+      if (best < 0) return false;
+      m_ = best;
+      nbA_ = 1;
+      QA_ = 1;
+      int64_t qall = 1;
+      for (int k = 0; k < nm; ++k) qall *= nqd[ml[k]];
+      for (int k = 0; k < best; ++k) { nbA_ *= nbd[ml[k]]; QA_ *= nqd[ml[k]]; }
+      nbB_ = (int)(nbt / nbA_);
+      QB_ = (int)(qall / QA_);
+      step0 = (int)p->steps.size();
+      for (int k = 0; k < best; ++k) {
+        evr::StepDev sd{};
+        int n1 = 1, n3 = 1;
+        for (int e2 = 0; e2 < k; ++e2) n1 *= nqd[ml[e2]];
+        for (int e2 = k + 1; e2 < best; ++e2) n3 *= nbd[ml[e2]];
+        sd.n1 = n1; sd.n3 = n3; sd.nin = nbd[ml[k]]; sd.nout = nqd[ml[k]];
+        sd.mat = p->slots[td.slot[ml[k]]].offB;
+        p->steps.push_back(sd);
+      }
+      for (int k = 0; k < best; ++k) {
+        evr::StepDev sd{};
+        int n1 = 1, n3 = 1;
+        for (int e2 = 0; e2 < k; ++e2) n1 *= nbd[ml[e2]];
+        for (int e2 = k + 1; e2 < best; ++e2) n3 *= nqd[ml[e2]];
+        sd.n1 = n1; sd.n3 = n3; sd.nin = nqd[ml[k]]; sd.nout = nbd[ml[k]];
+        sd.mat = p->slots[td.slot[ml[k]]].offW;
+        p->steps.push_back(sd);
+      }
+      words = std::max(words, base + std::max<size_t>(keo_words, 2 * (size_t)QA_ * nvw + 2 * nbB_));
+      return true;
+    };
+    if (!split(fullm, D, td.m, td.nbA, td.QA, td.nbB, td.QB, td.step0) ||
+        !split(td.vmode, td.Dv, td.vm, td.vnbA, td.vQA, td.vnbB, td.vQB, td.vstep0))
+      return fail(EVR_ERR_SIZE, "term does not fit in LDS");
     p->lds_bytes64[c] = std::max(p->lds_bytes64[c], words * 8);
     p->lds_bytes32[c] = std::max(p->lds_bytes32[c], words * 4);
-    cost[t] = (double)nqt * 50.0 + (double)nbt * D * D * 6.0;
+    cost[t] = (double)qv * 50.0 + (double)nbt * (td.na + 1) * (td.na + 1) * 6.0;
     p->flops_class[c] += term_flops(p.get(), td);
     onb += nbt;
     onq += nqt;
+    onvq += qv;
     p->max_nb = std::max(p->max_nb, nbt);
     p->max_nq = std::max(p->max_nq, nqt);
   }
   p->sum_nb = onb;
   p->sum_nq = onq;
+  p->sum_vq = onvq;
   if (onb > INT32_MAX) return fail(EVR_ERR_SIZE, "sum of term basis sizes exceeds int32");
 
   // ---- per-class work lists, most expensive terms first
@@ -657,6 +722,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     HIP_TRY(p->d_keoc.upload(kc.data(), kc.size(), p->stream));
   }
   HIP_TRY(p->d_V.alloc(onq));
+  HIP_TRY(p->d_Veff.alloc(onvq));
   HIP_TRY(p->d_X.alloc((size_t)p->nb_packed * max_nvec));
   HIP_TRY(p->d_R.alloc((size_t)onb * max_nvec));
   HIP_TRY(p->d_psi.alloc((size_t)p->nb_packed * max_nvec));
@@ -666,6 +732,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     if (rc) return rc;
   } else {
     HIP_TRY(hipMemsetAsync(p->d_V.ptr, 0, sizeof(double) * onq, p->stream));
+    HIP_TRY(hipMemsetAsync(p->d_Veff.ptr, 0, sizeof(double) * onvq, p->stream));
   }
   HIP_TRY(hipStreamSynchronize(p->stream));
   *out = p.release();
@@ -677,8 +744,9 @@ int evr_plan_destroy(evr_plan *p)
   if (!p) return EVR_OK;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   p->d_terms.release(); p->d_slots.release(); p->d_steps.release(); p->d_bw64.release(); p->d_keo64.release();
-  p->d_x.release(); p->d_bwdiag.release(); p->d_bw32.release(); p->d_keo32.release(); p->d_V32.release();
+  p->d_x.release(); p->d_bwdiag.release(); p->d_bw32.release(); p->d_keo32.release();
   p->d_map.release(); p->d_csr_ptr.release(); p->d_csr_idx.release(); p->d_V.release();
+  p->d_Veff.release(); p->d_Veff32.release(); p->d_gw.release();
   p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release(); p->d_keoc.release();
   for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
   p->d_X.release(); p->d_R.release(); p->d_psi.release(); p->d_hpsi.release(); p->d_G.release();
@@ -701,6 +769,8 @@ int evr_plan_set_potential(evr_plan *p, const double *V)
   }
   HIP_TRY(hipMemcpyAsync(p->d_V.ptr, src, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice,
                          p->stream));
+  const int rc = fold_v(p);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
   const bool had = p->hasV;
   p->hasV = true;
@@ -724,6 +794,8 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
   hipLaunchKernelGGL(evr::k_gen_poly, dim3((unsigned)p->nterms), dim3(256), 0, p->stream, D,
                      p->d_terms.ptr, p->d_slots.ptr, p->d_x.ptr, dcoef.ptr, p->d_V.ptr);
   HIP_TRY(hipGetLastError());
+  const int rc = fold_v(p);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
   dcoef.release();
   p->hasV = true;
@@ -786,9 +858,9 @@ int evr_plan_kernel_stats(evr_plan *p, int max, char *names, double *ms, double 
 static int ensure_v32(evr_plan *p)
 {
   if (p->V32_valid) return EVR_OK;
-  if (!p->d_V32.ptr) HIP_TRY(p->d_V32.alloc(p->sum_nq));
-  hipLaunchKernelGGL(evr::k_to_float, dim3(blocks_for(p->sum_nq)), dim3(256), 0, p->stream,
-                     p->d_V.ptr, p->d_V32.ptr, p->sum_nq);
+  if (!p->d_Veff32.ptr) HIP_TRY(p->d_Veff32.alloc(std::max<int64_t>(p->sum_vq, 1)));
+  hipLaunchKernelGGL(evr::k_to_float, dim3(blocks_for(p->sum_vq)), dim3(256), 0, p->stream,
+                     p->d_Veff.ptr, p->d_Veff32.ptr, p->sum_vq);
   HIP_TRY(hipGetLastError());
   p->V32_valid = true;
   return EVR_OK;
@@ -815,7 +887,7 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
                        p->nb_packed, nvec, X);
     HIP_TRY(hipGetLastError());
     rc = run_terms<double>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw64.ptr,
-                           p->d_keo64.ptr, p->d_V.ptr);
+                           p->d_keo64.ptr, p->d_Veff.ptr);
     if (rc) return rc;
     hipLaunchKernelGGL(evr::k_reduce<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, R,
                        p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, dh);
@@ -828,7 +900,7 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
                        p->nb_packed, nvec, X);
     HIP_TRY(hipGetLastError());
     rc = run_terms<float>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw32.ptr,
-                          p->d_keo32.ptr, p->d_V32.ptr);
+                          p->d_keo32.ptr, p->d_Veff32.ptr);
     if (rc) return rc;
     hipLaunchKernelGGL(evr::k_reduce<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, R,
                        p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, dh);

@@ -237,7 +237,6 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   // wave-uniform scalar loads.
   if constexpr (EPT <= 2) {
     if (p.mode == kModeHpsi && p.hasKeo) {
-      constexpr int IG = EPT == 1 ? 12 : 5;
       T *zb = work;
       T *zb2 = work + nunit;
       T *zb3 = work + 2 * nunit;
@@ -256,13 +255,59 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           base[r][d] = d < D ? u - (int)((dig[r] >> (5 * d)) & 31) * s_st[d] : 0;
       }
       const int fdiag = p.diagT2 ? 6 : 2;
-      for (int i0 = 0; i0 < D; i0 += IG) {
+      // Modes outside amode[] have one function and vanishing P, Q, R (odd
+      // integrands on the symmetric quadrature): every product holding one of
+      // those factors drops, the others fold into scalars of this term:
+      //   c_ii Td_i + a_iii U_i           -> s_f0 x
+      //   a_iik Td_i Q_k  (k active)      -> s_fq[k] Q_k x
+      //   a_ijj P_i S_j   (i active)      -> s_fs[i] added to M_i
+      __shared__ T s_fq[kMaxDev], s_fs[kMaxDev], s_f0;
+      const int na = td.na;
+      if (tid == 0) {
+        bool act[kMaxDev];
+        for (int d = 0; d < kMaxDev; ++d) {
+          act[d] = false;
+          s_fq[d] = T(0);
+          s_fs[d] = T(0);
+        }
+        for (int a = 0; a < na; ++a) act[td.amode[a]] = true;
+        T f0 = T(0);
+        for (int i = 0; i < D; ++i) {
+          if (act[i]) continue;
+          const T *Ki = smem + s_offK[i];  // 1 x 1 factors
+          f0 += T(kCd[i]) * Ki[fdiag];
+          if (p.hasA) {
+            f0 += T(kU[i]) * Ki[5];
+            for (int k = 0; k < D; ++k)
+              if (act[k]) {
+                s_fq[k] += T(kO[k * kDp + i]) * Ki[fdiag];
+                s_fs[k] += T(kS[i * kDp + k]) * Ki[4];
+              }
+          }
+        }
+        s_f0 = f0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        const int u = tid + r * NT;
+        if (u < nunit) yown[r] = s_f0 * sx[u];
+      }
+      auto keo_groups = [&](auto igc) {
+      constexpr int IG = decltype(igc)::value;
+      for (int i0 = 0; i0 < na; i0 += IG) {
         T accM[IG][EPT], accN[IG][EPT], accO[IG][EPT];
 #pragma unroll
         for (int ii = 0; ii < IG; ++ii)
 #pragma unroll
           for (int r = 0; r < EPT; ++r) accM[ii][r] = accN[ii][r] = accO[ii][r] = T(0);
-        for (int j = 0; j < D; ++j) {
+        // strip column of each accumulator: the active mode, or the all-zero
+        // padding column kDp - 1 past the end of the list
+        int icol[IG];
+#pragma unroll
+        for (int ii = 0; ii < IG; ++ii) icol[ii] = i0 + ii < na ? td.amode[i0 + ii] : kDp - 1;
+        for (int ja = 0; ja < na; ++ja) {
+          const int j = td.amode[ja];
           const int nbj = s_nb[j], stj = s_st[j], nnj = nbj * nbj;
           const T *Kj = smem + s_offK[j];
 #pragma unroll
@@ -276,12 +321,13 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           __syncthreads();
           {
             T cz[IG], cs[IG], cn[IG], co[IG];
+            const T fq = s_fq[j];
 #pragma unroll
             for (int ii = 0; ii < IG; ++ii) {
-              cz[ii] = T(kA[(j * kDp + j) * kDp + i0 + ii]);
-              cs[ii] = T(kS[j * kDp + i0 + ii]);
-              cn[ii] = T(kN[j * kDp + i0 + ii]);
-              co[ii] = T(kO[j * kDp + i0 + ii]);
+              cz[ii] = T(kA[(j * kDp + j) * kDp + icol[ii]]);
+              cs[ii] = T(kS[j * kDp + icol[ii]]);
+              cn[ii] = T(kN[j * kDp + icol[ii]]);
+              co[ii] = T(kO[j * kDp + icol[ii]]);
             }
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {
@@ -300,18 +346,20 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
                     accN[ii][r] = fma(cn[ii], zj, accN[ii][r]);
                     accO[ii][r] = fma(co[ii], qxv, accO[ii][r]);
                   }
+                  if (i0 == 0) yown[r] = fma(fq, qxv, yown[r]);
                 }
               }
             }
           }
           if (p.hasA) {
-            for (int k = 0; k < D; ++k) {
+            for (int ka = 0; ka < na; ++ka) {
+              const int k = td.amode[ka];
               if (k == j) continue;
               const int nbk = s_nb[k], stk = s_st[k];
               const T *Qk = smem + s_offK[k] + nbk * nbk;
               T ca[IG];
 #pragma unroll
-              for (int ii = 0; ii < IG; ++ii) ca[ii] = T(kA[(j * kDp + k) * kDp + i0 + ii]);
+              for (int ii = 0; ii < IG; ++ii) ca[ii] = T(kA[(j * kDp + k) * kDp + icol[ii]]);
 #pragma unroll
               for (int r = 0; r < EPT; ++r) {
                 const int u = tid + r * NT;
@@ -329,14 +377,14 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         // y += P_i M_i + R_i N_i + Tdiag_i (O_i + c_ii x) + a_iii U_i x + F1_i P_i x
 #pragma unroll
         for (int ii = 0; ii < IG; ++ii) {
-          const int i = i0 + ii;
-          if (i >= D) break;
-          const T cd = T(kCd[i]), cu = T(kU[i]), cf = T(kF[i]);
+          if (i0 + ii >= na) break;
+          const int i = icol[ii];
+          const T cd = T(kCd[i]), cu = T(kU[i]), cf = T(kF[i]), fs = s_fs[i];
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
             const int u = tid + r * NT;
             if (u < nunit) {
-              zb[u] = accM[ii][r];
+              zb[u] = fma(fs, sx[u], accM[ii][r]);
               zb2[u] = accN[ii][r];
               zb3[u] = fma(cd, sx[u], accO[ii][r]);
             }
@@ -362,6 +410,15 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           }
           __syncthreads();
         }
+      }
+      };
+      // group size fitted to the active modes (no accumulators for absent ones)
+      if constexpr (EPT == 1) {
+        if (na <= 4) keo_groups(std::integral_constant<int, 4>{});
+        else keo_groups(std::integral_constant<int, 8>{});
+      } else {
+        if (na <= 3) keo_groups(std::integral_constant<int, 3>{});
+        else keo_groups(std::integral_constant<int, 5>{});
       }
     }
   } else {
@@ -505,13 +562,28 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   // ------------------------------------------------------------- V part --
   const bool vpart = (p.mode == kModeHpsi && p.hasV) || p.mode == kModeB2G || p.mode == kModeG2B;
   if (vpart) {
-    const int m = td.m, QA = td.QA, nbA = td.nbA, nbB = td.nbB;
+    // H|psi> walks the reduced grid (one-function modes folded into V_eff,
+    // p.V); the basis <-> grid transforms walk the full term grid.
+    const bool red = p.mode == kModeHpsi;
+    const int Dg = red ? td.Dv : D;
+    const int m = red ? td.vm : td.m, QA = red ? td.vQA : td.QA, nbA = red ? td.vnbA : td.nbA,
+              nbB = red ? td.vnbB : td.nbB, QBg = red ? td.vQB : td.QB;
+    const int64_t goff = red ? td.off_vq : td.off_nq;
     T *slab0 = work;
     T *slab1 = work + QA * NVW;
     T *crow = work + 2 * QA * NVW;
     T *crowW = crow + nbB;
     __shared__ int s_qd[kMaxDev];  // digits of the current outer grid point
+    __shared__ int sv_nb[kMaxDev], sv_nq[kMaxDev], sv_offB[kMaxDev], sv_offW[kMaxDev];
     if (tid < kMaxDev) s_qd[tid] = 0;
+    if (tid < Dg) {
+      const int d = red ? td.vmode[tid] : tid;
+      sv_nb[tid] = s_nb[d];
+      sv_nq[tid] = s_nq[d];
+      sv_offB[tid] = s_offB[d];
+      sv_offW[tid] = s_offW[d];
+    }
+    __syncthreads();
     // per-unit offsets of the outer accumulation (no division inside leaves)
     int aoff[EPT], boff[EPT];
 #pragma unroll
@@ -526,42 +598,42 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     uint64_t bdig = 0;
     {
       int rb = tid;
-      for (int d = m; d < D; ++d) {
-        const int nbd = s_nb[d];
+      for (int d = m; d < Dg; ++d) {
+        const int nbd = sv_nb[d];
         const int q = rb / nbd;
         bdig |= (uint64_t)(rb - q * nbd) << (5 * d);
         rb = q;
       }
     }
-    const StepDev *stp = p.steps + td.step0;
+    const StepDev *stp = p.steps + (red ? td.vstep0 : td.step0);
     __syncthreads();
-    for (int qB = 0; qB < td.QB; ++qB) {
+    for (int qB = 0; qB < QBg; ++qB) {
       for (int bB = tid; bB < nbB; bB += NT) {
         uint64_t bd64 = bdig;
         if (bB != tid) {  // more outer functions than threads: digits on the fly
           bd64 = 0;
           int rb = bB;
-          for (int d = m; d < D; ++d) {
-            const int nbd = s_nb[d];
+          for (int d = m; d < Dg; ++d) {
+            const int nbd = sv_nb[d];
             const int q = rb / nbd;
             bd64 |= (uint64_t)(rb - q * nbd) << (5 * d);
             rb = q;
           }
         }
         T c = T(1), cw = T(1);
-        for (int d = m; d < D; ++d) {
-          const int nqd = s_nq[d], nbd = s_nb[d];
+        for (int d = m; d < Dg; ++d) {
+          const int nqd = sv_nq[d], nbd = sv_nb[d];
           const int qd = s_qd[d];
           const int bd = (int)((bd64 >> (5 * d)) & 31);
-          c *= smem[s_offB[d] + qd + nqd * bd];
-          cw *= smem[s_offW[d] + bd + nbd * qd];
+          c *= smem[sv_offB[d] + qd + nqd * bd];
+          cw *= smem[sv_offW[d] + bd + nbd * qd];
         }
         crow[bB] = c;
         crowW[bB] = cw;
       }
       __syncthreads();
       T *cur = slab0, *nxt = slab1;
-      const int64_t gbase = td.off_nq + (int64_t)QA * qB;
+      const int64_t gbase = goff + (int64_t)QA * qB;
       if (p.mode != kModeG2B) {
         const int na = nbA * NVW;
         for (int a2 = tid; a2 < na; a2 += NT) {
@@ -612,8 +684,8 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       }
       // next outer grid point (digits of modes m..D-1, mode m fastest)
       if (tid == 0) {
-        for (int d = m; d < D; ++d) {
-          if (++s_qd[d] < s_nq[d]) break;
+        for (int d = m; d < Dg; ++d) {
+          if (++s_qd[d] < sv_nq[d]) break;
           s_qd[d] = 0;
         }
       }
@@ -710,6 +782,66 @@ __global__ __launch_bounds__(256) void k_gen_poly(int D, const TermDev *__restri
       }
     }
     V[td.off_nq + pnt] = val;
+  }
+}
+
+// V_eff of one term, one workgroup per term.  For a mode d with one basis
+// function the term's V part factors as W_d diag(.) B_d = sum_z W_d[0,z]
+// B_d[z,0] (.)_z, so
+//   V_eff(r) = sum_z prod_{d one-function} W_d[0,z_d] B_d[z_d,0] V(r, z)
+// over the grid points z of those modes, r running over the grid of the
+// other modes (first mode fastest in both).  The same discrete operator, with
+// the one-function sums taken once per potential instead of per H|psi>.
+__global__ __launch_bounds__(256) void k_fold_v(int D, const TermDev *__restrict__ terms,
+                                                const SlotDev *__restrict__ slots,
+                                                const double *__restrict__ bw,
+                                                const double *__restrict__ V, double *Veff)
+{
+  const TermDev &td = terms[blockIdx.x];
+  int nq[kMaxDev], zm[kMaxDev];
+  int64_t stride[kMaxDev];
+  int64_t st = 1;
+  for (int d = 0; d < D; ++d) {
+    nq[d] = slots[td.slot[d]].nq;
+    stride[d] = st;
+    st *= nq[d];
+  }
+  // one-function modes: the complement of vmode[] (both ascending)
+  int nz = 0, nzc = 1;
+  for (int d = 0, j = 0; d < D; ++d) {
+    if (j < td.Dv && td.vmode[j] == d) {
+      ++j;
+      continue;
+    }
+    zm[nz++] = d;
+    nzc *= nq[d];
+  }
+  int64_t qv = 1;
+  for (int j = 0; j < td.Dv; ++j) qv *= nq[td.vmode[j]];
+  for (int64_t r = threadIdx.x; r < qv; r += blockDim.x) {
+    int64_t rem = r, base = 0;
+    for (int j = 0; j < td.Dv; ++j) {
+      const int d = td.vmode[j];
+      const int64_t q = rem / nq[d];
+      base += (rem - q * nq[d]) * stride[d];
+      rem = q;
+    }
+    double acc = 0.0;
+    for (int z = 0; z < nzc; ++z) {
+      int rz = z;
+      int64_t idx = base;
+      double c = 1.0;
+      for (int k = 0; k < nz; ++k) {
+        const int d = zm[k];
+        const int iz = rz % nq[d];
+        rz /= nq[d];
+        idx += iz * stride[d];
+        const SlotDev &s = slots[td.slot[d]];
+        c *= bw[s.offW + iz] * bw[s.offB + iz];  // W[0, iz] B[iz, 0] (nb = 1)
+      }
+      acc = fma(c, V[td.off_nq + idx], acc);
+    }
+    Veff[td.off_vq + r] = acc;
   }
 }
 
