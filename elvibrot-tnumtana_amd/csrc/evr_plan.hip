@@ -64,8 +64,10 @@ struct KClass {
   int nt, ept, nvw;
   int max_nbt() const { return nt * ept / nvw; }
 };
-constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {1024, 1, 4}, {1024, 1, 2},
-                               {1024, 1, 1}, {1024, 2, 1}, {1024, 4, 1}, {1024, 8, 1}};
+// (nt, ept, nvw): ept == nvw > 1 is element-major (each thread the nvw vectors
+// of one basis function)
+constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {1024, 1, 4}, {512, 2, 2},
+                               {1024, 2, 2}, {1024, 2, 1}, {1024, 4, 1}, {1024, 8, 1}};
 constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
 constexpr size_t kLdsLimit = 160 * 1024 - 1024;
 
@@ -140,8 +142,8 @@ hipError_t launch_class(int c, const evr::TermParams<T> &prm, int64_t units, int
     case 0: return launch_one<T, 256, 1, 4>(prm, units, nvec, lds, s);
     case 1: return launch_one<T, 512, 1, 4>(prm, units, nvec, lds, s);
     case 2: return launch_one<T, 1024, 1, 4>(prm, units, nvec, lds, s);
-    case 3: return launch_one<T, 1024, 1, 2>(prm, units, nvec, lds, s);
-    case 4: return launch_one<T, 1024, 1, 1>(prm, units, nvec, lds, s);
+    case 3: return launch_one<T, 512, 2, 2>(prm, units, nvec, lds, s);
+    case 4: return launch_one<T, 1024, 2, 2>(prm, units, nvec, lds, s);
     case 5: return launch_one<T, 1024, 2, 1>(prm, units, nvec, lds, s);
     case 6: return launch_one<T, 1024, 4, 1>(prm, units, nvec, lds, s);
     case 7: return launch_one<T, 1024, 8, 1>(prm, units, nvec, lds, s);
@@ -259,6 +261,20 @@ unsigned blocks_for(int64_t n)
 {
   int64_t b = (n + 255) / 256;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+// Deterministic Smolyak scatter R -> out (nb_packed x nvec, column-major).
+template <typename T>
+hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out)
+{
+  const size_t lds = sizeof(double) * evr::kReduceTile * (size_t)(nvec + 1);
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_reduce<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const int64_t blocks = (p->nb_packed + evr::kReduceTile - 1) / evr::kReduceTile;
+  hipLaunchKernelGGL(evr::k_reduce<T>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), lds,
+                     p->stream, R, p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, out);
+  return hipGetLastError();
 }
 
 // V_eff = V summed over the one-function modes of every term (k_fold_v),
@@ -889,9 +905,7 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
     rc = run_terms<double>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw64.ptr,
                            p->d_keo64.ptr, p->d_Veff.ptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(evr::k_reduce<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, R,
-                       p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, dh);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
   } else {
     rc = ensure_v32(p);
     if (rc) return rc;
@@ -902,9 +916,7 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
     rc = run_terms<float>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw32.ptr,
                           p->d_keo32.ptr, p->d_Veff32.ptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(evr::k_reduce<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, R,
-                       p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, dh);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_reduce<float>(p, R, nvec, dh));
   }
   if (!dev) {
     HIP_TRY(hipMemcpyAsync(hpsi, dh, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
@@ -950,9 +962,7 @@ int evr_plan_diagonal(evr_plan *p, double *diag, int flags)
                              p->d_bwdiag.ptr, p->d_keo64.ptr, p->d_V.ptr, 1);
   if (rc) return rc;
   double *dst = (flags & EVR_DEVICE_PTRS) ? diag : p->d_hpsi.ptr;
-  hipLaunchKernelGGL(evr::k_reduce<double>, dim3(blocks_for(p->nb_packed)), dim3(256), 0, p->stream,
-                     p->d_R.ptr, p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, 1, dst);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_reduce<double>(p, p->d_R.ptr, 1, dst));
   if (!(flags & EVR_DEVICE_PTRS))
     HIP_TRY(hipMemcpyAsync(diag, dst, sizeof(double) * p->nb_packed, hipMemcpyDeviceToHost,
                            p->stream));
@@ -976,9 +986,7 @@ int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB)
   int rc = run_terms<double>(p, evr::kModeG2B, 1, nullptr, p->d_R.ptr, nullptr, p->d_G.ptr,
                              p->d_bw64.ptr, p->d_keo64.ptr, p->d_V.ptr);
   if (rc) return rc;
-  hipLaunchKernelGGL(evr::k_reduce<double>, dim3(blocks_for(p->nb_packed)), dim3(256), 0, p->stream,
-                     p->d_R.ptr, p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, 1, p->d_hpsi.ptr);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_reduce<double>(p, p->d_R.ptr, 1, p->d_hpsi.ptr));
   HIP_TRY(hipMemcpyAsync(psiB, p->d_hpsi.ptr, sizeof(double) * p->nb_packed, hipMemcpyDeviceToHost,
                          p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));

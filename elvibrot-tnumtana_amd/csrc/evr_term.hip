@@ -135,6 +135,16 @@ __device__ __forceinline__ void contract(const T *__restrict__ in, T *__restrict
   contract_gen<T, NT, NVW>(in, out, M, n1, nin, nout, n3);
 }
 
+// Unit u = e*NVW + v handled by thread tid in its r-th slot.  When each thread
+// holds as many units as the tile has vectors, it takes the NVW vectors of one
+// element (element-major: shared digits, factor rows and coefficient loads);
+// otherwise units are dealt round-robin over the threads.
+template <int NT, int EPT, int NVW>
+__device__ __forceinline__ int unit_of(int tid, int r)
+{
+  return (NVW > 1 && EPT == NVW) ? tid * EPT + r : tid + r * NT;
+}
+
 template <int EPT>
 struct IGroup {
   static constexpr int value = EPT >= 8 ? 1 : (EPT >= 4 ? 3 : (EPT >= 2 ? 6 : 12));
@@ -214,12 +224,15 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
 
   T yown[EPT];
   uint64_t dig[EPT];
+  constexpr bool kElementMajor = NVW > 1 && EPT == NVW;
 #pragma unroll
   for (int r = 0; r < EPT; ++r) {
     yown[r] = T(0);
     dig[r] = 0;
-    const int u = tid + r * NT;
-    if (u < nunit) {
+    const int u = unit_of<NT, EPT, NVW>(tid, r);
+    if (kElementMajor && r > 0) {
+      dig[r] = dig[0];  // same element, another vector
+    } else if (u < nunit) {
       int rem = u / NVW;
       for (int d = 0; d < D; ++d) {
         const int nbd = s_nb[d];
@@ -250,7 +263,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       int base[EPT][kMaxDev];
 #pragma unroll
       for (int r = 0; r < EPT; ++r) {
-        const int u = tid + r * NT;
+        const int u = unit_of<NT, EPT, NVW>(tid, r);
         for (int d = 0; d < kMaxDev; ++d)
           base[r][d] = d < D ? u - (int)((dig[r] >> (5 * d)) & 31) * s_st[d] : 0;
       }
@@ -290,7 +303,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < EPT; ++r) {
-        const int u = tid + r * NT;
+        const int u = unit_of<NT, EPT, NVW>(tid, r);
         if (u < nunit) yown[r] = s_f0 * sx[u];
       }
       auto keo_groups = [&](auto igc) {
@@ -312,7 +325,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           const T *Kj = smem + s_offK[j];
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
-            const int u = tid + r * NT;
+            const int u = unit_of<NT, EPT, NVW>(tid, r);
             if (u < nunit) {
               const int bj = (int)((dig[r] >> (5 * j)) & 31);
               zb[u] = apply_k(Kj + bj, sx + base[r][j], nbj, stj);  // Z_j = P_j x
@@ -331,7 +344,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
             }
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {
-              const int u = tid + r * NT;
+              const int u = unit_of<NT, EPT, NVW>(tid, r);
               if (u < nunit) {
                 const T zj = zb[u];
 #pragma unroll
@@ -362,7 +375,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
               for (int ii = 0; ii < IG; ++ii) ca[ii] = T(kA[(j * kDp + k) * kDp + icol[ii]]);
 #pragma unroll
               for (int r = 0; r < EPT; ++r) {
-                const int u = tid + r * NT;
+                const int u = unit_of<NT, EPT, NVW>(tid, r);
                 if (u < nunit) {
                   const int bk = (int)((dig[r] >> (5 * k)) & 31);
                   const T qz = apply_k(Qk + bk, zb + base[r][k], nbk, stk);  // Q_k Z_j
@@ -382,7 +395,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           const T cd = T(kCd[i]), cu = T(kU[i]), cf = T(kF[i]), fs = s_fs[i];
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
-            const int u = tid + r * NT;
+            const int u = unit_of<NT, EPT, NVW>(tid, r);
             if (u < nunit) {
               zb[u] = fma(fs, sx[u], accM[ii][r]);
               zb2[u] = accN[ii][r];
@@ -394,7 +407,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           const T *Ki = smem + s_offK[i];
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
-            const int u = tid + r * NT;
+            const int u = unit_of<NT, EPT, NVW>(tid, r);
             if (u < nunit) {
               const int bi = (int)((dig[r] >> (5 * i)) & 31);
               const int bo = base[r][i];
@@ -438,7 +451,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         const T *Kj = smem + s_offK[j];
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int u = tid + r * NT;
+          const int u = unit_of<NT, EPT, NVW>(tid, r);
           if (u < nunit) {
             const int bj = (int)((dig[r] >> (5 * j)) & 31);
             zb[u] = apply_mode(Kj + 0 * nbj * nbj, sx, u, bj, nbj, stj);  // P_j x
@@ -447,7 +460,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int u = tid + r * NT;
+          const int u = unit_of<NT, EPT, NVW>(tid, r);
           if (u < nunit) {
             const int bj = (int)((dig[r] >> (5 * j)) & 31);
             const T zj = zb[u];
@@ -495,13 +508,13 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         const int nn = nbi * nbi;
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int u = tid + r * NT;
+          const int u = unit_of<NT, EPT, NVW>(tid, r);
           if (u < nunit) zb[u] = accM[ii][r];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int u = tid + r * NT;
+          const int u = unit_of<NT, EPT, NVW>(tid, r);
           if (u < nunit)
             yown[r] += apply_mode(Ki + 0 * nn, zb, u, (int)((dig[r] >> (5 * i)) & 31), nbi, sti);
         }
@@ -509,13 +522,13 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         if (p.hasA) {
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
-            const int u = tid + r * NT;
+            const int u = unit_of<NT, EPT, NVW>(tid, r);
             if (u < nunit) zb[u] = accN[ii][r];
           }
           __syncthreads();
 #pragma unroll
           for (int r = 0; r < EPT; ++r) {
-            const int u = tid + r * NT;
+            const int u = unit_of<NT, EPT, NVW>(tid, r);
             if (u < nunit)
               yown[r] += apply_mode(Ki + 3 * nn, zb, u, (int)((dig[r] >> (5 * i)) & 31), nbi, sti);
           }
@@ -524,14 +537,14 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         const T cii = T(cC[i * D + i]);
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int u = tid + r * NT;
+          const int u = unit_of<NT, EPT, NVW>(tid, r);
           if (u < nunit) zb[u] = accO[ii][r] + cii * sx[u];
         }
         __syncthreads();
         const int fdiag = p.diagT2 ? 6 : 2;
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int u = tid + r * NT;
+          const int u = unit_of<NT, EPT, NVW>(tid, r);
           if (u < nunit) {
             const int bi = (int)((dig[r] >> (5 * i)) & 31);
             yown[r] += apply_mode(Ki + fdiag * nn, zb, u, bi, nbi, sti);
@@ -545,7 +558,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     if (p.hasF1) {
 #pragma unroll
       for (int r = 0; r < EPT; ++r) {
-        const int u = tid + r * NT;
+        const int u = unit_of<NT, EPT, NVW>(tid, r);
         if (u < nunit)
           for (int i = 0; i < D; ++i) {
             const int nbi = s_nb[i];
@@ -588,7 +601,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     int aoff[EPT], boff[EPT];
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
-      const int u = tid + r * NT;
+      const int u = unit_of<NT, EPT, NVW>(tid, r);
       const int e = u / NVW;
       const int bB = e / nbA;
       boff[r] = bB;
@@ -678,7 +691,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         }
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
-          const int u = tid + r * NT;
+          const int u = unit_of<NT, EPT, NVW>(tid, r);
           if (u < nunit) yown[r] = fma(crowW[boff[r]], cur[aoff[r]], yown[r]);
         }
       }
@@ -701,7 +714,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     const int fdiag = p.diagT2 ? 6 : 2;
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
-      const int u = tid + r * NT;
+      const int u = unit_of<NT, EPT, NVW>(tid, r);
       if (u < nunit)
         for (int i = 0; i < D; ++i) {
           const int nbi = s_nb[i], bi = (int)((dig[r] >> (5 * i)) & 31), nn = nbi * nbi;
@@ -714,7 +727,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   if (p.mode == kModeG2B && !p.diagKeo && fabs(td.w) < 1e-6) w = T(0);  // BGSG4:1004
 #pragma unroll
   for (int r = 0; r < EPT; ++r) {
-    const int u = tid + r * NT;
+    const int u = unit_of<NT, EPT, NVW>(tid, r);
     if (u < nunit) {
       const int e = u / NVW, c = u - e * NVW;
       if (v0 + c < nvec) p.R[(td.off_nb + e) * nvec + v0 + c] = w * yown[r];
@@ -724,19 +737,38 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
 
 // Deterministic weighted Smolyak scatter: hpsi[pk, v] = sum over the term
 // rows that map to pk (CSR, fixed order) of R.  Output column-major.
+// One workgroup per tile of kReduceTile packed functions: the sums are formed
+// with the vector index fastest (coalesced rows of R, four independent partial
+// sums combined in a fixed order) into an LDS tile, which is written out with
+// the packed index fastest (coalesced columns of the output).
+constexpr int kReduceTile = 64;
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce(const T *__restrict__ R, const int *__restrict__ ptr,
                                                 const int *__restrict__ idx, int64_t nb_packed,
                                                 int nvec, double *__restrict__ out)
 {
-  const int64_t n = nb_packed * nvec;
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    const int v = (int)(g % nvec);
-    const int64_t pk = g / nvec;
-    T s = T(0);
-    for (int r = ptr[pk]; r < ptr[pk + 1]; ++r) s += R[(int64_t)idx[r] * nvec + v];
-    out[(int64_t)v * nb_packed + pk] = (double)s;
+  extern __shared__ double tile[];  // kReduceTile x (nvec + 1)
+  const int64_t pk0 = (int64_t)blockIdx.x * kReduceTile;
+  const int np = nb_packed - pk0 < kReduceTile ? (int)(nb_packed - pk0) : kReduceTile;
+  const int ld = nvec + 1;
+  for (int g = threadIdx.x; g < np * nvec; g += blockDim.x) {
+    const int pl = g / nvec, v = g - pl * nvec;
+    const int r0 = ptr[pk0 + pl], r1 = ptr[pk0 + pl + 1];
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+    int r = r0;
+    for (; r + 4 <= r1; r += 4) {
+      s0 += R[(int64_t)idx[r] * nvec + v];
+      s1 += R[(int64_t)idx[r + 1] * nvec + v];
+      s2 += R[(int64_t)idx[r + 2] * nvec + v];
+      s3 += R[(int64_t)idx[r + 3] * nvec + v];
+    }
+    for (; r < r1; ++r) s0 += R[(int64_t)idx[r] * nvec + v];
+    tile[pl * ld + v] = (double)((s0 + s1) + (s2 + s3));
+  }
+  __syncthreads();
+  for (int g = threadIdx.x; g < np * nvec; g += blockDim.x) {
+    const int v = g / np, pl = g - v * np;
+    out[(int64_t)v * nb_packed + pk0 + pl] = tile[pl * ld + v];
   }
 }
 
