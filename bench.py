@@ -67,6 +67,42 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}), flush=True)
 
 
+def davidson_levels(args):
+    """The lowest --nev levels by block Davidson on the symmetric form of the
+    same operator, plain FP64 and FP32-refined (FP32 H|psi> until the largest
+    residual drops below --refine-resi, then FP64): max |dE| between them."""
+    import numpy as np
+    import evrhip
+    from evrhip import hamiltonian as H
+    D, L = args.D, args.L
+    cfg_index = 2 if (D, L) == (9, 8) else (1 if (D, L) == (6, 5) else 9)
+    ham = H.smolyak_config(cfg_index, D)
+    basis = evrhip.Basis.sg4(D, L)
+    plan = evrhip.Plan(basis, type=10, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=args.nvec, hermitian=True)
+    plan.gen_potential_poly(ham)
+    out = {}
+    for tag, prec in (("fp64", evrhip.FP64), ("fp32_refined", evrhip.FP32)):
+        t0 = time.perf_counter()
+        ev, _, info = plan.davidson(args.nev, block=args.nvec, conv_resi=args.conv_resi, conv_ene=1e-7,
+                                    max_it=args.max_it, precision=prec, refine_resi=args.refine_resi,
+                                    verbose=True)
+        wall = time.perf_counter() - t0
+        out[tag] = dict(ev=ev, it=info.iterations, conv=info.converged, h64=info.hpsi_fp64,
+                        h32=info.hpsi_fp32, resi=info.max_resi, s=wall)
+        log("%s: %d iterations, converged=%d, H|psi> fp64 %d fp32 %d, max residual %.2e, %.1f s"
+            % (tag, info.iterations, info.converged, info.hpsi_fp64, info.hpsi_fp32, info.max_resi, wall))
+    dE = np.abs(out["fp64"]["ev"] - out["fp32_refined"]["ev"])
+    line = {"workload": "SG4 D=%d L=%d symmetric type-10, %d lowest levels, block %d" % (D, L, args.nev, args.nvec),
+            "max_abs_dE_cm-1": float(dE.max()), "mean_abs_dE_cm-1": float(dE.mean()),
+            "E0_cm-1": float(out["fp64"]["ev"][0]), "E_last_cm-1": float(out["fp64"]["ev"][-1]),
+            "conv_resi": args.conv_resi}
+    for tag in ("fp64", "fp32_refined"):
+        o = out[tag]
+        line[tag] = {"iterations": o["it"], "converged": o["conv"], "hpsi_fp64": o["h64"],
+                     "hpsi_fp32": o["h32"], "max_residual": o["resi"], "seconds": round(o["s"], 2)}
+    print(json.dumps(line), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary (profiles/traffic_k_term_sg4_d9l8.md), or None."""
@@ -95,6 +131,12 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale of the HBM counters)")
+    ap.add_argument("--davidson", action="store_true",
+                    help="only run the FP64 and FP32-refined Davidson levels and their difference")
+    ap.add_argument("--nev", type=int, default=200)
+    ap.add_argument("--conv-resi", type=float, default=1e-3)
+    ap.add_argument("--refine-resi", type=float, default=1e-2)
+    ap.add_argument("--max-it", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the H|psi> engine runs on a single MI355X "
@@ -102,6 +144,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.davidson:
+        davidson_levels(args)
         return
 
     import numpy as np

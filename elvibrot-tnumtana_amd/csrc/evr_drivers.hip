@@ -366,6 +366,10 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
   int it = 0, nconv = 0;
   double maxres = 0.0;
   bool done = false;
+  // FP32 phase: also refine once the residual stops falling (the FP32 floor
+  // of H|psi>, which the Smolyak combination amplifies, is above refine_resi)
+  double best_res = 1e300;
+  int stall = 0;
   for (it = 1; it <= max_it; ++it) {
     // Rayleigh-Ritz in FP64
     RTRY(d.gemm_tn(m, m, V.p, HV.p, S.p, m));
@@ -413,7 +417,15 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
       if (o->verbose > 1)
         for (int j = 0; j < ne; ++j) fprintf(stderr, "   %3d %.10f %.3e\n", j, th[j], rn[j]);
     }
-    if (!refined && (maxres < o->refine_resi || nconv == ne)) {
+    if (!refined) {
+      if (maxres < 0.9 * best_res) {
+        best_res = maxres;
+        stall = 0;
+      } else {
+        ++stall;
+      }
+    }
+    if (!refined && (maxres < o->refine_resi || nconv == ne || stall >= 6)) {
       // FP64 refinement: rebuild H V in FP64, continue with FP64 H|psi>
       flags = EVR_FP64;
       refined = true;

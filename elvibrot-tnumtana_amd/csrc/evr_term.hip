@@ -754,16 +754,17 @@ __global__ __launch_bounds__(256) void k_reduce(const T *__restrict__ R, const i
   for (int g = threadIdx.x; g < np * nvec; g += blockDim.x) {
     const int pl = g / nvec, v = g - pl * nvec;
     const int r0 = ptr[pk0 + pl], r1 = ptr[pk0 + pl + 1];
-    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+    // FP64 sums also for FP32 term rows: the signed Smolyak weights cancel
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     int r = r0;
     for (; r + 4 <= r1; r += 4) {
-      s0 += R[(int64_t)idx[r] * nvec + v];
-      s1 += R[(int64_t)idx[r + 1] * nvec + v];
-      s2 += R[(int64_t)idx[r + 2] * nvec + v];
-      s3 += R[(int64_t)idx[r + 3] * nvec + v];
+      s0 += (double)R[(int64_t)idx[r] * nvec + v];
+      s1 += (double)R[(int64_t)idx[r + 1] * nvec + v];
+      s2 += (double)R[(int64_t)idx[r + 2] * nvec + v];
+      s3 += (double)R[(int64_t)idx[r + 3] * nvec + v];
     }
-    for (; r < r1; ++r) s0 += R[(int64_t)idx[r] * nvec + v];
-    tile[pl * ld + v] = (double)((s0 + s1) + (s2 + s3));
+    for (; r < r1; ++r) s0 += (double)R[(int64_t)idx[r] * nvec + v];
+    tile[pl * ld + v] = (s0 + s1) + (s2 + s3);
   }
   __syncthreads();
   for (int g = threadIdx.x; g < np * nvec; g += blockDim.x) {

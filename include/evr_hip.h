@@ -192,7 +192,8 @@ int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB);
  * (DAV:1214-1288), new vectors (DAV:1290-1588: NewVec_type 2 when precond=1,
  * type 4 otherwise), collapse (DAV:516-606).  Symmetric H expected (type 1, or
  * a plan created with hermitian=1).  precision EVR_FP32: FP32 H|psi> until the
- * largest residual drops below refine_resi, then H V is rebuilt in FP64 and the
+ * largest residual drops below refine_resi (or stops falling: 6 iterations
+ * without a 10% decrease, the FP32 floor), then H V is rebuilt in FP64 and the
  * iterations finish in FP64 (the FP64 refinement). */
 typedef struct evr_davidson_opts {
   int nev;                 /* levels wanted                                     */
