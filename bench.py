@@ -21,6 +21,7 @@ sys.path.insert(0, os.path.join(ROOT, "elvibrot-tnumtana_amd"))
 
 PEAK_FP64 = 78.6   # TFLOP/s, MI355X dense FP64 (matrix and vector are equal on gfx950)
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32
+PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
 
 
 def log(msg):
@@ -129,6 +130,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-assembled", action="store_true",
+                    help="skip the assembled-operator lines (needs sum nb_t^2 x 12 bytes of HBM)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (scale of the HBM counters)")
     ap.add_argument("--davidson", action="store_true",
@@ -214,6 +217,42 @@ def main():
                 "kernel": name, "avg_launch_ms": round(ms / launches, 4),
                 "all_term_kernels_tflops": round(tot_fl / (tot_ms / 1e3) / 1e12, 3)}
 
+    # assembled per-term operators: build once, then dense MFMA H_t X_t per step
+    asm = None
+    if not args.no_assembled:
+        try:
+            t0 = time.perf_counter()
+            plan.assemble(evrhip.ASM_FP64 | evrhip.ASM_FP32)
+            t_asm = time.perf_counter() - t0
+            mask, nbytes = plan.assembled()
+            mat64 = nbytes * 8.0 / 12.0 if mask == 3 else nbytes
+            log("assembled: %.1f GB (FP64 + FP32) in %.1f s" % (nbytes / 1e9, t_asm))
+            ams64, _ = timed(evrhip.FP64)
+            ams32, _ = timed(evrhip.FP32)
+            log("assembled fp64: %.3f ms per step, fp32: %.3f ms" % (ams64, ams32))
+            plan.set_profiling(True)
+            step(evrhip.FP64)
+            step(evrhip.FP64)
+            torch.cuda.synchronize()
+            st = [s for s in plan.kernel_stats() if s[0] == "k_apply_asm"][0]
+            plan.set_profiling(False)
+            per_ms = st[1] / st[3]
+            # algorithmic bytes per launch: the FP64 matrices once, the packed
+            # block in, the term rows out
+            nbytes_launch = mat64 + 8.0 * nvec * (basis.nb_packed + basis.sum_nb)
+            gbs = nbytes_launch / (per_ms / 1e3) / 1e9
+            asm = {"value_fp64": round(nvec / (ams64 / 1e3), 3), "ms_per_step_fp64": round(ams64, 4),
+                   "value_fp32": round(nvec / (ams32 / 1e3), 3), "ms_per_step_fp32": round(ams32, 4),
+                   "fp32_speedup_vs_fp64_matrix_free": round(ms64 / ams32, 3),
+                   "assembly_seconds": round(t_asm, 2), "device_gbytes": round(nbytes / 1e9, 2),
+                   "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM,
+                                "unit": "GB/s", "frac": round(gbs / PEAK_HBM, 4),
+                                "kernel": "k_apply_asm<double,2>", "avg_launch_ms": round(per_ms, 4),
+                                "mfma_tflops": round(st[2] / (st[1] / 1e3) / 1e12, 3)}}
+            plan.release_assembled()
+        except RuntimeError as e:
+            log("assembled operator skipped: %s" % e)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -242,6 +281,7 @@ def main():
                                            "ms_per_step": round(ms32, 4),
                                            "speedup_vs_fp64": round(ms64 / ms32, 3)},
         "roofline": roofline,
+        "assembled": asm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -33,7 +33,18 @@ struct TermDev {
   // the other modes (one function each) enter through scalar folds only.
   int na;
   int amode[kMaxDev];
+  // Assembled operator: start of the term's tiled nb_t x nb_t matrix (MFMA
+  // A-fragment order, see asm_index) in the assembled pools.
+  int64_t off_asm;
 };
+
+// Position of element (row, col) of a term's assembled matrix: fragments of
+// 16 rows x 4 columns, each 64 values in the MFMA A-operand lane order
+// (lane = 16 * (col % 4) + row % 16), fragments ordered [row / 16][col / 4].
+__host__ __device__ __forceinline__ int64_t asm_index(int row, int col, int ks)
+{
+  return ((int64_t)(row >> 4) * ks + (col >> 2)) * 64 + (col & 3) * 16 + (row & 15);
+}
 
 // One contraction step inside a slab: out[i1 + n1*(q + nout*i3)] =
 // sum_k M[q + nout*k] in[i1 + n1*(k + nin*i3)], M at pool offset `mat`.
@@ -53,7 +64,9 @@ struct SlotDev {
   int64_t offX;    // nq grid points (pool "x")
 };
 
-enum TermMode { kModeHpsi = 0, kModeB2G = 1, kModeG2B = 2 };
+// kModeAssemble: H|psi> of the term's own unit vectors e_c, c = c0 .. c0+NVW-1,
+// written as columns of the term's assembled matrix.
+enum TermMode { kModeHpsi = 0, kModeB2G = 1, kModeG2B = 2, kModeAssemble = 3 };
 
 template <typename T>
 struct TermParams {
@@ -79,6 +92,9 @@ struct TermParams {
   T *R;                       // term results, [sum nb_t][nvec]
   T *G;                       // grid output (B2G), column v at G + v*gld
   int64_t gld;                // column stride of G (sum nq_t)
+  const int2 *items;          // assemble: (term, first column) per workgroup
+  double *A64;                // assemble: FP64 pool (or NULL)
+  float *A32;                 // assemble: FP32 pool (or NULL)
   const T *Gin;               // grid input (G2B), [sum nq_t]
 };
 

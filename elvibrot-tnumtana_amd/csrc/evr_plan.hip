@@ -109,6 +109,14 @@ struct evr_plan {
   // work buffers
   DevBuf<double> d_X, d_R, d_psi, d_hpsi, d_G;
   DevBuf<double> d_gw;  // w_t * quadrature weight of every term-grid point (lazy)
+  // assembled per-term operators (evr_plan_assemble)
+  int64_t asm_elems = 0;           // sum over terms of the tiled nb_t x nb_t sizes
+  DevBuf<double> d_A64;
+  DevBuf<float> d_A32;
+  DevBuf<int2> d_asm_items;        // (term, first 16-row tile) per apply workgroup
+  int64_t n_asm_items = 0;
+  double asm_flops_per_vec = 0.0;  // 2 sum nb_t^2
+  double asm_bytes64 = 0.0;        // matrix bytes read per pass (FP64)
   bool V32_valid = false;
   // per-launch accounting: algorithmic flops per vector of each kernel class
   double flops_class[kNumClasses] = {};
@@ -275,6 +283,61 @@ hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out)
   hipLaunchKernelGGL(evr::k_reduce<T>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), lds,
                      p->stream, R, p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, out);
   return hipGetLastError();
+}
+
+// R = H_t X_t for every term from the assembled pool A (32 vectors per pass).
+template <typename T>
+hipError_t apply_assembled(evr_plan *p, const T *A, const T *X, int nvec, T *R)
+{
+  const int slot = kNumClasses;  // profiling slot of the assembled apply
+  if (p->profiling) (void)hipEventRecord(p->ev[2 * slot], p->stream);
+  for (int vb = 0; vb < nvec; vb += 32) {
+    const dim3 grid((unsigned)p->n_asm_items);
+    if (nvec - vb > 16)
+      hipLaunchKernelGGL((evr::k_apply_asm<T, 2>), grid, dim3(256), 0, p->stream, p->d_terms.ptr,
+                         p->d_asm_items.ptr, A, p->d_map.ptr, X, nvec, vb, R);
+    else
+      hipLaunchKernelGGL((evr::k_apply_asm<T, 1>), grid, dim3(256), 0, p->stream, p->d_terms.ptr,
+                         p->d_asm_items.ptr, A, p->d_map.ptr, X, nvec, vb, R);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (p->profiling) {
+    (void)hipEventRecord(p->ev[2 * slot + 1], p->stream);
+    (void)hipEventSynchronize(p->ev[2 * slot + 1]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p->ev[2 * slot], p->ev[2 * slot + 1]);
+    p->ms_acc[slot] += ms;
+    p->flops_acc[slot] += p->asm_flops_per_vec * nvec;
+    p->launches[slot] += 1;
+  }
+  return hipSuccess;
+}
+
+template <typename T, int NT, int EPT, int NVW>
+hipError_t launch_items_one(const evr::TermParams<T> &prm, int64_t nitems, size_t lds, hipStream_t s)
+{
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_term<T, NT, EPT, NVW>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL((evr::k_term<T, NT, EPT, NVW>), dim3((unsigned)nitems), dim3(NT), lds, s, prm);
+  return hipGetLastError();
+}
+
+hipError_t launch_items(int c, const evr::TermParams<double> &prm, int64_t nitems, size_t lds,
+                        hipStream_t s)
+{
+  switch (c) {
+    case 0: return launch_items_one<double, 256, 1, 4>(prm, nitems, lds, s);
+    case 1: return launch_items_one<double, 512, 1, 4>(prm, nitems, lds, s);
+    case 2: return launch_items_one<double, 1024, 1, 4>(prm, nitems, lds, s);
+    case 3: return launch_items_one<double, 512, 2, 2>(prm, nitems, lds, s);
+    case 4: return launch_items_one<double, 1024, 2, 2>(prm, nitems, lds, s);
+    case 5: return launch_items_one<double, 1024, 2, 1>(prm, nitems, lds, s);
+    case 6: return launch_items_one<double, 1024, 4, 1>(prm, nitems, lds, s);
+    case 7: return launch_items_one<double, 1024, 8, 1>(prm, nitems, lds, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // V_eff = V summed over the one-function modes of every term (k_fold_v),
@@ -581,6 +644,9 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     for (int d = td.Dv; d < evr::kMaxDev; ++d) td.vmode[d] = 0;
     for (int d = td.na; d < evr::kMaxDev; ++d) td.amode[d] = 0;
     td.off_vq = onvq;
+    td.off_asm = p->asm_elems;
+    p->asm_elems += ((nbt + 15) / 16) * ((nbt + 3) / 4) * 64;
+    p->asm_flops_per_vec += 2.0 * (double)nbt * (double)nbt;
     // largest slab (first m modes of a list) that fits the LDS budget (double
     // words); then the 2m slab contraction steps, B->G then G->B
     const size_t base = td.lds_mats + (size_t)nbt * nvw;
@@ -766,6 +832,7 @@ int evr_plan_destroy(evr_plan *p)
   p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release(); p->d_keoc.release();
   for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
   p->d_X.release(); p->d_R.release(); p->d_psi.release(); p->d_hpsi.release(); p->d_G.release();
+  p->d_A64.release(); p->d_A32.release(); p->d_asm_items.release();
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -787,6 +854,10 @@ int evr_plan_set_potential(evr_plan *p, const double *V)
                          p->stream));
   const int rc = fold_v(p);
   if (rc) return rc;
+  // an assembled operator holds the old V: drop it (evr_plan_assemble again)
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->d_A64.release();
+  p->d_A32.release();
   HIP_TRY(hipStreamSynchronize(p->stream));
   const bool had = p->hasV;
   p->hasV = true;
@@ -812,6 +883,10 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
   HIP_TRY(hipGetLastError());
   const int rc = fold_v(p);
   if (rc) return rc;
+  // an assembled operator holds the old V: drop it (evr_plan_assemble again)
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->d_A64.release();
+  p->d_A32.release();
   HIP_TRY(hipStreamSynchronize(p->stream));
   dcoef.release();
   p->hasV = true;
@@ -831,6 +906,86 @@ int evr_plan_sizes(const evr_plan *p, int64_t *out)
 void *evr_plan_stream(evr_plan *p) { return p ? (void *)p->stream : nullptr; }
 
 int evr_plan_max_nvec(const evr_plan *p) { return p ? p->max_nvec : fail(EVR_ERR_ARG, "NULL plan"); }
+
+int evr_plan_assemble(evr_plan *p, int which)
+{
+  if (!p) return fail(EVR_ERR_ARG, "NULL plan");
+  if (which < 1 || which > (EVR_ASM_FP64 | EVR_ASM_FP32))
+    return fail(EVR_ERR_ARG, "which must be EVR_ASM_FP64 and/or EVR_ASM_FP32");
+  const size_t n = (size_t)p->asm_elems;
+  if ((which & EVR_ASM_FP64) && !p->d_A64.ptr) {
+    if (p->d_A64.alloc(n) != hipSuccess) {
+      p->d_A64.ptr = nullptr;
+      return fail(EVR_ERR_SIZE, "not enough device memory for the FP64 assembled operator (" +
+                                    std::to_string(8.0 * n / 1e9) + " GB)");
+    }
+    HIP_TRY(hipMemsetAsync(p->d_A64.ptr, 0, sizeof(double) * n, p->stream));
+  }
+  if ((which & EVR_ASM_FP32) && !p->d_A32.ptr) {
+    if (p->d_A32.alloc(n) != hipSuccess) {
+      p->d_A32.ptr = nullptr;
+      return fail(EVR_ERR_SIZE, "not enough device memory for the FP32 assembled operator (" +
+                                    std::to_string(4.0 * n / 1e9) + " GB)");
+    }
+    HIP_TRY(hipMemsetAsync(p->d_A32.ptr, 0, sizeof(float) * n, p->stream));
+  }
+  // columns: the FP64 term kernel applied to each term's own unit vectors
+  for (int c = 0; c < kNumClasses; ++c) {
+    std::vector<int2> items;
+    for (int64_t t = 0; t < p->nterms; ++t) {
+      const int nbt = p->terms[t].nbt;
+      int cc = 0;
+      while (kClasses[cc].max_nbt() < nbt) ++cc;
+      if (cc != c) continue;
+      for (int c0 = 0; c0 < nbt; c0 += kClasses[c].nvw) items.push_back(int2{(int)t, c0});
+    }
+    if (items.empty()) continue;
+    DevBuf<int2> ditems;
+    HIP_TRY(ditems.upload(items.data(), items.size(), p->stream));
+    evr::TermParams<double> prm = base_params<double>(p, evr::kModeAssemble, 1);
+    prm.bw = p->d_bw64.ptr;
+    prm.keo = p->d_keo64.ptr;
+    prm.V = p->d_Veff.ptr;
+    prm.items = ditems.ptr;
+    prm.A64 = (which & EVR_ASM_FP64) ? p->d_A64.ptr : nullptr;
+    prm.A32 = (which & EVR_ASM_FP32) ? p->d_A32.ptr : nullptr;
+    HIP_TRY(launch_items(c, prm, (int64_t)items.size(), p->lds_bytes64[c], p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+  }
+  // apply work items: (term, first of 16 row tiles)
+  if (!p->d_asm_items.ptr) {
+    std::vector<int2> items;
+    for (int64_t t = 0; t < p->nterms; ++t) {
+      const int mtn = (p->terms[t].nbt + 15) / 16;
+      for (int m0 = 0; m0 < mtn; m0 += 16) items.push_back(int2{(int)t, m0});
+    }
+    HIP_TRY(p->d_asm_items.upload(items.data(), items.size(), p->stream));
+    p->n_asm_items = (int64_t)items.size();
+  }
+  p->asm_bytes64 = 8.0 * (double)n;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return EVR_OK;
+}
+
+int evr_plan_assembled(const evr_plan *p, double *bytes)
+{
+  if (!p) return fail(EVR_ERR_ARG, "NULL plan");
+  int m = 0;
+  double b = 0.0;
+  if (p->d_A64.ptr) { m |= EVR_ASM_FP64; b += 8.0 * p->asm_elems; }
+  if (p->d_A32.ptr) { m |= EVR_ASM_FP32; b += 4.0 * p->asm_elems; }
+  if (bytes) *bytes = b;
+  return m;
+}
+
+int evr_plan_release_assembled(evr_plan *p)
+{
+  if (!p) return fail(EVR_ERR_ARG, "NULL plan");
+  (void)hipStreamSynchronize(p->stream);
+  p->d_A64.release();
+  p->d_A32.release();
+  return EVR_OK;
+}
 
 static void recount_flops(evr_plan *p)
 {
@@ -867,6 +1022,13 @@ int evr_plan_kernel_stats(evr_plan *p, int max, char *names, double *ms, double 
     if (launches) launches[k] = p->launches[c];
     ++k;
   }
+  if (p->launches[kNumClasses] > 0 && k < max) {
+    if (names) snprintf(names + 64 * k, 64, "k_apply_asm");
+    if (ms) ms[k] = p->ms_acc[kNumClasses];
+    if (flops) flops[k] = p->flops_acc[kNumClasses];
+    if (launches) launches[k] = p->launches[kNumClasses];
+    ++k;
+  }
   *n = k;
   return EVR_OK;
 }
@@ -897,7 +1059,25 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
     dh = p->d_hpsi.ptr;
   }
   int rc;
-  if (!fp32) {
+  const bool use_asm =
+      !(flags & EVR_MATRIX_FREE) && (fp32 ? p->d_A32.ptr != nullptr : p->d_A64.ptr != nullptr);
+  if (use_asm) {
+    if (fp32) {
+      float *X = reinterpret_cast<float *>(p->d_X.ptr), *R = reinterpret_cast<float *>(p->d_R.ptr);
+      hipLaunchKernelGGL(evr::k_pack_in<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                         p->nb_packed, nvec, X);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(apply_assembled<float>(p, p->d_A32.ptr, X, nvec, R));
+      HIP_TRY(launch_reduce<float>(p, R, nvec, dh));
+    } else {
+      double *X = p->d_X.ptr, *R = p->d_R.ptr;
+      hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                         p->nb_packed, nvec, X);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(apply_assembled<double>(p, p->d_A64.ptr, X, nvec, R));
+      HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
+    }
+  } else if (!fp32) {
     double *X = p->d_X.ptr, *R = p->d_R.ptr;
     hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
                        p->nb_packed, nvec, X);

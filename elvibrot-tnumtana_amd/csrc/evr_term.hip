@@ -163,22 +163,32 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   // XCD-aware decode: consecutive blocks go round-robin over the 8 XCDs; all
   // vector tiles of one term land on the same XCD (shared V slab and
   // matrices in its L2) and the cost-sorted terms are dealt to the XCDs in turn.
-  const int nvt = (p.nvec + NVW - 1) / NVW;
-  const int64_t b = blockIdx.x;
-  const int xcd = (int)(b & 7);
-  const int64_t slot = b >> 3;
-  const int vt = (int)(slot % nvt);
-  const int64_t tr = (slot / nvt) * 8 + xcd;
-  if (tr >= p.nunits) return;
-  const int t = p.order[tr];
+  // Assembly: one workgroup per (term, first column) work item instead.
+  const bool asmb = p.mode == kModeAssemble;
+  const bool hmode = p.mode == kModeHpsi || asmb;  // the full operator is applied
+  int t, v0;
+  if (asmb) {
+    const int2 it = p.items[blockIdx.x];
+    t = it.x;
+    v0 = it.y;
+  } else {
+    const int nvt = (p.nvec + NVW - 1) / NVW;
+    const int64_t b = blockIdx.x;
+    const int xcd = (int)(b & 7);
+    const int64_t slot = b >> 3;
+    const int vt = (int)(slot % nvt);
+    const int64_t tr = (slot / nvt) * 8 + xcd;
+    if (tr >= p.nunits) return;
+    t = p.order[tr];
+    v0 = vt * NVW;
+  }
   if (t < 0) return;
   const TermDev &td = p.terms[t];
   const int D = p.D;
   const int tid = threadIdx.x;
   const int nbt = td.nbt;
   const int nunit = nbt * NVW;
-  const int v0 = vt * NVW;
-  const int64_t nvec = p.nvec;
+  const int64_t nvec = asmb ? nbt : p.nvec;  // assembly: columns of the term
 
   if (tid == 0) {
     int off = 0, st = NVW;
@@ -212,7 +222,12 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   }
   T *sx = smem + td.lds_mats;
   T *work = sx + nunit;
-  if (p.mode != kModeG2B) {
+  if (asmb) {
+    for (int u = tid; u < nunit; u += NT) {
+      const int e = u / NVW, c = u - e * NVW;
+      sx[u] = (e == v0 + c) ? T(1) : T(0);  // unit vector e_{v0+c} of the term
+    }
+  } else if (p.mode != kModeG2B) {
     const int *map = p.map + td.off_nb;
     for (int u = tid; u < nunit; u += NT) {
       const int e = u / NVW, c = u - e * NVW;
@@ -249,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   // pre-zeroed on the host (no branches, no index tests) and read with
   // wave-uniform scalar loads.
   if constexpr (EPT <= 2) {
-    if (p.mode == kModeHpsi && p.hasKeo) {
+    if (hmode && p.hasKeo) {
       T *zb = work;
       T *zb2 = work + nunit;
       T *zb3 = work + 2 * nunit;
@@ -435,7 +450,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       }
     }
   } else {
-  if (p.mode == kModeHpsi && p.hasKeo) {
+  if (hmode && p.hasKeo) {
     constexpr int IG = IGroup<EPT>::value;
     T *zb = work;
     const double *cC = p.coefC;
@@ -573,11 +588,11 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   }
 
   // ------------------------------------------------------------- V part --
-  const bool vpart = (p.mode == kModeHpsi && p.hasV) || p.mode == kModeB2G || p.mode == kModeG2B;
+  const bool vpart = (hmode && p.hasV) || p.mode == kModeB2G || p.mode == kModeG2B;
   if (vpart) {
     // H|psi> walks the reduced grid (one-function modes folded into V_eff,
     // p.V); the basis <-> grid transforms walk the full term grid.
-    const bool red = p.mode == kModeHpsi;
+    const bool red = hmode;
     const int Dg = red ? td.Dv : D;
     const int m = red ? td.vm : td.m, QA = red ? td.vQA : td.QA, nbA = red ? td.vnbA : td.nbA,
               nbB = red ? td.vnbB : td.nbB, QBg = red ? td.vQB : td.QB;
@@ -725,6 +740,24 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   }
   T w = T(td.w);
   if (p.mode == kModeG2B && !p.diagKeo && fabs(td.w) < 1e-6) w = T(0);  // BGSG4:1004
+  if (asmb) {
+    // column v0 + c of the weighted term matrix, in the tiled pools
+    const int ks = (nbt + 3) >> 2;
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      const int u = unit_of<NT, EPT, NVW>(tid, r);
+      if (u < nunit) {
+        const int e = u / NVW, c = u - e * NVW;
+        if (v0 + c < nbt) {
+          const int64_t o = td.off_asm + asm_index(e, v0 + c, ks);
+          const double val = (double)(w * yown[r]);
+          if (p.A64) p.A64[o] = val;
+          if (p.A32) p.A32[o] = (float)val;
+        }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < EPT; ++r) {
     const int u = unit_of<NT, EPT, NVW>(tid, r);
@@ -732,6 +765,96 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       const int e = u / NVW, c = u - e * NVW;
       if (v0 + c < nvec) p.R[(td.off_nb + e) * nvec + v0 + c] = w * yown[r];
     }
+  }
+}
+
+// Assembled H|psi>: R_t = H_t X_t for every term, X_t = the term's rows of the
+// packed block gathered through the map.  One workgroup of four waves per
+// (term, 256-row block); each wave owns four 16-row tiles and NTILE 16-vector
+// tiles of MFMA accumulators.  The term's X rows stream through LDS in chunks
+// of kAsmKc rows; the matrix fragments are read once, coalesced, from HBM.
+constexpr int kAsmKc = 128;
+template <typename T, int NTILE>
+__global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ terms,
+                                                   const int2 *__restrict__ items,
+                                                   const T *__restrict__ A, const int *__restrict__ map,
+                                                   const T *__restrict__ X, int nvec, int vbase,
+                                                   T *__restrict__ R)
+{
+  constexpr int NV = 16 * NTILE;
+  constexpr int LD = NV + 1;
+  __shared__ T xs[kAsmKc * LD];
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int nbt = td.nbt, mtn = (nbt + 15) >> 4, ks = (nbt + 3) >> 2;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int mt0 = it.y + 4 * wave;
+  const T *At = A + td.off_asm;
+  using acc_t = T __attribute__((ext_vector_type(4)));
+  acc_t acc[4][NTILE];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int n = 0; n < NTILE; ++n) acc[m][n] = acc_t{T(0), T(0), T(0), T(0)};
+  const int *mp = map + td.off_nb;
+  for (int kc0 = 0; kc0 < nbt; kc0 += kAsmKc) {
+    for (int i = threadIdx.x; i < kAsmKc * NV; i += 256) {
+      const int k = i / NV, v = i - k * NV, kg = kc0 + k;
+      T val = T(0);
+      if (kg < nbt && vbase + v < nvec) {
+        const int pk = mp[kg];
+        if (pk >= 0) val = X[(int64_t)pk * nvec + vbase + v];
+      }
+      xs[k * LD + v] = val;
+    }
+    __syncthreads();
+    const int s1 = min(ks, (kc0 + kAsmKc) >> 2);
+    // groups of four K-steps: the 16 matrix fragments of a group are requested
+    // together before their MFMAs, so each wave keeps 8 KB (FP64) in flight
+    for (int s = kc0 >> 2; s < s1; s += 4) {
+      T af[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int mt = mt0 + m;
+          af[q][m] = (mt < mtn && s + q < s1) ? At[((int64_t)mt * ks + s + q) * 64 + lane] : T(0);
+        }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (s + q >= s1) break;
+        const int kl = 4 * (s + q) - kc0 + (lane >> 4);
+        T bf[NTILE];
+#pragma unroll
+        for (int n = 0; n < NTILE; ++n) bf[n] = xs[kl * LD + 16 * n + (lane & 15)];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          if (mt0 + m >= mtn) continue;
+#pragma unroll
+          for (int n = 0; n < NTILE; ++n) {
+            if constexpr (sizeof(T) == 8)
+              acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][m], bf[n], acc[m][n], 0, 0, 0);
+            else
+              acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q][m], bf[n], acc[m][n], 0, 0, 0);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // D layout: f64 row (lane / 16) + 4 r, f32 row 4 (lane / 16) + r; column lane % 16
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int mt = mt0 + m;
+    if (mt >= mtn) continue;
+#pragma unroll
+    for (int n = 0; n < NTILE; ++n)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * mt + (sizeof(T) == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+        const int v = vbase + 16 * n + (lane & 15);
+        if (row < nbt && v < nvec) R[(td.off_nb + row) * nvec + v] = acc[m][n][r];
+      }
   }
 }
 

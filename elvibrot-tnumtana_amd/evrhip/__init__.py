@@ -21,6 +21,9 @@ _ip = ctypes.POINTER(ctypes.c_int)
 FP64 = 0
 FP32 = 1
 DEVICE_PTRS = 2
+MATRIX_FREE = 4
+ASM_FP64 = 1
+ASM_FP32 = 2
 CHEBY = 1
 SIL = 2
 
@@ -174,6 +177,21 @@ class Plan:
             nm = names.raw[64 * k:64 * (k + 1)].split(b"\0")[0].decode()
             out.append((nm, ms[k], fl[k], int(la[k])))
         return out
+
+    def assemble(self, which=ASM_FP64):
+        """Build the per-term dense operators (ASM_FP64 | ASM_FP32) used by TabOpPsi."""
+        check(load().evr_plan_assemble(self.h, which))
+
+    def assembled(self):
+        """(mask of assembled precisions, bytes held)."""
+        b = ctypes.c_double()
+        m = load().evr_plan_assembled(self.h, ctypes.byref(b))
+        if m < 0:
+            check(m)
+        return m, b.value
+
+    def release_assembled(self):
+        check(load().evr_plan_release_assembled(self.h))
 
     def diagonal(self):
         """Diagonal of H in the packed basis."""

@@ -89,6 +89,9 @@ _SIGS = {
     "evr_plan_max_nvec": (ctypes.c_int, [ctypes.c_void_p]),
     "evr_propagate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PropaOpts), ctypes.c_int, ctypes.c_void_p,
                                      _dp, ctypes.POINTER(PropaInfo)]),
+    "evr_plan_assemble": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "evr_plan_assembled": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "evr_plan_release_assembled": (ctypes.c_int, [ctypes.c_void_p]),
     "evr_plan_bounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
     "evr_control_field": (ctypes.c_int, [ctypes.POINTER(ControlOpts), ctypes.c_int, _dp, _dp]),
     "evr_control_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ControlOpts),

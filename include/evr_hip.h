@@ -53,6 +53,11 @@ extern "C" {
 #define EVR_FP64 0          /* all contractions in FP64                         */
 #define EVR_FP32 1          /* contractions in FP32, FP32 accumulation          */
 #define EVR_DEVICE_PTRS 2   /* psi/hpsi are device pointers on the plan stream  */
+#define EVR_MATRIX_FREE 4   /* ignore an assembled operator (sum-factorised path) */
+
+/* evr_plan_assemble precisions */
+#define EVR_ASM_FP64 1
+#define EVR_ASM_FP32 2
 
 /* ------------------------------------------------------------------------
  * Basis description: what the reference's SG4 basis type holds
@@ -153,6 +158,20 @@ int evr_plan_sizes(const evr_plan *p, int64_t *out);
 void *evr_plan_stream(evr_plan *p);
 /* Largest nvec one evr_oppsi call accepts (the max_nvec given at creation). */
 int evr_plan_max_nvec(const evr_plan *p);
+
+/* Assembled operator (no counterpart in the reference, which applies every
+ * term matrix-free on each call): every term's nb_t x nb_t matrix H_t (its
+ * kinetic and potential parts, Smolyak weight included) is built once by the
+ * FP64 term kernel applied to the term's unit vectors and kept on the device
+ * in MFMA fragment order, FP64 and/or FP32 (`which`: EVR_ASM_FP64 |
+ * EVR_ASM_FP32; sum_t nb_t^2 values each, e.g. 142 GB FP64 at D=9 L=8, so
+ * this trades HBM capacity for dense MFMA work).  evr_oppsi then computes
+ * R_t = H_t X_t with MFMA and the same scatter, in the precision asked,
+ * unless EVR_MATRIX_FREE is set.  Setting a new potential drops the assembled
+ * operator.  evr_plan_assembled returns the mask held and the bytes used. */
+int evr_plan_assemble(evr_plan *p, int which);
+int evr_plan_assembled(const evr_plan *p, double *bytes);
+int evr_plan_release_assembled(evr_plan *p);
 /* Profiling (no counterpart in the reference, whose analogue is the
  * nb_mult_BTOG/GTOB counters, SYS:120): when on, every term-kernel launch is
  * bracketed by HIP events on the plan stream; evr_plan_kernel_stats returns,
