@@ -797,7 +797,10 @@ __global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ t
 #pragma unroll
     for (int n = 0; n < NTILE; ++n) acc[m][n] = acc_t{T(0), T(0), T(0), T(0)};
   const int *mp = map + td.off_nb;
-  for (int kc0 = 0; kc0 < nbt; kc0 += kAsmKc) {
+  constexpr int spc = kAsmKc / 4;  // K-steps per staged chunk of X rows
+  // X rows [kc0, kc0 + kAsmKc) of the term into LDS (zero past nb_t / pruned)
+  auto stage = [&](int kc0) {
+    __syncthreads();  // the previous chunk is consumed
     for (int i = threadIdx.x; i < kAsmKc * NV; i += 256) {
       const int k = i / NV, v = i - k * NV, kg = kc0 + k;
       T val = T(0);
@@ -808,39 +811,53 @@ __global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ t
       xs[k * LD + v] = val;
     }
     __syncthreads();
-    const int s1 = min(ks, (kc0 + kAsmKc) >> 2);
-    // groups of four K-steps: the 16 matrix fragments of a group are requested
-    // together before their MFMAs, so each wave keeps 8 KB (FP64) in flight
-    for (int s = kc0 >> 2; s < s1; s += 4) {
-      T af[4][4];
+  };
+  // the 16 matrix fragments of K-steps s..s+3 (zero past the term)
+  auto load = [&](T(&af)[4][4], int s) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const int mt = mt0 + m;
-          af[q][m] = (mt < mtn && s + q < s1) ? At[((int64_t)mt * ks + s + q) * 64 + lane] : T(0);
-        }
+      for (int m = 0; m < 4; ++m) {
+        const int mt = mt0 + m;
+        af[q][m] = (mt < mtn && s + q < ks) ? At[((int64_t)mt * ks + s + q) * 64 + lane] : T(0);
+      }
+  };
+  auto compute = [&](const T(&af)[4][4], int s) {
+    const int kc0 = (s / spc) * kAsmKc;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (s + q >= s1) break;
-        const int kl = 4 * (s + q) - kc0 + (lane >> 4);
-        T bf[NTILE];
+    for (int q = 0; q < 4; ++q) {
+      if (s + q >= ks) break;
+      const int kl = 4 * (s + q) - kc0 + (lane >> 4);
+      T bf[NTILE];
 #pragma unroll
-        for (int n = 0; n < NTILE; ++n) bf[n] = xs[kl * LD + 16 * n + (lane & 15)];
+      for (int n = 0; n < NTILE; ++n) bf[n] = xs[kl * LD + 16 * n + (lane & 15)];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          if (mt0 + m >= mtn) continue;
+      for (int m = 0; m < 4; ++m) {
+        if (mt0 + m >= mtn) continue;
 #pragma unroll
-          for (int n = 0; n < NTILE; ++n) {
-            if constexpr (sizeof(T) == 8)
-              acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][m], bf[n], acc[m][n], 0, 0, 0);
-            else
-              acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q][m], bf[n], acc[m][n], 0, 0, 0);
-          }
+        for (int n = 0; n < NTILE; ++n) {
+          if constexpr (sizeof(T) == 8)
+            acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][m], bf[n], acc[m][n], 0, 0, 0);
+          else
+            acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q][m], bf[n], acc[m][n], 0, 0, 0);
         }
       }
     }
-    __syncthreads();
+  };
+  // groups of four K-steps, two groups in flight: the fragments of the next
+  // group are requested before the MFMAs of the current one
+  T a0[4][4], a1[4][4];
+  load(a0, 0);
+  for (int s = 0; s < ks;) {
+    if (s % spc == 0) stage(4 * s);
+    if (s + 4 < ks) load(a1, s + 4);
+    compute(a0, s);
+    s += 4;
+    if (s >= ks) break;
+    if (s % spc == 0) stage(4 * s);
+    if (s + 4 < ks) load(a0, s + 4);
+    compute(a1, s);
+    s += 4;
   }
   // D layout: f64 row (lane / 16) + 4 r, f32 row 4 (lane / 16) + r; column lane % 16
 #pragma unroll
@@ -864,7 +881,7 @@ __global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ t
 // with the vector index fastest (coalesced rows of R, four independent partial
 // sums combined in a fixed order) into an LDS tile, which is written out with
 // the packed index fastest (coalesced columns of the output).
-constexpr int kReduceTile = 64;
+constexpr int kReduceTile = 16;  // small tiles: many workgroups, rows in flight
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce(const T *__restrict__ R, const int *__restrict__ ptr,
                                                 const int *__restrict__ idx, int64_t nb_packed,
