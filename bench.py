@@ -260,15 +260,28 @@ def main():
         cpu = cpu_baseline(D, L, ham, 1, args.cpu_stride, threads)
         log("cpu: %.4g Hpsi/s per vector" % cpu["value"])
 
-    value = nvec / (ms64 / 1e3)
+    # headline: the fastest FP64 H|psi> of the plan (assembled operators when
+    # they fit in HBM, else the sum-factorised kernels); both paths are reported
+    matrix_free = {"value_fp64": round(nvec / (ms64 / 1e3), 3), "ms_per_step_fp64": round(ms64, 4),
+                   "value_fp32": None if ms32 is None else round(nvec / (ms32 / 1e3), 3),
+                   "ms_per_step_fp32": None if ms32 is None else round(ms32, 4),
+                   "roofline": roofline}
+    if asm is not None:
+        path = "assembled per-term operators (FP64 MFMA, %.0f GB FP64 + FP32 in HBM, built in %.1f s)" % (
+            asm["device_gbytes"], asm["assembly_seconds"])
+        h64, h32, head_roof = asm["ms_per_step_fp64"], asm["ms_per_step_fp32"], dict(asm["roofline"])
+        head_roof["traffic"] = measured_traffic("k_apply_asm")
+    else:
+        path = "sum-factorised term kernels (matrix-free)"
+        h64, h32, head_roof = ms64, ms32, roofline
     line = {
         "metric": "Hpsi/s per vector, D=%d SG4 L=%d" % (D, L),
-        "value": round(value, 3),
+        "value": round(nvec / (h64 / 1e3), 3),
         "unit": "Hpsi/s per vector",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms64, 4),
+        "ms_per_step": round(h64, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -276,11 +289,14 @@ def main():
         "data": "synthetic (seeded cubic/quartic potential, q-linear G, random normalised vectors)",
         "config": {"workload": "SG4 D=%d L=%d type-10 H|psi>, %d Smolyak terms, block of %d "
                                "vectors" % (D, L, basis.nterms, nvec),
-                   "nb_packed": basis.nb_packed, "grid_points": basis.sum_nq, "nvec": nvec},
-        "fp32": None if ms32 is None else {"value": round(nvec / (ms32 / 1e3), 3),
-                                           "ms_per_step": round(ms32, 4),
-                                           "speedup_vs_fp64": round(ms64 / ms32, 3)},
-        "roofline": roofline,
+                   "path": path, "nb_packed": basis.nb_packed, "grid_points": basis.sum_nq,
+                   "nvec": nvec},
+        "fp32": None if h32 is None else {"value": round(nvec / (h32 / 1e3), 3),
+                                          "ms_per_step": round(h32, 4),
+                                          "speedup_vs_fp64": round(h64 / h32, 3),
+                                          "speedup_vs_fp64_matrix_free": round(ms64 / h32, 3)},
+        "roofline": head_roof,
+        "matrix_free": matrix_free,
         "assembled": asm,
         "cpu_baseline": cpu,
     }

@@ -46,6 +46,11 @@ __host__ __device__ __forceinline__ int64_t asm_index(int row, int col, int ks)
   return ((int64_t)(row >> 4) * ks + (col >> 2)) * 64 + (col & 3) * 16 + (row & 15);
 }
 
+// K-steps (4 columns each) stored per 16-row tile of an nb_t x nb_t matrix:
+// rounded up to a multiple of 4 (zero columns), so the apply kernel reads
+// whole groups of four fragments without bounds tests.
+__host__ __device__ __forceinline__ int asm_ksteps(int nbt) { return (((nbt + 3) >> 2) + 3) & ~3; }
+
 // One contraction step inside a slab: out[i1 + n1*(q + nout*i3)] =
 // sum_k M[q + nout*k] in[i1 + n1*(k + nin*i3)], M at pool offset `mat`.
 struct StepDev {
@@ -83,13 +88,14 @@ struct TermParams {
   const T *bw;                // pool of B and W matrices
   const T *keo;               // pool of basis-space KEO factors
   const int *map;             // term-local -> packed
+  const int *rowpos;          // term row -> its row of R (scatter order), -1 if pruned
   const T *V;                 // grid potential (term-concatenated)
   const double *coefC;        // D x D: c[i*D+j]
   const double *coefA;        // D^3: a[(i*D+j)*D+k]
   const double *coefF1;       // D
   const double *keoc;         // pre-zeroed coefficient strips (see plan)
   const T *X;                 // packed input, [nb_packed][nvec]
-  T *R;                       // term results, [sum nb_t][nvec]
+  T *R;                       // term results in scatter order, [rows][nvec]
   T *G;                       // grid output (B2G), column v at G + v*gld
   int64_t gld;                // column stride of G (sum nq_t)
   const int2 *items;          // assemble: (term, first column) per workgroup

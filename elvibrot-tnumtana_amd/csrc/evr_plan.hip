@@ -98,7 +98,7 @@ struct evr_plan {
   DevBuf<evr::StepDev> d_steps;
   DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag;
   DevBuf<float> d_bw32, d_keo32;
-  DevBuf<int> d_map, d_csr_ptr, d_csr_idx;
+  DevBuf<int> d_map, d_csr_ptr, d_rowpos;
   DevBuf<double> d_V;
   DevBuf<double> d_Veff;  // V summed over the one-function modes of each term
   DevBuf<float> d_Veff32;
@@ -175,6 +175,7 @@ evr::TermParams<T> base_params(evr_plan *p, int mode, int nvec)
   prm.slots = p->d_slots.ptr;
   prm.steps = p->d_steps.ptr;
   prm.map = p->d_map.ptr;
+  prm.rowpos = p->d_rowpos.ptr;
   prm.coefC = p->d_coefC.ptr;
   prm.coefA = p->d_coefA.ptr;
   prm.coefF1 = p->d_coefF1.ptr;
@@ -281,7 +282,7 @@ hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out)
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int64_t blocks = (p->nb_packed + evr::kReduceTile - 1) / evr::kReduceTile;
   hipLaunchKernelGGL(evr::k_reduce<T>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), lds,
-                     p->stream, R, p->d_csr_ptr.ptr, p->d_csr_idx.ptr, p->nb_packed, nvec, out);
+                     p->stream, R, p->d_csr_ptr.ptr, p->nb_packed, nvec, out);
   return hipGetLastError();
 }
 
@@ -295,10 +296,10 @@ hipError_t apply_assembled(evr_plan *p, const T *A, const T *X, int nvec, T *R)
     const dim3 grid((unsigned)p->n_asm_items);
     if (nvec - vb > 16)
       hipLaunchKernelGGL((evr::k_apply_asm<T, 2>), grid, dim3(256), 0, p->stream, p->d_terms.ptr,
-                         p->d_asm_items.ptr, A, p->d_map.ptr, X, nvec, vb, R);
+                         p->d_asm_items.ptr, A, p->d_map.ptr, p->d_rowpos.ptr, X, nvec, vb, R);
     else
       hipLaunchKernelGGL((evr::k_apply_asm<T, 1>), grid, dim3(256), 0, p->stream, p->d_terms.ptr,
-                         p->d_asm_items.ptr, A, p->d_map.ptr, X, nvec, vb, R);
+                         p->d_asm_items.ptr, A, p->d_map.ptr, p->d_rowpos.ptr, X, nvec, vb, R);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -645,7 +646,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     for (int d = td.na; d < evr::kMaxDev; ++d) td.amode[d] = 0;
     td.off_vq = onvq;
     td.off_asm = p->asm_elems;
-    p->asm_elems += ((nbt + 15) / 16) * ((nbt + 3) / 4) * 64;
+    p->asm_elems += (int64_t)((nbt + 15) / 16) * evr::asm_ksteps(nbt) * 64;
     p->asm_flops_per_vec += 2.0 * (double)nbt * (double)nbt;
     // largest slab (first m modes of a list) that fits the LDS budget (double
     // words); then the 2m slab contraction steps, B->G then G->B
@@ -729,8 +730,10 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     HIP_TRY(p->d_order[c].upload(ids.data(), ids.size(), p->stream));
   }
 
-  // ---- deterministic scatter: CSR of term rows per packed function
-  std::vector<int> cnt(p->nb_packed + 1, 0), idx(onb);
+  // ---- deterministic scatter: the term rows of packed function pk, in
+  // increasing term order, are rows cnt[pk] .. cnt[pk+1]-1 of R; rowpos maps a
+  // term row to that position (-1 for pruned functions)
+  std::vector<int> cnt(p->nb_packed + 1, 0), rowpos(onb, -1);
   for (int64_t r = 0; r < onb; ++r) {
     const int mp = bd->map[r];
     if (mp >= p->nb_packed) return fail(EVR_ERR_ARG, "map entry beyond nb_packed");
@@ -741,7 +744,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     for (int64_t r = 0; r < onb; ++r) {
       const int mp = bd->map[r];
-      if (mp >= 0) idx[fill[mp]++] = (int)r;
+      if (mp >= 0) rowpos[r] = fill[mp]++;
     }
   }
 
@@ -773,7 +776,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   HIP_TRY(p->d_x.upload(xs.data(), xs.size(), p->stream));
   HIP_TRY(p->d_map.upload(bd->map, onb, p->stream));
   HIP_TRY(p->d_csr_ptr.upload(cnt.data(), cnt.size(), p->stream));
-  HIP_TRY(p->d_csr_idx.upload(idx.data(), idx.size(), p->stream));
+  HIP_TRY(p->d_rowpos.upload(rowpos.data(), rowpos.size(), p->stream));
   HIP_TRY(p->d_coefC.upload(cC.data(), cC.size(), p->stream));
   if (!cA.empty()) HIP_TRY(p->d_coefA.upload(cA.data(), cA.size(), p->stream));
   if (!cF1.empty()) HIP_TRY(p->d_coefF1.upload(cF1.data(), cF1.size(), p->stream));
@@ -827,7 +830,7 @@ int evr_plan_destroy(evr_plan *p)
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   p->d_terms.release(); p->d_slots.release(); p->d_steps.release(); p->d_bw64.release(); p->d_keo64.release();
   p->d_x.release(); p->d_bwdiag.release(); p->d_bw32.release(); p->d_keo32.release();
-  p->d_map.release(); p->d_csr_ptr.release(); p->d_csr_idx.release(); p->d_V.release();
+  p->d_map.release(); p->d_csr_ptr.release(); p->d_rowpos.release(); p->d_V.release();
   p->d_Veff.release(); p->d_Veff32.release(); p->d_gw.release();
   p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release(); p->d_keoc.release();
   for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
@@ -952,12 +955,43 @@ int evr_plan_assemble(evr_plan *p, int which)
     HIP_TRY(launch_items(c, prm, (int64_t)items.size(), p->lds_bytes64[c], p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
   }
-  // apply work items: (term, first of 16 row tiles)
+  // apply work items: (term, first of 16 row tiles).  Workgroups are handed to
+  // the 8 XCDs round-robin (workgroup i on XCD i % 8), so all items of a term
+  // go to one XCD, in consecutive turns: the term's X rows are then read from
+  // HBM once and from that XCD's L2 afterwards.  Terms are spread over the
+  // XCDs by matrix size, largest first.
   if (!p->d_asm_items.ptr) {
+    constexpr int kXcd = 8;
+    std::vector<int64_t> order(p->nterms);
+    for (int64_t t = 0; t < p->nterms; ++t) order[t] = t;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return p->terms[a].nbt > p->terms[b].nbt; });
+    std::vector<std::vector<int2>> queue(kXcd);
+    std::vector<double> bytes(kXcd, 0.0);
+    size_t total = 0;
+    for (int64_t t : order) {
+      const int nbt = p->terms[t].nbt, mtn = (nbt + 15) / 16;
+      const int x = (int)(std::min_element(bytes.begin(), bytes.end()) - bytes.begin());
+      for (int m0 = 0; m0 < mtn; m0 += 16) queue[x].push_back(int2{(int)t, m0});
+      bytes[x] += (double)mtn * evr::asm_ksteps(nbt);
+      total += (mtn + 15) / 16;
+    }
     std::vector<int2> items;
-    for (int64_t t = 0; t < p->nterms; ++t) {
-      const int mtn = (p->terms[t].nbt + 15) / 16;
-      for (int m0 = 0; m0 < mtn; m0 += 16) items.push_back(int2{(int)t, m0});
+    items.reserve(total);
+    std::vector<size_t> head(kXcd, 0);
+    while (items.size() < total) {
+      for (int x = 0; x < kXcd && items.size() < total; ++x) {
+        if (head[x] < queue[x].size()) {
+          items.push_back(queue[x][head[x]++]);
+        } else {
+          // this XCD's queue is empty: take the last item of the longest one
+          int y = 0;
+          for (int z = 1; z < kXcd; ++z)
+            if (queue[z].size() - head[z] > queue[y].size() - head[y]) y = z;
+          items.push_back(queue[y].back());
+          queue[y].pop_back();
+        }
+      }
     }
     HIP_TRY(p->d_asm_items.upload(items.data(), items.size(), p->stream));
     p->n_asm_items = (int64_t)items.size();

@@ -223,9 +223,12 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   T *sx = smem + td.lds_mats;
   T *work = sx + nunit;
   if (asmb) {
+    // unit vector e_{v0+c} of the term; a function pruned from the packed
+    // basis has no input value, so its column of the assembled matrix is zero
+    const int *map = p.map + td.off_nb;
     for (int u = tid; u < nunit; u += NT) {
       const int e = u / NVW, c = u - e * NVW;
-      sx[u] = (e == v0 + c) ? T(1) : T(0);  // unit vector e_{v0+c} of the term
+      sx[u] = (e == v0 + c && map[e] >= 0) ? T(1) : T(0);
     }
   } else if (p.mode != kModeG2B) {
     const int *map = p.map + td.off_nb;
@@ -742,7 +745,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   if (p.mode == kModeG2B && !p.diagKeo && fabs(td.w) < 1e-6) w = T(0);  // BGSG4:1004
   if (asmb) {
     // column v0 + c of the weighted term matrix, in the tiled pools
-    const int ks = (nbt + 3) >> 2;
+    const int ks = asm_ksteps(nbt);
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
       const int u = unit_of<NT, EPT, NVW>(tid, r);
@@ -763,77 +766,76 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     const int u = unit_of<NT, EPT, NVW>(tid, r);
     if (u < nunit) {
       const int e = u / NVW, c = u - e * NVW;
-      if (v0 + c < nvec) p.R[(td.off_nb + e) * nvec + v0 + c] = w * yown[r];
+      const int rp = p.rowpos[td.off_nb + e];
+      if (rp >= 0 && v0 + c < nvec) p.R[(int64_t)rp * nvec + v0 + c] = w * yown[r];
     }
   }
 }
 
 // Assembled H|psi>: R_t = H_t X_t for every term, X_t = the term's rows of the
 // packed block gathered through the map.  One workgroup of four waves per
-// (term, 256-row block); each wave owns four 16-row tiles and NTILE 16-vector
-// tiles of MFMA accumulators.  The term's X rows stream through LDS in chunks
-// of kAsmKc rows; the matrix fragments are read once, coalesced, from HBM.
+// (term, 256-row block); each wave owns up to four 16-row tiles (NM, a
+// compile-time count, so the inner loop has no bounds tests) and NTILE
+// 16-vector tiles of MFMA accumulators.  The term's X rows stream through LDS
+// in chunks of kAsmKc rows; the matrix fragments are read once, coalesced,
+// from HBM, one group of four K-steps ahead of the MFMAs that use them.
 constexpr int kAsmKc = 128;
-template <typename T, int NTILE>
-__global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ terms,
-                                                   const int2 *__restrict__ items,
-                                                   const T *__restrict__ A, const int *__restrict__ map,
-                                                   const T *__restrict__ X, int nvec, int vbase,
-                                                   T *__restrict__ R)
+template <typename T, int NTILE, int NM>
+__device__ __forceinline__ void asm_tiles(const T *__restrict__ At, const int *__restrict__ mp,
+                                          const T *__restrict__ X, T *__restrict__ xs, int nbt, int ks,
+                                          int mt0, int nvec, int vbase, const int *__restrict__ rpt,
+                                          T *__restrict__ R)
 {
-  constexpr int NV = 16 * NTILE;
-  constexpr int LD = NV + 1;
-  __shared__ T xs[kAsmKc * LD];
-  const int2 it = items[blockIdx.x];
-  const TermDev &td = terms[it.x];
-  const int nbt = td.nbt, mtn = (nbt + 15) >> 4, ks = (nbt + 3) >> 2;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int mt0 = it.y + 4 * wave;
-  const T *At = A + td.off_asm;
+  constexpr int NV = 16 * NTILE, LD = NV + 1, MA = NM > 0 ? NM : 1;
+  constexpr int spc = kAsmKc / 4;          // K-steps per staged chunk of X rows
+  constexpr int RPT = kAsmKc * NV / 256;   // X values staged per thread
+  constexpr int KST = 256 / NV;            // row stride between them
+  const int lane = threadIdx.x & 63;
   using acc_t = T __attribute__((ext_vector_type(4)));
-  acc_t acc[4][NTILE];
+  acc_t acc[MA][NTILE];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int m = 0; m < MA; ++m)
 #pragma unroll
     for (int n = 0; n < NTILE; ++n) acc[m][n] = acc_t{T(0), T(0), T(0), T(0)};
-  const int *mp = map + td.off_nb;
-  constexpr int spc = kAsmKc / 4;  // K-steps per staged chunk of X rows
-  // X rows [kc0, kc0 + kAsmKc) of the term into LDS (zero past nb_t / pruned)
+  // X rows [kc0, kc0 + kAsmKc) into LDS.  Rows past nb_t and pruned rows
+  // meet zero columns of the assembled matrix (padding, and pruned functions
+  // are assembled as zero columns), and vectors past nvec are not stored, so
+  // every read goes to a clamped valid address with no test: all RPT reads of
+  // a thread are in flight together.  (A non-finite value in row 0 of the
+  // block would reach the terms with pruned rows through 0 x inf.)
+  const int sv = threadIdx.x % NV, sk = threadIdx.x / NV;
+  const int vcol = vbase + sv < nvec ? vbase + sv : nvec - 1;
   auto stage = [&](int kc0) {
     __syncthreads();  // the previous chunk is consumed
-    for (int i = threadIdx.x; i < kAsmKc * NV; i += 256) {
-      const int k = i / NV, v = i - k * NV, kg = kc0 + k;
-      T val = T(0);
-      if (kg < nbt && vbase + v < nvec) {
-        const int pk = mp[kg];
-        if (pk >= 0) val = X[(int64_t)pk * nvec + vbase + v];
-      }
-      xs[k * LD + v] = val;
+    int pk[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int kg = kc0 + sk + KST * j;
+      pk[j] = mp[kg < nbt ? kg : nbt - 1];
     }
+    T val[RPT];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) val[j] = X[(int64_t)(pk[j] >= 0 ? pk[j] : 0) * nvec + vcol];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) xs[(sk + KST * j) * LD + sv] = val[j];
     __syncthreads();
   };
-  // the 16 matrix fragments of K-steps s..s+3 (zero past the term)
-  auto load = [&](T(&af)[4][4], int s) {
+  // the 4 x NM matrix fragments of K-steps s..s+3 (ks is a multiple of 4)
+  auto load = [&](T(&af)[4][MA], int s) {
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int mt = mt0 + m;
-        af[q][m] = (mt < mtn && s + q < ks) ? At[((int64_t)mt * ks + s + q) * 64 + lane] : T(0);
-      }
+      for (int m = 0; m < NM; ++m) af[q][m] = At[((int64_t)(mt0 + m) * ks + s + q) * 64 + lane];
   };
-  auto compute = [&](const T(&af)[4][4], int s) {
-    const int kc0 = (s / spc) * kAsmKc;
+  auto compute = [&](const T(&af)[4][MA], int s) {
+    const int kl0 = 4 * (s % spc) + (lane >> 4);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (s + q >= ks) break;
-      const int kl = 4 * (s + q) - kc0 + (lane >> 4);
       T bf[NTILE];
 #pragma unroll
-      for (int n = 0; n < NTILE; ++n) bf[n] = xs[kl * LD + 16 * n + (lane & 15)];
+      for (int n = 0; n < NTILE; ++n) bf[n] = xs[(kl0 + 4 * q) * LD + 16 * n + (lane & 15)];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        if (mt0 + m >= mtn) continue;
+      for (int m = 0; m < NM; ++m)
 #pragma unroll
         for (int n = 0; n < NTILE; ++n) {
           if constexpr (sizeof(T) == 8)
@@ -841,37 +843,72 @@ __global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ t
           else
             acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q][m], bf[n], acc[m][n], 0, 0, 0);
         }
-      }
     }
   };
-  // groups of four K-steps, two groups in flight: the fragments of the next
-  // group are requested before the MFMAs of the current one
-  T a0[4][4], a1[4][4];
+  // two groups in flight: the fragments of the next group are requested
+  // before the MFMAs of the current one (scheduling barriers keep that order);
+  // past the end the last group is requested again (cache hits) instead of a
+  // test, so the loads never sit behind a branch.  Chunks of X start at even
+  // groups (spc is a multiple of 8).
+  static_assert(spc % 8 == 0, "X chunks must start at even groups");
+  T a0[4][MA], a1[4][MA];
   load(a0, 0);
-  for (int s = 0; s < ks;) {
+  const int ng = ks >> 2;
+  int s = 0;
+  for (int g = 0; g + 1 < ng; g += 2, s += 8) {
     if (s % spc == 0) stage(4 * s);
-    if (s + 4 < ks) load(a1, s + 4);
+    load(a1, s + 4);
+    __builtin_amdgcn_sched_barrier(0);
     compute(a0, s);
-    s += 4;
-    if (s >= ks) break;
+    __builtin_amdgcn_sched_barrier(0);
+    load(a0, s + 8 < ks ? s + 8 : ks - 4);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(a1, s + 4);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (ng & 1) {
     if (s % spc == 0) stage(4 * s);
-    if (s + 4 < ks) load(a0, s + 4);
-    compute(a1, s);
-    s += 4;
+    compute(a0, s);
   }
   // D layout: f64 row (lane / 16) + 4 r, f32 row 4 (lane / 16) + r; column lane % 16
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int mt = mt0 + m;
-    if (mt >= mtn) continue;
+  for (int m = 0; m < NM; ++m)
 #pragma unroll
     for (int n = 0; n < NTILE; ++n)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 16 * mt + (sizeof(T) == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+        const int row = 16 * (mt0 + m) + (sizeof(T) == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
         const int v = vbase + 16 * n + (lane & 15);
-        if (row < nbt && v < nvec) R[(td.off_nb + row) * nvec + v] = acc[m][n][r];
+        const int rp = row < nbt ? rpt[row] : -1;  // row of R in scatter order
+        if (rp >= 0 && v < nvec) R[(int64_t)rp * nvec + v] = acc[m][n][r];
       }
+}
+
+template <typename T, int NTILE>
+__global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ terms,
+                                                   const int2 *__restrict__ items,
+                                                   const T *__restrict__ A, const int *__restrict__ map,
+                                                   const int *__restrict__ rowpos,
+                                                   const T *__restrict__ X, int nvec, int vbase,
+                                                   T *__restrict__ R)
+{
+  __shared__ T xs[kAsmKc * (16 * NTILE + 1)];
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int nbt = td.nbt, mtn = (nbt + 15) >> 4, ks = asm_ksteps(nbt);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int mt0 = it.y + 4 * wave;
+  const int nm = mtn - mt0 < 0 ? 0 : (mtn - mt0 > 4 ? 4 : mtn - mt0);
+  const T *At = A + td.off_asm;
+  const int *mp = map + td.off_nb;
+  const int *rpt = rowpos + td.off_nb;
+  // every wave runs the same chunk loop (same barriers), with its own tile count
+  switch (nm) {
+    case 4: asm_tiles<T, NTILE, 4>(At, mp, X, xs, nbt, ks, mt0, nvec, vbase, rpt, R); break;
+    case 3: asm_tiles<T, NTILE, 3>(At, mp, X, xs, nbt, ks, mt0, nvec, vbase, rpt, R); break;
+    case 2: asm_tiles<T, NTILE, 2>(At, mp, X, xs, nbt, ks, mt0, nvec, vbase, rpt, R); break;
+    case 1: asm_tiles<T, NTILE, 1>(At, mp, X, xs, nbt, ks, mt0, nvec, vbase, rpt, R); break;
+    default: asm_tiles<T, NTILE, 0>(At, mp, X, xs, nbt, ks, mt0, nvec, vbase, rpt, R); break;
   }
 }
 
@@ -884,8 +921,7 @@ __global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ t
 constexpr int kReduceTile = 16;  // small tiles: many workgroups, rows in flight
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce(const T *__restrict__ R, const int *__restrict__ ptr,
-                                                const int *__restrict__ idx, int64_t nb_packed,
-                                                int nvec, double *__restrict__ out)
+                                                int64_t nb_packed, int nvec, double *__restrict__ out)
 {
   extern __shared__ double tile[];  // kReduceTile x (nvec + 1)
   const int64_t pk0 = (int64_t)blockIdx.x * kReduceTile;
@@ -893,17 +929,22 @@ __global__ __launch_bounds__(256) void k_reduce(const T *__restrict__ R, const i
   const int ld = nvec + 1;
   for (int g = threadIdx.x; g < np * nvec; g += blockDim.x) {
     const int pl = g / nvec, v = g - pl * nvec;
-    const int r0 = ptr[pk0 + pl], r1 = ptr[pk0 + pl + 1];
+    // the term rows of packed function pk are rows ptr[pk] .. ptr[pk+1]-1 of
+    // R (written there by the term kernels, in increasing term order): the
+    // workgroup streams one contiguous stretch of R
+    const int r0 = ptr[pk0 + pl], n = ptr[pk0 + pl + 1] - r0;
+    const T *Rv = R + (int64_t)r0 * nvec + v;
     // FP64 sums also for FP32 term rows: the signed Smolyak weights cancel
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int r = r0;
-    for (; r + 4 <= r1; r += 4) {
-      s0 += (double)R[(int64_t)idx[r] * nvec + v];
-      s1 += (double)R[(int64_t)idx[r + 1] * nvec + v];
-      s2 += (double)R[(int64_t)idx[r + 2] * nvec + v];
-      s3 += (double)R[(int64_t)idx[r + 3] * nvec + v];
+    int i = 0;
+#pragma unroll 2
+    for (; i + 4 <= n; i += 4) {
+      s0 += (double)Rv[(int64_t)i * nvec];
+      s1 += (double)Rv[(int64_t)(i + 1) * nvec];
+      s2 += (double)Rv[(int64_t)(i + 2) * nvec];
+      s3 += (double)Rv[(int64_t)(i + 3) * nvec];
     }
-    for (; r < r1; ++r) s0 += (double)R[(int64_t)idx[r] * nvec + v];
+    for (; i < n; ++i) s0 += (double)Rv[(int64_t)i * nvec];
     tile[pl * ld + v] = (s0 + s1) + (s2 + s3);
   }
   __syncthreads();
