@@ -98,7 +98,12 @@ struct evr_plan {
   DevBuf<evr::StepDev> d_steps;
   DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag;
   DevBuf<float> d_bw32, d_keo32;
-  DevBuf<int> d_map, d_csr_ptr, d_rowpos;
+  DevBuf<int> d_map, d_rowpos;
+  // reduction segments: rows seg[s] .. seg[s+1]-1 of R; the segments of packed
+  // function pk are fseg[pk] .. fseg[pk+1]-1
+  DevBuf<int> d_seg, d_fseg;
+  int64_t nseg = 0;
+  DevBuf<double> d_P;  // segment sums, nseg x max_nvec
   DevBuf<double> d_V;
   DevBuf<double> d_Veff;  // V summed over the one-function modes of each term
   DevBuf<float> d_Veff32;
@@ -138,6 +143,8 @@ hipError_t launch_one(const evr::TermParams<T> &prm, int64_t units, int nvec, si
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int64_t nblk = units * ((nvec + NVW - 1) / NVW);
   if (nblk <= 0) return hipSuccess;
+  // the dispatch grid is counted in work-items in 32 bits
+  if (nblk > (int64_t)0xFFFFFFFFu / NT) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL((evr::k_term<T, NT, EPT, NVW>), dim3((unsigned)nblk), dim3(NT), lds, s, prm);
   return hipGetLastError();
 }
@@ -272,17 +279,30 @@ unsigned blocks_for(int64_t n)
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
 }
 
+// a[0 .. n) = 0 with a 64-bit fill kernel (the assembled pools are well past
+// 2^34 words)
+template <typename T>
+hipError_t zero_fill(T *a, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(evr::k_zero<T>, dim3(blocks_for(n)), dim3(256), 0, s, a, n);
+  return hipGetLastError();
+}
+
 // Deterministic Smolyak scatter R -> out (nb_packed x nvec, column-major).
 template <typename T>
 hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out)
 {
+  const int64_t sblocks = (p->nseg + evr::kReduceTile - 1) / evr::kReduceTile;
+  hipLaunchKernelGGL(evr::k_reduce_seg<T>, dim3((unsigned)std::max<int64_t>(sblocks, 1)), dim3(256), 0,
+                     p->stream, R, p->d_seg.ptr, p->nseg, nvec, p->d_P.ptr);
   const size_t lds = sizeof(double) * evr::kReduceTile * (size_t)(nvec + 1);
   if (lds > 64 * 1024)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_reduce<T>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_reduce_fn),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int64_t blocks = (p->nb_packed + evr::kReduceTile - 1) / evr::kReduceTile;
-  hipLaunchKernelGGL(evr::k_reduce<T>, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), lds,
-                     p->stream, R, p->d_csr_ptr.ptr, p->nb_packed, nvec, out);
+  hipLaunchKernelGGL(evr::k_reduce_fn, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), lds,
+                     p->stream, p->d_P.ptr, p->d_fseg.ptr, p->nb_packed, nvec, out);
   return hipGetLastError();
 }
 
@@ -320,9 +340,18 @@ hipError_t launch_items_one(const evr::TermParams<T> &prm, int64_t nitems, size_
 {
   (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_term<T, NT, EPT, NVW>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL((evr::k_term<T, NT, EPT, NVW>), dim3((unsigned)nitems), dim3(NT), lds, s, prm);
-  return hipGetLastError();
+  // the dispatch grid is counted in work-items in 32 bits: at most
+  // (2^32 - 1) / NT workgroups per launch, so long item lists go in pieces
+  constexpr int64_t kMaxBlocks = (int64_t)0xFFFFFFFFu / NT;
+  for (int64_t i0 = 0; i0 < nitems; i0 += kMaxBlocks) {
+    evr::TermParams<T> q = prm;
+    q.items = prm.items + i0;
+    const int64_t nb = std::min<int64_t>(kMaxBlocks, nitems - i0);
+    hipLaunchKernelGGL((evr::k_term<T, NT, EPT, NVW>), dim3((unsigned)nb), dim3(NT), lds, s, q);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_items(int c, const evr::TermParams<double> &prm, int64_t nitems, size_t lds,
@@ -775,7 +804,20 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   HIP_TRY(p->d_keo32.upload(keo32.data(), keo32.size(), p->stream));
   HIP_TRY(p->d_x.upload(xs.data(), xs.size(), p->stream));
   HIP_TRY(p->d_map.upload(bd->map, onb, p->stream));
-  HIP_TRY(p->d_csr_ptr.upload(cnt.data(), cnt.size(), p->stream));
+  {
+    std::vector<int> seg, fseg(p->nb_packed + 1);
+    seg.reserve(p->nb_packed + p->nb_packed / 8 + 1);
+    for (int64_t k = 0; k < p->nb_packed; ++k) {
+      fseg[k] = (int)seg.size();
+      for (int r = cnt[k]; r < cnt[k + 1]; r += evr::kReduceSeg) seg.push_back(r);
+    }
+    fseg[p->nb_packed] = (int)seg.size();
+    p->nseg = (int64_t)seg.size();
+    seg.push_back(cnt[p->nb_packed]);
+    HIP_TRY(p->d_seg.upload(seg.data(), seg.size(), p->stream));
+    HIP_TRY(p->d_fseg.upload(fseg.data(), fseg.size(), p->stream));
+    HIP_TRY(p->d_P.alloc((size_t)std::max<int64_t>(p->nseg, 1) * max_nvec));
+  }
   HIP_TRY(p->d_rowpos.upload(rowpos.data(), rowpos.size(), p->stream));
   HIP_TRY(p->d_coefC.upload(cC.data(), cC.size(), p->stream));
   if (!cA.empty()) HIP_TRY(p->d_coefA.upload(cA.data(), cA.size(), p->stream));
@@ -816,8 +858,8 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     int rc = evr_plan_set_potential(p.get(), op->V);
     if (rc) return rc;
   } else {
-    HIP_TRY(hipMemsetAsync(p->d_V.ptr, 0, sizeof(double) * onq, p->stream));
-    HIP_TRY(hipMemsetAsync(p->d_Veff.ptr, 0, sizeof(double) * onvq, p->stream));
+    HIP_TRY(zero_fill(p->d_V.ptr, onq, p->stream));
+    HIP_TRY(zero_fill(p->d_Veff.ptr, onvq, p->stream));
   }
   HIP_TRY(hipStreamSynchronize(p->stream));
   *out = p.release();
@@ -830,7 +872,7 @@ int evr_plan_destroy(evr_plan *p)
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   p->d_terms.release(); p->d_slots.release(); p->d_steps.release(); p->d_bw64.release(); p->d_keo64.release();
   p->d_x.release(); p->d_bwdiag.release(); p->d_bw32.release(); p->d_keo32.release();
-  p->d_map.release(); p->d_csr_ptr.release(); p->d_rowpos.release(); p->d_V.release();
+  p->d_map.release(); p->d_seg.release(); p->d_fseg.release(); p->d_P.release(); p->d_rowpos.release(); p->d_V.release();
   p->d_Veff.release(); p->d_Veff32.release(); p->d_gw.release();
   p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release(); p->d_keoc.release();
   for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
@@ -922,7 +964,7 @@ int evr_plan_assemble(evr_plan *p, int which)
       return fail(EVR_ERR_SIZE, "not enough device memory for the FP64 assembled operator (" +
                                     std::to_string(8.0 * n / 1e9) + " GB)");
     }
-    HIP_TRY(hipMemsetAsync(p->d_A64.ptr, 0, sizeof(double) * n, p->stream));
+    HIP_TRY(zero_fill(p->d_A64.ptr, (int64_t)n, p->stream));
   }
   if ((which & EVR_ASM_FP32) && !p->d_A32.ptr) {
     if (p->d_A32.alloc(n) != hipSuccess) {
@@ -930,7 +972,7 @@ int evr_plan_assemble(evr_plan *p, int which)
       return fail(EVR_ERR_SIZE, "not enough device memory for the FP32 assembled operator (" +
                                     std::to_string(4.0 * n / 1e9) + " GB)");
     }
-    HIP_TRY(hipMemsetAsync(p->d_A32.ptr, 0, sizeof(float) * n, p->stream));
+    HIP_TRY(zero_fill(p->d_A32.ptr, (int64_t)n, p->stream));
   }
   // columns: the FP64 term kernel applied to each term's own unit vectors
   for (int c = 0; c < kNumClasses; ++c) {
@@ -993,6 +1035,8 @@ int evr_plan_assemble(evr_plan *p, int which)
         }
       }
     }
+    if ((int64_t)items.size() > (int64_t)0xFFFFFFFFu / 256)
+      return fail(EVR_ERR_SIZE, "too many assembled work items for one launch");
     HIP_TRY(p->d_asm_items.upload(items.data(), items.size(), p->stream));
     p->n_asm_items = (int64_t)items.size();
   }
@@ -1169,7 +1213,7 @@ int evr_plan_diagonal(evr_plan *p, double *diag, int flags)
   const double *Gin = p->hasV ? p->d_V.ptr : nullptr;
   if (!Gin) {
     if (!p->d_G.ptr) HIP_TRY(p->d_G.alloc(p->sum_nq));
-    HIP_TRY(hipMemsetAsync(p->d_G.ptr, 0, sizeof(double) * p->sum_nq, p->stream));
+    HIP_TRY(zero_fill(p->d_G.ptr, p->sum_nq, p->stream));
     Gin = p->d_G.ptr;
   }
   int rc = run_terms<double>(p, evr::kModeG2B, 1, nullptr, p->d_R.ptr, nullptr, Gin,

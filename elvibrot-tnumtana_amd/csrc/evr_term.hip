@@ -913,15 +913,48 @@ __global__ __launch_bounds__(256) void k_apply_asm(const TermDev *__restrict__ t
 }
 
 // Deterministic weighted Smolyak scatter: hpsi[pk, v] = sum over the term
-// rows that map to pk (CSR, fixed order) of R.  Output column-major.
-// One workgroup per tile of kReduceTile packed functions: the sums are formed
-// with the vector index fastest (coalesced rows of R, four independent partial
-// sums combined in a fixed order) into an LDS tile, which is written out with
-// the packed index fastest (coalesced columns of the output).
-constexpr int kReduceTile = 16;  // small tiles: many workgroups, rows in flight
+// rows that map to pk (fixed order) of R.  Output column-major.
+//
+// The term kernels write the rows of packed function pk, in increasing term
+// order, to one contiguous stretch of R.  The stretches are very uneven (at
+// D=9 L=8 the median function has 1 row, the ground function 24,310: it sits
+// in every term), so each is cut into segments of at most kReduceSeg rows and
+// the sum runs in two passes with a fixed order: segment sums (four
+// interleaved partial sums), then the segments of each function in order.
+constexpr int kReduceTile = 16;   // segments / functions per workgroup
+constexpr int kReduceSeg = 128;   // rows per segment at most
+
+// pass 1: P[s][v] = sum of rows seg[s] .. seg[s+1]-1 of R (coalesced rows in,
+// coalesced rows out)
 template <typename T>
-__global__ __launch_bounds__(256) void k_reduce(const T *__restrict__ R, const int *__restrict__ ptr,
-                                                int64_t nb_packed, int nvec, double *__restrict__ out)
+__global__ __launch_bounds__(256) void k_reduce_seg(const T *__restrict__ R, const int *__restrict__ seg,
+                                                    int64_t nseg, int nvec, double *__restrict__ P)
+{
+  const int64_t s0 = (int64_t)blockIdx.x * kReduceTile;
+  const int ns = nseg - s0 < kReduceTile ? (int)(nseg - s0) : kReduceTile;
+  for (int g = threadIdx.x; g < ns * nvec; g += blockDim.x) {
+    const int sl = g / nvec, v = g - sl * nvec;
+    const int r0 = seg[s0 + sl], n = seg[s0 + sl + 1] - r0;
+    const T *Rv = R + (int64_t)r0 * nvec + v;
+    // FP64 sums also for FP32 term rows: the signed Smolyak weights cancel
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = 0;
+#pragma unroll 4
+    for (; i + 4 <= n; i += 4) {
+      a0 += (double)Rv[(int64_t)i * nvec];
+      a1 += (double)Rv[(int64_t)(i + 1) * nvec];
+      a2 += (double)Rv[(int64_t)(i + 2) * nvec];
+      a3 += (double)Rv[(int64_t)(i + 3) * nvec];
+    }
+    for (; i < n; ++i) a0 += (double)Rv[(int64_t)i * nvec];
+    P[(s0 + sl) * nvec + v] = (a0 + a1) + (a2 + a3);
+  }
+}
+
+// pass 2: out[v][pk] = sum over the segments fseg[pk] .. fseg[pk+1]-1 of P,
+// in order, through an LDS tile (the packed index fastest on the way out)
+__global__ __launch_bounds__(256) void k_reduce_fn(const double *__restrict__ P, const int *__restrict__ fseg,
+                                                   int64_t nb_packed, int nvec, double *__restrict__ out)
 {
   extern __shared__ double tile[];  // kReduceTile x (nvec + 1)
   const int64_t pk0 = (int64_t)blockIdx.x * kReduceTile;
@@ -929,23 +962,11 @@ __global__ __launch_bounds__(256) void k_reduce(const T *__restrict__ R, const i
   const int ld = nvec + 1;
   for (int g = threadIdx.x; g < np * nvec; g += blockDim.x) {
     const int pl = g / nvec, v = g - pl * nvec;
-    // the term rows of packed function pk are rows ptr[pk] .. ptr[pk+1]-1 of
-    // R (written there by the term kernels, in increasing term order): the
-    // workgroup streams one contiguous stretch of R
-    const int r0 = ptr[pk0 + pl], n = ptr[pk0 + pl + 1] - r0;
-    const T *Rv = R + (int64_t)r0 * nvec + v;
-    // FP64 sums also for FP32 term rows: the signed Smolyak weights cancel
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = 0;
-#pragma unroll 2
-    for (; i + 4 <= n; i += 4) {
-      s0 += (double)Rv[(int64_t)i * nvec];
-      s1 += (double)Rv[(int64_t)(i + 1) * nvec];
-      s2 += (double)Rv[(int64_t)(i + 2) * nvec];
-      s3 += (double)Rv[(int64_t)(i + 3) * nvec];
-    }
-    for (; i < n; ++i) s0 += (double)Rv[(int64_t)i * nvec];
-    tile[pl * ld + v] = (s0 + s1) + (s2 + s3);
+    const int a = fseg[pk0 + pl], b = fseg[pk0 + pl + 1];
+    double s = 0.0;
+#pragma unroll 8
+    for (int k = a; k < b; ++k) s += P[(int64_t)k * nvec + v];
+    tile[pl * ld + v] = s;
   }
   __syncthreads();
   for (int g = threadIdx.x; g < np * nvec; g += blockDim.x) {
@@ -1065,6 +1086,16 @@ __global__ __launch_bounds__(256) void k_to_float(const double *__restrict__ a, 
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
        g += (int64_t)gridDim.x * blockDim.x)
     b[g] = (float)a[g];
+}
+
+// Zero fill with 64-bit indices (the assembled pools exceed 2^34 words; a
+// fill kernel is also safe inside captured graphs, unlike memset nodes).
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero(T *__restrict__ a, int64_t n)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x)
+    a[g] = T(0);
 }
 
 // explicit instantiations are produced by the launch table in evr_plan.hip
