@@ -118,15 +118,17 @@ def davidson_levels(args):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
-    summary (profiles/traffic_k_term_sg4_d9l8.md), or None."""
-    import re
-    path = os.path.join(ROOT, "profiles", "traffic_k_term_sg4_d9l8.md")
+    summary of this workload (profiles/traffic_sg4_d9l8.md), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_sg4_d9l8.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        m = re.match(r"\|\s*(\S+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|\s*([0-9.eE+]+)\s*\|", line)
-        if m and m.group(1) == kernel:
-            return float(m.group(4))
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == kernel:
+            try:
+                return float(cells[4])
+            except ValueError:
+                return None
     return None
 
 

@@ -5,7 +5,8 @@
 ! lambda = 0.111803, the 6D/21D working tests' potential), then:
 !   1. H psi for 4 fixed real vectors and 2 complex ones,
 !   2. the 8 lowest levels by Davidson,
-!   3. 5 Chebyshev steps of a complex wavepacket.
+!   3. 5 Chebyshev steps of a complex wavepacket,
+!   4. H psi again on the assembled per-term operators (evr_plan_assemble).
 ! Results go to the text file named by the first argument.
 !===============================================================================
 program evr_f_host
@@ -18,7 +19,8 @@ program evr_f_host
   type(evr_basis_desc), pointer :: bd
   type(evr_op_desc) :: op
   integer(c_int64_t) :: sz(4)
-  real(c_double), allocatable, target :: q(:, :), V(:), psi(:, :), Hpsi(:, :), Vec(:, :)
+  real(c_double), allocatable, target :: q(:, :), V(:), psi(:, :), Hpsi(:, :), Vec(:, :), Hasm(:, :)
+  real(c_double) :: abytes
   real(c_double), target :: F2(D, D)
   real(c_double) :: Ene(nev)
   complex(c_double), allocatable :: zpsi(:, :), zH(:, :)
@@ -50,7 +52,7 @@ program evr_f_host
   op%F2 = c_loc(F2)
   call evr_check(evr_plan_create(desc, op, 8, plan), 'evr_plan_create')
 
-  allocate(psi(nb, nvec), Hpsi(nb, nvec), zpsi(nb, 2), zH(nb, 2), Vec(nb, nev))
+  allocate(psi(nb, nvec), Hpsi(nb, nvec), zpsi(nb, 2), zH(nb, 2), Vec(nb, nev), Hasm(nb, nvec))
   do k = 1, nvec
     do i = 1, nb
       psi(i, k) = sin(0.37d0 * i + 1.3d0 * k) / (1.0d0 + 0.01d0 * i)
@@ -63,6 +65,10 @@ program evr_f_host
   call evr_propagation_Davidson(plan, nev, 8, 1.0d-10, 1.0d-12, Ene, Vec, dinfo)
   zpsi(:, 1) = zpsi(:, 1) / sqrt(sum(abs(zpsi(:, 1))**2))
   call evr_march_cheby(plan, zpsi(:, 1:1), 0.5d0, 5, 200, 1.0d-14, pinfo)
+  call evr_check(evr_plan_assemble(plan, EVR_ASM_FP64), 'evr_plan_assemble')
+  if (evr_plan_assembled(plan, abytes) /= EVR_ASM_FP64) stop 'evr_plan_assembled'
+  call evr_sub_TabOpPsi(plan, psi, Hasm)
+  call evr_check(evr_plan_release_assembled(plan), 'evr_plan_release_assembled')
 
   open(newunit=u, file=trim(out), status='replace', action='write')
   write(u, '(i0,1x,i0)') nb, npts
@@ -81,6 +87,11 @@ program evr_f_host
   end do
   do i = 1, nb
     write(u, '(2es25.17)') real(zpsi(i, 1)), aimag(zpsi(i, 1))
+  end do
+  do k = 1, nvec
+    do i = 1, nb
+      write(u, '(es25.17)') Hasm(i, k)
+    end do
   end do
   close(u)
   write(*, '(a,i0,a,i0,a)') 'evr_f_host: Davidson ', dinfo%iterations, ' iterations, Chebyshev order ', &

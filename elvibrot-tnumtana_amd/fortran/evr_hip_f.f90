@@ -21,6 +21,7 @@ module evr_hip_f
 
   integer(c_int), parameter :: EVR_OK = 0, EVR_FP64 = 0, EVR_FP32 = 1, EVR_DEVICE_PTRS = 2
   integer(c_int), parameter :: EVR_PROPA_CHEBY = 1, EVR_PROPA_SIL = 2
+  integer(c_int), parameter :: EVR_MATRIX_FREE = 4, EVR_ASM_FP64 = 1, EVR_ASM_FP32 = 2
 
   type, bind(C) :: evr_basis_desc
     integer(c_int)     :: D, nlev
@@ -213,6 +214,23 @@ module evr_hip_f
       integer(c_int), value :: nwp
       type(evr_propa_info), intent(out) :: info
       integer(c_int) :: evr_propagate
+    end function
+    function evr_plan_assemble(p, which) bind(C, name="evr_plan_assemble")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+      integer(c_int), value :: which
+      integer(c_int) :: evr_plan_assemble
+    end function
+    function evr_plan_assembled(p, bytes) bind(C, name="evr_plan_assembled")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: p
+      real(c_double), intent(out) :: bytes
+      integer(c_int) :: evr_plan_assembled
+    end function
+    function evr_plan_release_assembled(p) bind(C, name="evr_plan_release_assembled")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+      integer(c_int) :: evr_plan_release_assembled
     end function
     function evr_plan_bounds(p, steps, Hmin, Hmax) bind(C, name="evr_plan_bounds")
       import :: c_int, c_ptr, c_double

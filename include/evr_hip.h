@@ -164,8 +164,9 @@ int evr_plan_max_nvec(const evr_plan *p);
  * kinetic and potential parts, Smolyak weight included) is built once by the
  * FP64 term kernel applied to the term's unit vectors and kept on the device
  * in MFMA fragment order, FP64 and/or FP32 (`which`: EVR_ASM_FP64 |
- * EVR_ASM_FP32; sum_t nb_t^2 values each, e.g. 142 GB FP64 at D=9 L=8, so
- * this trades HBM capacity for dense MFMA work).  evr_oppsi then computes
+ * EVR_ASM_FP32; about sum_t nb_t^2 values each, padded to 16 x 16 tiles,
+ * e.g. 144 GB FP64 + 72 GB FP32 at D=9 L=8, so this trades HBM capacity for
+ * dense MFMA work; EVR_ERR_SIZE when the device memory is short).  evr_oppsi then computes
  * R_t = H_t X_t with MFMA and the same scatter, in the precision asked,
  * unless EVR_MATRIX_FREE is set.  Setting a new potential drops the assembled
  * operator.  evr_plan_assembled returns the mask held and the bytes used. */

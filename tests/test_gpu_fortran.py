@@ -2,6 +2,7 @@
 
 evr_f_host builds a D=4 SG4 Henon-Heiles type-1 operator through evr_hip_f,
 applies it to real and complex vectors, runs Davidson and 5 Chebyshev steps,
+builds the assembled per-term operators and applies them again,
 and writes the results; the oracle recomputes all of them.
 """
 import os
@@ -62,3 +63,6 @@ def test_fortran_host(oracle_mod, tmp_path):
     for _ in range(5):
         ex = propa.exact_step(Hs, ex, 0.5)
     assert np.abs(zp - ex[:, 0]).max() < 1e-10
+    off += 2 * nb
+    Hasm = x[off:off + 4 * nb].reshape(4, nb).T
+    assert np.abs(Hasm - ref).max() < 1e-12 * np.abs(ref).max()
