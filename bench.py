@@ -24,6 +24,12 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X dense FP32
 PEAK_HBM = 8000.0  # GB/s, MI355X HBM3E
 
 
+def config_index(D, L):
+    """Seed family of the Hamiltonian generator: the D=6 L=5, D=9 L=8 and
+    D=12 L=6 Smolyak workloads have their own; any other size uses family 9."""
+    return {(6, 5): 1, (9, 8): 2, (12, 6): 4}.get((D, L), 9)
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -76,7 +82,7 @@ def davidson_levels(args):
     import evrhip
     from evrhip import hamiltonian as H
     D, L = args.D, args.L
-    cfg_index = 2 if (D, L) == (9, 8) else (1 if (D, L) == (6, 5) else 9)
+    cfg_index = config_index(D, L)
     ham = H.smolyak_config(cfg_index, D)
     basis = evrhip.Basis.sg4(D, L)
     plan = evrhip.Plan(basis, type=10, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=args.nvec, hermitian=True)
@@ -172,7 +178,7 @@ def main():
     from evrhip import hamiltonian as H
 
     D, L, nvec = args.D, args.L, args.nvec
-    cfg_index = 2 if (D, L) == (9, 8) else (1 if (D, L) == (6, 5) else 9)
+    cfg_index = config_index(D, L)
     ham = H.smolyak_config(cfg_index, D)
     t0 = time.perf_counter()
     basis = evrhip.Basis.sg4(D, L)

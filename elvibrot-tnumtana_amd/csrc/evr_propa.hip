@@ -117,6 +117,42 @@ __global__ void k_field(int64_t nb, int nwp, const double *E, const double *MW, 
     HW[g] -= E[g / (2 * nb)] * MW[g];
 }
 
+// Kernels of the captured Chebyshev step: coefficients are kernel arguments
+// (no uploads inside the graph) and the step's field values are picked on
+// the device from all steps' values by a step counter the graph advances.
+// Y = a X (real a) over n values
+__global__ void k_rscale(int64_t n, double a, const double *X, double *Y)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x)
+    Y[g] = a * X[g];
+}
+
+// Y_w = (c + i s) Y_w for every wavepacket
+__global__ void k_zphase(int64_t nb, int nwp, double c, double s, double *Y)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < nb * nwp;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int w = (int)(g / nb);
+    const int64_t r = g + nb * w, m = r + nb;
+    const double yr = Y[r], yi = Y[m];
+    Y[r] = c * yr - s * yi;
+    Y[m] = s * yr + c * yi;
+  }
+}
+
+// E[w] = field[step * nwp + w], step = ctr[0]
+__global__ void k_field_pick(int nwp, const double *field, const double *ctr, double *E)
+{
+  const int64_t step = (int64_t)ctr[0];
+  for (int w = threadIdx.x; w < nwp; w += blockDim.x) E[w] = field[step * nwp + w];
+}
+
+__global__ void k_step_next(double *ctr)
+{
+  if (threadIdx.x == 0) ctr[0] += 1.0;
+}
+
 // out[j][w] = <X_j,w | Y_w> (complex), X_j = X + j*xs; grid (nwp, nk)
 __global__ __launch_bounds__(256) void k_zdots(int64_t nb, int nwp, const double *X, int64_t xs,
                                                const double *Y, double *out)
@@ -363,21 +399,37 @@ int Batch::init(evr_plan *H_, evr_plan *mu_, int nwp_, int flags_)
   HTRY(dred.alloc(2 * (int64_t)nwp * 64));
   HTRY(dcoef.alloc(2 * (int64_t)nwp * 65));
   HTRY(dE.alloc(nwp));
+  if (!fork) HTRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  if (!join) HTRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
   return EVR_OK;
+}
+
+Batch::~Batch()
+{
+  if (fork) (void)hipEventDestroy(fork);
+  if (join) (void)hipEventDestroy(join);
 }
 
 int Batch::apply(evr_plan *p, const double *X, double *Y, int fl)
 {
   const int ncol = 2 * nwp;
   const int mv = std::max(1, evr_plan_max_nvec(p));
-  // evr_oppsi runs on the stream of plan p: order it after this batch's work
+  // evr_oppsi runs on the stream of plan p: ordered after this batch's work
+  // and before what follows by events (no host wait, so a captured step can
+  // take the coupling plan's stream along)
   hipStream_t ps = (hipStream_t)evr_plan_stream(p);
-  if (ps != st) HTRY(hipStreamSynchronize(st));
+  if (ps != st) {
+    HTRY(hipEventRecord(fork, st));
+    HTRY(hipStreamWaitEvent(ps, fork, 0));
+  }
   for (int c0 = 0; c0 < ncol; c0 += mv) {
     const int k = std::min(mv, ncol - c0);
     RTRY(evr_oppsi(p, k, X + nb * c0, Y + nb * c0, fl | EVR_DEVICE_PTRS));
   }
-  if (ps != st) HTRY(hipStreamSynchronize(ps));
+  if (ps != st) {
+    HTRY(hipEventRecord(join, ps));
+    HTRY(hipStreamWaitEvent(st, join, 0));
+  }
   if (p == H) hpsi += ncol;
   return EVR_OK;
 }
@@ -536,6 +588,113 @@ int cheb_step(Batch &P, ChebBufs &cw, double *PSI, const double *dEp, double *MW
                                       ", Hmax " + std::to_string(hmax) + "): reduce dt or raise npoly");
 }
 
+// One Chebyshev step of fixed order captured as a HIP graph: the launches of
+// every order (H|psi> kernels on the plan streams, recurrence, series), the
+// exit-test norm <w_K|w_K> into P.dred and the phase, with nothing read back
+// inside.  Replaying it advances PSI by one step; with a field the graph
+// first picks the step's values from all steps' values by a device counter
+// and advances the counter last.
+struct ChebGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  ChebRecord rec;
+  double cK = 0.0;             // 2 J_K: the exit test's coefficient
+  int64_t hpsi_per_step = 0;   // H|psi> columns one replay applies
+  ChebGraph() = default;
+  ChebGraph(const ChebGraph &) = delete;
+  ChebGraph &operator=(const ChebGraph &) = delete;
+  void reset()
+  {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+  }
+  ~ChebGraph() { reset(); }
+};
+
+static int capture_cheb_step(Batch &P, ChebBufs &cw, double *PSI, double *MWp, double dt,
+                             const double *field_all, double *ctr, const std::vector<double> &J,
+                             const ChebRecord &rec)
+{
+  const int64_t nblk = P.nb * 2 * P.nwp;
+  const size_t bytes = sizeof(double) * nblk;
+  const double *dEp = field_all ? P.dE.p : nullptr;
+  if (field_all) {
+    hipLaunchKernelGGL(k_field_pick, dim3(1), dim3(256), 0, P.st, P.nwp, field_all, ctr, P.dE.p);
+    HTRY(hipGetLastError());
+  }
+  double *w0 = cw.W1.p, *w1 = cw.W2.p, *w2 = cw.W3.p;
+  HTRY(hipMemcpyAsync(cw.SAVE.p, PSI, bytes, hipMemcpyDeviceToDevice, P.st));
+  HTRY(hipMemcpyAsync(w0, PSI, bytes, hipMemcpyDeviceToDevice, P.st));
+  hipLaunchKernelGGL(k_rscale, dim3(nblocks(nblk)), dim3(256), 0, P.st, nblk, J[0], cw.SAVE.p, PSI);
+  HTRY(hipGetLastError());
+  RTRY(P.apply_field(w0, cw.HW.p, MWp, dEp));
+  RTRY(launch_cheb_order(P, dEp, rec.e0, rec.esc, nullptr, w0, cw.HW.p, MWp, w1, PSI, 2.0 * J[1]));
+  for (int order = 2; order <= rec.order; ++order) {
+    RTRY(P.apply_field(w1, cw.HW.p, MWp, dEp));
+    RTRY(launch_cheb_order(P, dEp, rec.e0, rec.esc, w0, w1, cw.HW.p, MWp, w2, PSI, 2.0 * J[order]));
+    double *t = w0;
+    w0 = w1;
+    w1 = w2;
+    w2 = t;
+  }
+  hipLaunchKernelGGL(k_zdots, dim3(P.nwp, 1), dim3(256), 0, P.st, P.nb, P.nwp, w1, (int64_t)0, w1,
+                     P.dred.p);
+  HTRY(hipGetLastError());
+  hipLaunchKernelGGL(k_zphase, dim3(nblocks(P.nb * P.nwp)), dim3(256), 0, P.st, P.nb, P.nwp,
+                     std::cos(rec.e0 * dt), -std::sin(rec.e0 * dt), PSI);
+  HTRY(hipGetLastError());
+  if (field_all) {
+    hipLaunchKernelGGL(k_step_next, dim3(1), dim3(64), 0, P.st, ctr);
+    HTRY(hipGetLastError());
+  }
+  return EVR_OK;
+}
+
+static int build_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double *MWp, double dt,
+                            const double *field_all, double *ctr, const ChebRecord &rec, ChebGraph &G)
+{
+  G.reset();
+  const std::vector<double> J = bessel_j(rec.order, rec.esc * dt);
+  const int64_t h0 = P.hpsi;
+  HTRY(hipStreamBeginCapture(P.st, hipStreamCaptureModeThreadLocal));
+  const int rc = capture_cheb_step(P, cw, PSI, MWp, dt, field_all, ctr, J, rec);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(P.st, &g);
+  G.hpsi_per_step = P.hpsi - h0;
+  P.hpsi = h0;  // nothing ran yet
+  if (rc != EVR_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != EVR_OK) return rc;
+    HTRY(e);
+  }
+  G.g = g;
+  HTRY(hipGraphInstantiate(&G.x, g, nullptr, nullptr, 0));
+  G.rec = rec;
+  G.cK = 2.0 * J[rec.order];
+  return EVR_OK;
+}
+
+// Replay one step; ok = the exit test |c_K| <w_K|w_K> < poly_tol held for
+// every wavepacket (else PSI is restored from the saved step start).
+static int replay_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double poly_tol, ChebGraph &G,
+                             bool &ok)
+{
+  HTRY(hipGraphLaunch(G.x, P.st));
+  P.hpsi += G.hpsi_per_step;
+  std::vector<double> h(2 * (size_t)P.nwp);
+  HTRY(hipMemcpyAsync(h.data(), P.dred.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, P.st));
+  HTRY(hipStreamSynchronize(P.st));
+  double crit = 0.0;
+  for (int w = 0; w < P.nwp; ++w) crit = std::max(crit, std::fabs(h[2 * w] * G.cK));
+  ok = crit < poly_tol;  // NaN fails too
+  if (!ok)
+    HTRY(hipMemcpyAsync(PSI, cw.SAVE.p, sizeof(double) * P.nb * 2 * P.nwp, hipMemcpyDeviceToDevice,
+                        P.st));
+  return EVR_OK;
+}
+
 }  // namespace propa
 }  // namespace evr
 
@@ -596,20 +755,60 @@ extern "C" int evr_propagate(evr_plan *H, const evr_propa_opts *o, int nwp, doub
     HTRY(Q.alloc(nblk * (o->npoly + 1)));
   }
   std::vector<double> E(nwp, 0.0);
+  // captured steps: all steps' field values and the step counter on the device
+  const bool use_graph = o->graph && o->method == EVR_PROPA_CHEBY && o->nsteps > 1;
+  ChebGraph G;
+  DBuf field_all, ctr;
+  if (use_graph && o->field) {
+    HTRY(field_all.alloc((int64_t)o->nsteps * nwp));
+    HTRY(ctr.alloc(1));
+    HTRY(evr::upload_now(field_all.p, o->field, sizeof(double) * o->nsteps * nwp, P.st));
+  }
   for (int step = 0; step < o->nsteps; ++step) {
-    if (o->field) {
-      for (int w = 0; w < nwp; ++w) E[w] = o->field[(int64_t)step * nwp + w];
-      RTRY(P.set_field(E.data()));
-    }
     const double *dEp = o->field ? P.dE.p : nullptr;
     double *MWp = o->field ? MW.p : nullptr;
-    if (o->method == EVR_PROPA_CHEBY) {
+    if (o->method == EVR_PROPA_CHEBY && G.x) {
+      // ---- a captured step of the last order; redone adaptively (and the
+      // graph rebuilt) when its exit test fails
+      bool ok = false;
+      RTRY(replay_cheb_graph(P, cw, PSI.p, o->poly_tol, G, ok));
+      if (ok) {
+        max_order = std::max(max_order, G.rec.order);
+      } else {
+        if (o->field) {
+          for (int w = 0; w < nwp; ++w) E[w] = o->field[(int64_t)step * nwp + w];
+          RTRY(P.set_field(E.data()));
+        }
+        ChebRecord rec;
+        RTRY(cheb_step(P, cw, PSI.p, dEp, MWp, o->dt, o->npoly, o->poly_tol, hmin, hmax, rec, retries,
+                       nullptr, false));
+        max_order = std::max(max_order, rec.order);
+        if (rec.order > G.rec.order || rec.e0 != G.rec.e0 || rec.esc != G.rec.esc) {
+          RTRY(build_cheb_graph(P, cw, PSI.p, MWp, o->dt, field_all.p, ctr.p, rec, G));
+        }
+      }
+    } else if (o->method == EVR_PROPA_CHEBY) {
       // ---- march_cheby (MARCH:4142-4433)
+      if (o->field) {
+        for (int w = 0; w < nwp; ++w) E[w] = o->field[(int64_t)step * nwp + w];
+        RTRY(P.set_field(E.data()));
+      }
       ChebRecord rec;
       RTRY(cheb_step(P, cw, PSI.p, dEp, MWp, o->dt, o->npoly, o->poly_tol, hmin, hmax, rec, retries,
                      nullptr, false));
       max_order = std::max(max_order, rec.order);
+      if (use_graph && step + 1 < o->nsteps) {
+        if (o->field) {
+          const double next = (double)(step + 1);
+          HTRY(evr::upload_now(ctr.p, &next, sizeof(double), P.st));
+        }
+        RTRY(build_cheb_graph(P, cw, PSI.p, MWp, o->dt, field_all.p, ctr.p, rec, G));
+      }
     } else {
+      if (o->field) {
+        for (int w = 0; w < nwp; ++w) E[w] = o->field[(int64_t)step * nwp + w];
+        RTRY(P.set_field(E.data()));
+      }
       // ---- march_SIL (MARCH:2899-3100)
       const int kmax = o->npoly;
       std::vector<cplx> nn;

@@ -53,6 +53,12 @@ struct Batch {
   int nwp = 0, flags = 0;
   int64_t hpsi = 0;
   DBuf dred, dcoef, dE;
+  hipEvent_t fork = nullptr, join = nullptr;  // cross-stream order with a coupling plan
+
+  Batch() = default;
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+  ~Batch();
 
   int init(evr_plan *H, evr_plan *mu, int nwp, int flags);
   // Y = p X on all 2 nwp columns (chunks of the plan's max_nvec)

@@ -223,11 +223,13 @@ class Plan:
         return lo.value, hi.value
 
     def propagate(self, psi, dt, nsteps, method=CHEBY, npoly=200, poly_tol=1e-12, Hmin=0.0, Hmax=0.0,
-                  field=None, mu=None, psi0=None, precision=FP64, autocorr=False):
+                  field=None, mu=None, psi0=None, precision=FP64, autocorr=False, graph=False):
         """psi(t + nsteps dt) for complex psi (nb_packed,) or (nb_packed, nwp): march_cheby / march_SIL.
 
         field: (nsteps, nwp) values of the field coupling -E mu (mu a Plan on the
-        same basis), constant over each step.  Returns (psi, autocorr or None, info)."""
+        same basis), constant over each step.  graph: Chebyshev steps after the
+        first replayed as one captured HIP graph each (fixed order, exit test
+        checked after the replay).  Returns (psi, autocorr or None, info)."""
         r = _to_pairs(psi)
         nwp = r.shape[1] // 2
         fl = None
@@ -236,7 +238,8 @@ class Plan:
         p0 = None if psi0 is None else _to_pairs(psi0)
         o = PropaOpts(method, dt, nsteps, npoly, poly_tol, Hmin, Hmax, _d(fl),
                       None if mu is None else mu.h,
-                      None if p0 is None else p0.ctypes.data_as(ctypes.c_void_p), precision, 0)
+                      None if p0 is None else p0.ctypes.data_as(ctypes.c_void_p), precision, 0,
+                      1 if graph else 0)
         ac = np.zeros((nsteps, nwp, 2)) if autocorr else None
         info = PropaInfo()
         check(load().evr_propagate(self.h, ctypes.byref(o), nwp, r.ctypes.data_as(ctypes.c_void_p), _d(ac),

@@ -42,7 +42,7 @@ class PropaOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("dt", ctypes.c_double), ("nsteps", ctypes.c_int),
                 ("npoly", ctypes.c_int), ("poly_tol", ctypes.c_double), ("Hmin", ctypes.c_double),
                 ("Hmax", ctypes.c_double), ("field", _dp), ("mu", ctypes.c_void_p), ("psi0", ctypes.c_void_p),
-                ("precision", ctypes.c_int), ("device_ptrs", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("device_ptrs", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class PropaInfo(ctypes.Structure):

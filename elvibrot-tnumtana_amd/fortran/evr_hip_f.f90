@@ -61,7 +61,7 @@ module evr_hip_f
     integer(c_int) :: nsteps = 1, npoly = 200
     real(c_double) :: poly_tol = 1.0d-12, Hmin = 0.0d0, Hmax = 0.0d0
     type(c_ptr)    :: field = c_null_ptr, mu = c_null_ptr, psi0 = c_null_ptr
-    integer(c_int) :: precision = EVR_FP64, device_ptrs = 0
+    integer(c_int) :: precision = EVR_FP64, device_ptrs = 0, graph = 0
   end type evr_propa_opts
 
   type, bind(C) :: evr_propa_info

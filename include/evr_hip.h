@@ -287,6 +287,12 @@ typedef struct evr_propa_opts {
                               NULL: the initial psi                             */
   int precision;           /* EVR_FP64 (EVR_FP32: H|psi> in FP32)               */
   int device_ptrs;         /* psi / psi0 are device pointers                    */
+  int graph;               /* Chebyshev: 1 = replay each step after the first as
+                              one captured HIP graph of fixed order (the first
+                              step's), the exit test |c_K| <w_K|w_K> < poly_tol
+                              checked after every replay and the step redone
+                              adaptively when it fails; 0 = adaptive order per
+                              step with a host test per order (march_cheby)     */
 } evr_propa_opts;
 
 typedef struct evr_propa_info {

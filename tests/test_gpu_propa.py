@@ -142,3 +142,30 @@ def test_cheby_c0_direct_product(evr, oracle_mod):
         ex = propa.exact_step(Hs, ex, dt)
     print("C0 cheby", info.max_order, np.abs(out - ex).max())
     assert np.abs(out - ex).max() < 1e-10
+
+
+@pytest.mark.parametrize("with_field", [False, True])
+def test_cheby_graph(evr, sg4_case, with_field):
+    """Steps after the first replayed as one captured HIP graph each (fixed
+    order from the first step, exit test checked after every replay): the
+    same propagation as the adaptive driver, to the exact propagator's bar."""
+    c = sg4_case
+    nwp, nst, dt = 3, 12, 2e-3
+    psi = rand_psi(c["nb"], nwp, 8)
+    hmin, hmax = c["ev"][0] - 1.0, c["ev"][-1] + 1.0
+    field = np.random.default_rng(9).uniform(-300.0, 300.0, (nst, nwp)) if with_field else None
+    mu = c["mplan"] if with_field else None
+    kw = dict(method=evr.CHEBY, npoly=300, poly_tol=1e-14, Hmin=hmin, Hmax=hmax, field=field, mu=mu,
+              autocorr=True)
+    eager, ac_e, ie = c["plan"].propagate(psi, dt, nst, **kw)
+    graph, ac_g, ig = c["plan"].propagate(psi, dt, nst, graph=True, **kw)
+    ex = psi
+    for n in range(nst):
+        ex = propa.exact_step(c["Hs"], ex, dt, None if field is None else field[n],
+                              None if field is None else c["Mu"])
+    print("graph field=%s" % with_field, ig.max_order, ie.max_order, ig.hpsi, ie.hpsi,
+          np.abs(graph - eager).max(), np.abs(graph - ex).max())
+    assert np.abs(graph - ex).max() < 1e-10
+    assert np.abs(graph - eager).max() < 1e-12
+    assert np.abs(ac_g - ac_e).max() < 1e-12
+    assert ig.max_order == ie.max_order
