@@ -122,6 +122,61 @@ def davidson_levels(args):
     print(json.dumps(line), flush=True)
 
 
+def control_run(args):
+    """Optimal control on D=6 SG4 L=5 in atomic units: 32 seeded fields of 64
+    sine coefficients, T = 2000 au in 200 Chebyshev steps of 10 au, objective
+    |<phi_1|psi_f(T)>|^2 from the ground state, gradient in the coefficients
+    and the V grid values by the adjoint sweep with checkpoints every 32 steps;
+    then the 32 forward propagations alone, adaptive and as captured graphs.
+    Sum-factorised operators first, then the assembled ones."""
+    import numpy as np
+    import evrhip
+    from evrhip import hamiltonian as H
+    D, L, T, nsteps = 6, 5, 2000.0, 200
+    ham = H.smolyak_config(1, D)
+    basis = evrhip.Basis.sg4(D, L)
+    q = basis.grid()
+    au = H.AU_TO_CM
+    plan = evrhip.Plan(basis, type=10, V=ham.potential(q) / au, G0=ham.G0 / au, Ga=ham.Ga_fortran() / au,
+                       max_nvec=64, hermitian=True)
+    m1, m2, amp = H.control_config(D)
+    mplan = evrhip.Plan(basis, type=0, V=H.dipole(q, m1, m2), max_nvec=64)
+    ev, vec, _ = plan.davidson(2, block=2, conv_resi=1e-9, want_vectors=True)
+    psi0, phi1 = vec[:, 0].astype(complex), vec[:, 1].astype(complex)
+    nf = amp.shape[1]
+    field = evrhip.control_field(amp, T, nsteps)
+    line = {"workload": "optimal control, SG4 D=6 L=5 (%d packed functions), %d fields x %d coefficients, "
+                        "T=%g au, %d Chebyshev steps, checkpoints every 32 steps" % (basis.nb_packed, nf,
+                                                                                    amp.shape[0], T, nsteps),
+            "E0_au": float(ev[0]), "E1_au": float(ev[1])}
+    ref = None
+    for tag in ("matrix_free", "assembled"):
+        if tag == "assembled":
+            plan.assemble(evrhip.ASM_FP64)
+            mplan.assemble(evrhip.ASM_FP64)
+        t0 = time.perf_counter()
+        J, g, info = plan.control_gradient(mplan, amp, psi0, phi1, T, nsteps, checkpoint=32, want=("a", "V"))
+        wall = time.perf_counter() - t0
+        run = {"seconds": round(wall, 3), "seconds_forward": round(info.seconds_forward, 3),
+               "seconds_backward": round(info.seconds_backward, 3), "hpsi_columns": info.hpsi,
+               "max_order": info.max_order, "J_mean": float(J.mean())}
+        if ref is None:
+            ref = (J, g)
+        else:
+            run["max_abs_dJ_vs_matrix_free"] = float(np.abs(J - ref[0]).max())
+            run["max_rel_dgrad_a_vs_matrix_free"] = float(np.abs(g["a"] - ref[1]["a"]).max() /
+                                                          np.abs(ref[1]["a"]).max())
+        psi = np.repeat(psi0[:, None], nf, axis=1)
+        for graph in (False, True):
+            t0 = time.perf_counter()
+            out, _, pinfo = plan.propagate(psi, T / nsteps, nsteps, npoly=200, poly_tol=1e-14, field=field,
+                                           mu=mplan, graph=graph)
+            run["forward_%s_seconds" % ("graph" if graph else "adaptive")] = round(time.perf_counter() - t0, 3)
+        line[tag] = run
+        log("%s: %s" % (tag, run))
+    print(json.dumps(line), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this workload (profiles/traffic_sg4_d9l8.md), or None."""
@@ -156,6 +211,8 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (scale of the HBM counters)")
     ap.add_argument("--davidson", action="store_true",
                     help="only run the FP64 and FP32-refined Davidson levels and their difference")
+    ap.add_argument("--control", action="store_true",
+                    help="only run the optimal-control objective, gradient and field-driven propagation")
     ap.add_argument("--nev", type=int, default=200)
     ap.add_argument("--conv-resi", type=float, default=1e-3)
     ap.add_argument("--refine-resi", type=float, default=1e-2)
@@ -170,6 +227,9 @@ def main():
         return
     if args.davidson:
         davidson_levels(args)
+        return
+    if args.control:
+        control_run(args)
         return
 
     import numpy as np

@@ -124,6 +124,28 @@ def smolyak_config(index, D):
     return Hamiltonian(D, omega, c3, d22, np.full(D, 20.0), Ga)
 
 
+def control_config(D, nk=64, nf=32, index=3):
+    """Optimal-control inputs of configuration 3: the dipole surface
+    mu(q) = sum_i m_i q_i + sum_{i<=j} m_ij q_i q_j with m_i, m_ij uniform in
+    +-0.1 au, and nf fields of nk sine coefficients a_kf uniform in +-1e-3 au
+    (nk x nf).  Returns (m1, m2 upper triangle, amp)."""
+    seed = config_seed(index)
+    m1 = 0.1 * SplitMix64(seed, FAM_MU1).uniform(D)
+    r2 = SplitMix64(seed, FAM_MU2)
+    m2 = np.zeros((D, D))
+    for i in range(D):
+        for j in range(i, D):
+            m2[i, j] = 0.1 * r2.uniform(1)[0]
+    amp = 1e-3 * SplitMix64(seed, FAM_FIELD).uniform(nk * nf).reshape(nf, nk).T.copy(order="F")
+    return m1, m2, amp
+
+
+def dipole(q, m1, m2):
+    """mu at points q (npts x D) for control_config's coefficients."""
+    q = np.asarray(q, float)
+    return q @ m1 + np.einsum("pi,ij,pj->p", q, m2, q)
+
+
 def test_vectors(n, nvec, index=0):
     """Normalised random test vectors (n x nvec, column-major)."""
     rng = SplitMix64(config_seed(index), FAM_PSI)
