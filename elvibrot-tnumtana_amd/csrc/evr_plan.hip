@@ -103,6 +103,7 @@ struct evr_plan {
   // function pk are fseg[pk] .. fseg[pk+1]-1
   DevBuf<int> d_seg, d_fseg;
   int64_t nseg = 0;
+  int64_t nrows = 0;   // rows of R: term rows that map to a packed function
   DevBuf<double> d_P;  // segment sums, nseg x max_nvec
   DevBuf<double> d_V;
   DevBuf<double> d_Veff;  // V summed over the one-function modes of each term
@@ -289,40 +290,114 @@ hipError_t zero_fill(T *a, int64_t n, hipStream_t s)
   return hipGetLastError();
 }
 
+// What the assembled apply and the scatter read: one plan's tables, or the
+// concatenated tables of several plans (evr_multi).
+struct AsmOp {
+  const evr::TermDev *terms;
+  const int2 *items;
+  int64_t n_items;
+  const int *map, *rowpos, *seg, *fseg;
+  int64_t nseg, nb_packed;
+  double *P;               // segment sums
+  hipStream_t stream;
+};
+
+AsmOp plan_op(const evr_plan *p)
+{
+  return AsmOp{p->d_terms.ptr, p->d_asm_items.ptr, p->n_asm_items, p->d_map.ptr, p->d_rowpos.ptr,
+               p->d_seg.ptr, p->d_fseg.ptr, p->nseg, p->nb_packed, p->d_P.ptr, p->stream};
+}
+
 // Deterministic Smolyak scatter R -> out (nb_packed x nvec, column-major).
 template <typename T>
-hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out)
+hipError_t reduce_op(const AsmOp &o, const T *R, int nvec, double *out)
 {
-  const int64_t sblocks = (p->nseg + evr::kReduceTile - 1) / evr::kReduceTile;
+  const int64_t sblocks = (o.nseg + evr::kReduceTile - 1) / evr::kReduceTile;
   hipLaunchKernelGGL(evr::k_reduce_seg<T>, dim3((unsigned)std::max<int64_t>(sblocks, 1)), dim3(256), 0,
-                     p->stream, R, p->d_seg.ptr, p->nseg, nvec, p->d_P.ptr);
+                     o.stream, R, o.seg, o.nseg, nvec, o.P);
   const size_t lds = sizeof(double) * evr::kReduceTile * (size_t)(nvec + 1);
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&evr::k_reduce_fn),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  const int64_t blocks = (p->nb_packed + evr::kReduceTile - 1) / evr::kReduceTile;
+  const int64_t blocks = (o.nb_packed + evr::kReduceTile - 1) / evr::kReduceTile;
   hipLaunchKernelGGL(evr::k_reduce_fn, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), lds,
-                     p->stream, p->d_P.ptr, p->d_fseg.ptr, p->nb_packed, nvec, out);
+                     o.stream, o.P, o.fseg, o.nb_packed, nvec, out);
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out)
+{
+  return reduce_op<T>(plan_op(p), R, nvec, out);
+}
+
 // R = H_t X_t for every term from the assembled pool A (32 vectors per pass).
+template <typename T>
+hipError_t apply_op(const AsmOp &o, const T *A, const T *X, int nvec, T *R)
+{
+  for (int vb = 0; vb < nvec; vb += 32) {
+    const dim3 grid((unsigned)o.n_items);
+    if (nvec - vb > 16)
+      hipLaunchKernelGGL((evr::k_apply_asm<T, 2>), grid, dim3(256), 0, o.stream, o.terms, o.items, A,
+                         o.map, o.rowpos, X, nvec, vb, R);
+    else
+      hipLaunchKernelGGL((evr::k_apply_asm<T, 1>), grid, dim3(256), 0, o.stream, o.terms, o.items, A,
+                         o.map, o.rowpos, X, nvec, vb, R);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Work items of the assembled apply: (term, first of 16 row tiles).
+// Workgroups are handed to the 8 XCDs round-robin (workgroup i on XCD i % 8),
+// so all items of a term go to one XCD, in consecutive turns: the term's X
+// rows are then read from HBM once and from that XCD's L2 afterwards.  Terms
+// are spread over the XCDs by matrix size, largest first.
+std::vector<int2> asm_items(const std::vector<int> &nbt)
+{
+  constexpr int kXcd = 8;
+  const int64_t nt = (int64_t)nbt.size();
+  std::vector<int64_t> order(nt);
+  for (int64_t t = 0; t < nt; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return nbt[a] > nbt[b]; });
+  std::vector<std::vector<int2>> queue(kXcd);
+  std::vector<double> bytes(kXcd, 0.0);
+  size_t total = 0;
+  for (int64_t t : order) {
+    const int mtn = (nbt[t] + 15) / 16;
+    const int x = (int)(std::min_element(bytes.begin(), bytes.end()) - bytes.begin());
+    for (int m0 = 0; m0 < mtn; m0 += 16) queue[x].push_back(int2{(int)t, m0});
+    bytes[x] += (double)mtn * evr::asm_ksteps(nbt[t]);
+    total += (mtn + 15) / 16;
+  }
+  std::vector<int2> items;
+  items.reserve(total);
+  std::vector<size_t> head(kXcd, 0);
+  while (items.size() < total) {
+    for (int x = 0; x < kXcd && items.size() < total; ++x) {
+      if (head[x] < queue[x].size()) {
+        items.push_back(queue[x][head[x]++]);
+      } else {
+        // this XCD's queue is empty: take the last item of the longest one
+        int y = 0;
+        for (int z = 1; z < kXcd; ++z)
+          if (queue[z].size() - head[z] > queue[y].size() - head[y]) y = z;
+        items.push_back(queue[y].back());
+        queue[y].pop_back();
+      }
+    }
+  }
+  return items;
+}
+
 template <typename T>
 hipError_t apply_assembled(evr_plan *p, const T *A, const T *X, int nvec, T *R)
 {
   const int slot = kNumClasses;  // profiling slot of the assembled apply
   if (p->profiling) (void)hipEventRecord(p->ev[2 * slot], p->stream);
-  for (int vb = 0; vb < nvec; vb += 32) {
-    const dim3 grid((unsigned)p->n_asm_items);
-    if (nvec - vb > 16)
-      hipLaunchKernelGGL((evr::k_apply_asm<T, 2>), grid, dim3(256), 0, p->stream, p->d_terms.ptr,
-                         p->d_asm_items.ptr, A, p->d_map.ptr, p->d_rowpos.ptr, X, nvec, vb, R);
-    else
-      hipLaunchKernelGGL((evr::k_apply_asm<T, 1>), grid, dim3(256), 0, p->stream, p->d_terms.ptr,
-                         p->d_asm_items.ptr, A, p->d_map.ptr, p->d_rowpos.ptr, X, nvec, vb, R);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
+  const hipError_t e = apply_op<T>(plan_op(p), A, X, nvec, R);
+  if (e != hipSuccess) return e;
   if (p->profiling) {
     (void)hipEventRecord(p->ev[2 * slot + 1], p->stream);
     (void)hipEventSynchronize(p->ev[2 * slot + 1]);
@@ -813,6 +888,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     }
     fseg[p->nb_packed] = (int)seg.size();
     p->nseg = (int64_t)seg.size();
+    p->nrows = cnt[p->nb_packed];
     seg.push_back(cnt[p->nb_packed]);
     HIP_TRY(p->d_seg.upload(seg.data(), seg.size(), p->stream));
     HIP_TRY(p->d_fseg.upload(fseg.data(), fseg.size(), p->stream));
@@ -997,44 +1073,10 @@ int evr_plan_assemble(evr_plan *p, int which)
     HIP_TRY(launch_items(c, prm, (int64_t)items.size(), p->lds_bytes64[c], p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
   }
-  // apply work items: (term, first of 16 row tiles).  Workgroups are handed to
-  // the 8 XCDs round-robin (workgroup i on XCD i % 8), so all items of a term
-  // go to one XCD, in consecutive turns: the term's X rows are then read from
-  // HBM once and from that XCD's L2 afterwards.  Terms are spread over the
-  // XCDs by matrix size, largest first.
   if (!p->d_asm_items.ptr) {
-    constexpr int kXcd = 8;
-    std::vector<int64_t> order(p->nterms);
-    for (int64_t t = 0; t < p->nterms; ++t) order[t] = t;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t b) { return p->terms[a].nbt > p->terms[b].nbt; });
-    std::vector<std::vector<int2>> queue(kXcd);
-    std::vector<double> bytes(kXcd, 0.0);
-    size_t total = 0;
-    for (int64_t t : order) {
-      const int nbt = p->terms[t].nbt, mtn = (nbt + 15) / 16;
-      const int x = (int)(std::min_element(bytes.begin(), bytes.end()) - bytes.begin());
-      for (int m0 = 0; m0 < mtn; m0 += 16) queue[x].push_back(int2{(int)t, m0});
-      bytes[x] += (double)mtn * evr::asm_ksteps(nbt);
-      total += (mtn + 15) / 16;
-    }
-    std::vector<int2> items;
-    items.reserve(total);
-    std::vector<size_t> head(kXcd, 0);
-    while (items.size() < total) {
-      for (int x = 0; x < kXcd && items.size() < total; ++x) {
-        if (head[x] < queue[x].size()) {
-          items.push_back(queue[x][head[x]++]);
-        } else {
-          // this XCD's queue is empty: take the last item of the longest one
-          int y = 0;
-          for (int z = 1; z < kXcd; ++z)
-            if (queue[z].size() - head[z] > queue[y].size() - head[y]) y = z;
-          items.push_back(queue[y].back());
-          queue[y].pop_back();
-        }
-      }
-    }
+    std::vector<int> nbt(p->nterms);
+    for (int64_t t = 0; t < p->nterms; ++t) nbt[t] = p->terms[t].nbt;
+    const std::vector<int2> items = asm_items(nbt);
     if ((int64_t)items.size() > (int64_t)0xFFFFFFFFu / 256)
       return fail(EVR_ERR_SIZE, "too many assembled work items for one launch");
     HIP_TRY(p->d_asm_items.upload(items.data(), items.size(), p->stream));
@@ -1248,6 +1290,182 @@ int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB)
   HIP_TRY(hipMemcpyAsync(psiB, p->d_hpsi.ptr, sizeof(double) * p->nb_packed, hipMemcpyDeviceToHost,
                          p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
+  return EVR_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ multi-system operator --
+// n assembled plans as one block-diagonal operator: the per-term tables are
+// concatenated with their offsets shifted (term rows, packed functions, rows
+// of R, reduction segments, pool words), so the assembled apply and the
+// scatter run over every system's terms in one launch each.
+struct evr_multi {
+  int n = 0, max_nvec = 0;
+  std::vector<int64_t> nb_off;  // packed offsets, n + 1
+  int64_t nterms = 0, sum_nb = 0, nrows = 0, nseg = 0, nwords = 0, n_items = 0;
+  DevBuf<evr::TermDev> d_terms;
+  DevBuf<double> d_A64;
+  DevBuf<int> d_map, d_rowpos, d_seg, d_fseg;
+  DevBuf<int2> d_items;
+  DevBuf<double> d_X, d_R, d_P, d_psi, d_hpsi;
+  hipStream_t stream = nullptr;
+  void release()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    d_terms.release(); d_A64.release(); d_map.release(); d_rowpos.release(); d_seg.release();
+    d_fseg.release(); d_items.release(); d_X.release(); d_R.release(); d_P.release();
+    d_psi.release(); d_hpsi.release();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+namespace {
+
+AsmOp multi_op(const evr_multi *m)
+{
+  return AsmOp{m->d_terms.ptr, m->d_items.ptr, m->n_items, m->d_map.ptr, m->d_rowpos.ptr, m->d_seg.ptr,
+               m->d_fseg.ptr, m->nseg, m->nb_off.back(), m->d_P.ptr, m->stream};
+}
+
+// dst[0..n) = src[0..n) + add (entries < 0 stay -1)
+hipError_t offset_copy(const int *src, int64_t n, int64_t add, int *dst, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(evr::k_offset_ints, dim3(blocks_for(n)), dim3(256), 0, s, src, n, add, dst);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int evr_multi_create(evr_plan *const *plans, int n, int max_nvec, evr_multi **out)
+{
+  if (!plans || !out || n < 1 || max_nvec < 1) return fail(EVR_ERR_ARG, "bad argument");
+  *out = nullptr;
+  for (int k = 0; k < n; ++k) {
+    if (!plans[k]) return fail(EVR_ERR_ARG, "NULL plan " + std::to_string(k));
+    if (!plans[k]->d_A64.ptr)
+      return fail(EVR_ERR_STATE, "plan " + std::to_string(k) + " has no FP64 assembled operator");
+  }
+  auto m = std::make_unique<evr_multi>();
+  m->n = n;
+  m->max_nvec = max_nvec;
+  m->nb_off.assign(n + 1, 0);
+  std::vector<int64_t> row_off(n + 1, 0), nb_rows(n + 1, 0), seg_off(n + 1, 0), word_off(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    const evr_plan *p = plans[k];
+    m->nb_off[k + 1] = m->nb_off[k] + p->nb_packed;
+    nb_rows[k + 1] = nb_rows[k] + p->sum_nb;
+    row_off[k + 1] = row_off[k] + p->nrows;
+    seg_off[k + 1] = seg_off[k] + p->nseg;
+    word_off[k + 1] = word_off[k] + p->asm_elems;
+    m->nterms += p->nterms;
+  }
+  m->sum_nb = nb_rows[n];
+  m->nrows = row_off[n];
+  m->nseg = seg_off[n];
+  m->nwords = word_off[n];
+  if (m->nb_off[n] >= INT32_MAX || m->sum_nb >= INT32_MAX || m->nseg >= INT32_MAX)
+    return fail(EVR_ERR_SIZE, "systems too large to combine (32-bit tables)");
+  HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  hipStream_t s = m->stream;
+  // terms with shifted offsets, and the work items over all of them
+  std::vector<evr::TermDev> terms;
+  std::vector<int> nbt;
+  terms.reserve(m->nterms);
+  nbt.reserve(m->nterms);
+  for (int k = 0; k < n; ++k)
+    for (evr::TermDev td : plans[k]->terms) {
+      td.off_nb += nb_rows[k];
+      td.off_asm += word_off[k];
+      terms.push_back(td);
+      nbt.push_back(td.nbt);
+    }
+  const std::vector<int2> items = asm_items(nbt);
+  if ((int64_t)items.size() > (int64_t)0xFFFFFFFFu / 256)
+    return fail(EVR_ERR_SIZE, "too many assembled work items for one launch");
+  HIP_TRY(m->d_terms.upload(terms.data(), terms.size(), s));
+  HIP_TRY(m->d_items.upload(items.data(), items.size(), s));
+  m->n_items = (int64_t)items.size();
+  // pools and tables
+  if (m->d_A64.alloc((size_t)m->nwords) != hipSuccess) {
+    m->d_A64.ptr = nullptr;
+    return fail(EVR_ERR_SIZE, "not enough device memory for the combined assembled operators (" +
+                                  std::to_string(8.0 * m->nwords / 1e9) + " GB)");
+  }
+  HIP_TRY(m->d_map.alloc((size_t)m->sum_nb));
+  HIP_TRY(m->d_rowpos.alloc((size_t)m->sum_nb));
+  HIP_TRY(m->d_seg.alloc((size_t)m->nseg + 1));
+  HIP_TRY(m->d_fseg.alloc((size_t)m->nb_off[n] + 1));
+  for (int k = 0; k < n; ++k) {
+    evr_plan *p = plans[k];
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpyAsync(m->d_A64.ptr + word_off[k], p->d_A64.ptr, sizeof(double) * p->asm_elems,
+                           hipMemcpyDeviceToDevice, s));
+    HIP_TRY(offset_copy(p->d_map.ptr, p->sum_nb, m->nb_off[k], m->d_map.ptr + nb_rows[k], s));
+    HIP_TRY(offset_copy(p->d_rowpos.ptr, p->sum_nb, row_off[k], m->d_rowpos.ptr + nb_rows[k], s));
+    HIP_TRY(offset_copy(p->d_seg.ptr, p->nseg, row_off[k], m->d_seg.ptr + seg_off[k], s));
+    HIP_TRY(offset_copy(p->d_fseg.ptr, p->nb_packed, seg_off[k], m->d_fseg.ptr + m->nb_off[k], s));
+  }
+  const int seg_end = (int)m->nrows, fseg_end = (int)m->nseg;
+  HIP_TRY(hipMemcpyAsync(m->d_seg.ptr + m->nseg, &seg_end, sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(m->d_fseg.ptr + m->nb_off[n], &fseg_end, sizeof(int), hipMemcpyHostToDevice, s));
+  // work buffers
+  HIP_TRY(m->d_X.alloc((size_t)m->nb_off[n] * max_nvec));
+  HIP_TRY(m->d_R.alloc((size_t)std::max<int64_t>(m->nrows, 1) * max_nvec));
+  HIP_TRY(m->d_P.alloc((size_t)std::max<int64_t>(m->nseg, 1) * max_nvec));
+  HIP_TRY(m->d_psi.alloc((size_t)m->nb_off[n] * max_nvec));
+  HIP_TRY(m->d_hpsi.alloc((size_t)m->nb_off[n] * max_nvec));
+  HIP_TRY(hipStreamSynchronize(s));
+  *out = m.release();
+  return EVR_OK;
+}
+
+int evr_multi_destroy(evr_multi *m)
+{
+  if (!m) return EVR_OK;
+  m->release();
+  delete m;
+  return EVR_OK;
+}
+
+int evr_multi_offsets(const evr_multi *m, int64_t *nb_offsets)
+{
+  if (!m || !nb_offsets) return fail(EVR_ERR_ARG, "NULL argument");
+  std::copy(m->nb_off.begin(), m->nb_off.end(), nb_offsets);
+  return EVR_OK;
+}
+
+void *evr_multi_stream(evr_multi *m) { return m ? (void *)m->stream : nullptr; }
+
+int evr_multi_count(const evr_multi *m) { return m ? m->n : fail(EVR_ERR_ARG, "NULL argument"); }
+
+int evr_multi_oppsi(evr_multi *m, int nvec, const double *psi, double *hpsi, int flags)
+{
+  if (!m || !psi || !hpsi) return fail(EVR_ERR_ARG, "NULL argument");
+  if (nvec < 1 || nvec > m->max_nvec) return fail(EVR_ERR_ARG, "nvec must be in 1..max_nvec");
+  const bool dev = (flags & EVR_DEVICE_PTRS) != 0;
+  const int64_t nbp = m->nb_off.back(), n = nbp * nvec;
+  const double *dpsi = psi;
+  double *dh = hpsi;
+  if (!dev) {
+    HIP_TRY(hipMemcpyAsync(m->d_psi.ptr, psi, sizeof(double) * n, hipMemcpyHostToDevice, m->stream));
+    dpsi = m->d_psi.ptr;
+    dh = m->d_hpsi.ptr;
+  }
+  hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, m->stream, dpsi, nbp, nvec,
+                     m->d_X.ptr);
+  HIP_TRY(hipGetLastError());
+  const AsmOp o = multi_op(m);
+  HIP_TRY(apply_op<double>(o, m->d_A64.ptr, m->d_X.ptr, nvec, m->d_R.ptr));
+  HIP_TRY(reduce_op<double>(o, m->d_R.ptr, nvec, dh));
+  if (!dev) {
+    HIP_TRY(hipMemcpyAsync(hpsi, dh, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+  }
   return EVR_OK;
 }
 

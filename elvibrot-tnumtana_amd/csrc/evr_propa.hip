@@ -153,6 +153,79 @@ __global__ void k_step_next(double *ctr)
   if (threadIdx.x == 0) ctr[0] += 1.0;
 }
 
+// ---- several systems, one wavepacket each (Re column, then Im column, of
+// nbt = sum_k nb_k rows; row i belongs to system sys[i]) -----------------------
+
+// k_cheb_order with the shift, scale and series coefficient of each row's
+// system; a system whose series has ended (act = 0) takes no further term
+__global__ void k_mcheb_order(int64_t nbt, const int *sys, const double *e0, const double *esc,
+                              const double *c, const int *act, const double *W1, const double *W2,
+                              const double *HW, double *W3, double *PSI)
+{
+  const double f = W1 ? 2.0 : 1.0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nbt;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int k = sys[r];
+    const int64_t m = r + nbt;
+    const double inv = 1.0 / esc[k];
+    double tr = HW[r] - e0[k] * W2[r], ti = HW[m] - e0[k] * W2[m];
+    tr *= inv;
+    ti *= inv;
+    double xr = f * ti, xi = -f * tr;
+    if (W1) {
+      xr += W1[r];
+      xi += W1[m];
+    }
+    W3[r] = xr;
+    W3[m] = xi;
+    if (act[k]) {
+      PSI[r] += c[k] * xr;
+      PSI[m] += c[k] * xi;
+    }
+  }
+}
+
+// Y = a_k X per row (both columns)
+__global__ void k_mscale(int64_t nbt, const int *sys, const double *a, const double *X, double *Y)
+{
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nbt;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const double ak = a[sys[r]];
+    Y[r] = ak * X[r];
+    Y[r + nbt] = ak * X[r + nbt];
+  }
+}
+
+// Y = (cs_k + i sn_k) Y per row
+__global__ void k_mphase(int64_t nbt, const int *sys, const double *cs, const double *sn, double *Y)
+{
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nbt;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int k = sys[r];
+    const double yr = Y[r], yi = Y[r + nbt];
+    Y[r] = cs[k] * yr - sn[k] * yi;
+    Y[r + nbt] = sn[k] * yr + cs[k] * yi;
+  }
+}
+
+// out[k] = <w|w> over the rows of system k (one workgroup per system)
+__global__ __launch_bounds__(256) void k_msys_norm(int64_t nbt, const int64_t *off, const double *W,
+                                                   double *out)
+{
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int64_t r = off[k] + threadIdx.x; r < off[k + 1]; r += 256)
+    s = fma(W[r], W[r], fma(W[r + nbt], W[r + nbt], s));
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
 // out[j][w] = <X_j,w | Y_w> (complex), X_j = X + j*xs; grid (nwp, nk)
 __global__ __launch_bounds__(256) void k_zdots(int64_t nb, int nwp, const double *X, int64_t xs,
                                                const double *Y, double *out)
@@ -906,5 +979,137 @@ extern "C" int evr_bessel_j(int n, double r, double *out)
   if (n < 0 || !out) return set_error(EVR_ERR_ARG, "bad argument");
   const std::vector<double> j = bessel_j(n, r);
   std::copy(j.begin(), j.end(), out);
+  return EVR_OK;
+}
+
+// ---- march_cheby over a multi-system operator ---------------------------------
+// Each system k keeps its own E0_k, Esc_k, Bessel coefficients and adaptive
+// order (the exit test of march_cheby, MARCH:4306-4335, per system); one
+// H|psi> launch sequence per order advances all of them.
+extern "C" int evr_multi_propagate(evr_multi *m, double dt, int nsteps, int npoly, double poly_tol,
+                                   const double *Hmin, const double *Hmax, double *psi, int device_ptrs,
+                                   evr_propa_info *info)
+{
+  if (!m || !Hmin || !Hmax || !psi || npoly < 3 || nsteps < 0 || !(dt > 0.0))
+    return set_error(EVR_ERR_ARG, "bad argument");
+  const auto t_start = std::chrono::steady_clock::now();
+  const int n = evr_multi_count(m);
+  if (n < 1) return n;
+  std::vector<int64_t> off(n + 1);
+  RTRY(evr_multi_offsets(m, off.data()));
+  const int64_t nbt = off[n], nblk = 2 * nbt;
+  hipStream_t st = (hipStream_t)evr_multi_stream(m);
+  std::vector<double> e0(n), esc(n), cs(n), sn(n);
+  std::vector<std::vector<double>> J(n);
+  for (int k = 0; k < n; ++k) {
+    if (!(Hmin[k] < Hmax[k])) return set_error(EVR_ERR_ARG, "Hmin[k] < Hmax[k] needed for every system");
+    const double dE = Hmax[k] - Hmin[k];
+    e0[k] = Hmin[k] + 0.5 * dE;  // MARCH:4211-4231
+    esc[k] = 0.5 * dE;
+    J[k] = bessel_j(npoly - 1, esc[k] * dt);
+    const std::complex<double> ph = std::exp(std::complex<double>(0.0, -e0[k] * dt));
+    cs[k] = ph.real();
+    sn[k] = ph.imag();
+  }
+  // coefficient table: row 0 = J_0, row j >= 1 = 2 J_j, one value per system
+  std::vector<double> ctab((size_t)npoly * n);
+  for (int j = 0; j < npoly; ++j)
+    for (int k = 0; k < n; ++k) ctab[(size_t)j * n + k] = (j == 0 ? 1.0 : 2.0) * J[k][j];
+  std::vector<int> sys(nbt);
+  for (int k = 0; k < n; ++k)
+    for (int64_t r = off[k]; r < off[k + 1]; ++r) sys[r] = k;
+  DBuf PSI, W1, W2, W3, HW, dE0, dEsc, dC, dCs, dSn, dNorm;
+  for (DBuf *b : {&PSI, &W1, &W2, &W3, &HW}) HTRY(b->alloc(nblk));
+  HTRY(dE0.alloc(n)); HTRY(dEsc.alloc(n)); HTRY(dCs.alloc(n)); HTRY(dSn.alloc(n)); HTRY(dNorm.alloc(n));
+  HTRY(dC.alloc((int64_t)npoly * n));
+  int *dSys = nullptr, *dAct = nullptr;
+  int64_t *dOff = nullptr;
+  struct Free {
+    void *p[3];
+    ~Free() { for (void *q : p) if (q) (void)hipFree(q); }
+  } guard{{nullptr, nullptr, nullptr}};
+  HTRY(hipMalloc((void **)&dSys, sizeof(int) * std::max<int64_t>(nbt, 1)));
+  guard.p[0] = dSys;
+  HTRY(hipMalloc((void **)&dAct, sizeof(int) * n));
+  guard.p[1] = dAct;
+  HTRY(hipMalloc((void **)&dOff, sizeof(int64_t) * (n + 1)));
+  guard.p[2] = dOff;
+  HTRY(evr::upload_now(dSys, sys.data(), sizeof(int) * nbt, st));
+  HTRY(evr::upload_now(dOff, off.data(), sizeof(int64_t) * (n + 1), st));
+  HTRY(evr::upload_now(dE0.p, e0.data(), sizeof(double) * n, st));
+  HTRY(evr::upload_now(dEsc.p, esc.data(), sizeof(double) * n, st));
+  HTRY(evr::upload_now(dCs.p, cs.data(), sizeof(double) * n, st));
+  HTRY(evr::upload_now(dSn.p, sn.data(), sizeof(double) * n, st));
+  HTRY(evr::upload_now(dC.p, ctab.data(), sizeof(double) * ctab.size(), st));
+  HTRY(hipMemcpyAsync(PSI.p, psi, sizeof(double) * nblk,
+                      device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  const unsigned nbk = nblocks(nbt);
+  int64_t hpsi = 0;
+  int max_order = 0;
+  std::vector<int> act(n);
+  std::vector<double> nrm(n);
+  for (int step = 0; step < nsteps; ++step) {
+    std::fill(act.begin(), act.end(), 1);
+    HTRY(evr::upload_now(dAct, act.data(), sizeof(int) * n, st));
+    double *w0 = W1.p, *w1 = W2.p, *w2 = W3.p;
+    // order 0: w0 = psi, psi = J_0 psi; order 1: w1 = -i Hs w0, psi += 2 J_1 w1
+    HTRY(hipMemcpyAsync(w0, PSI.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_mscale, dim3(nbk), dim3(256), 0, st, nbt, dSys, dC.p, w0, PSI.p);
+    HTRY(hipGetLastError());
+    RTRY(evr_multi_oppsi(m, 2, w0, HW.p, EVR_DEVICE_PTRS));
+    hipLaunchKernelGGL(k_mcheb_order, dim3(nbk), dim3(256), 0, st, nbt, dSys, dE0.p, dEsc.p, dC.p + n, dAct,
+                       nullptr, w0, HW.p, w1, PSI.p);
+    HTRY(hipGetLastError());
+    hpsi += 2;
+    int left = n, order = 1;
+    for (order = 2; order < npoly && left > 0; ++order) {
+      RTRY(evr_multi_oppsi(m, 2, w1, HW.p, EVR_DEVICE_PTRS));
+      hipLaunchKernelGGL(k_mcheb_order, dim3(nbk), dim3(256), 0, st, nbt, dSys, dE0.p, dEsc.p,
+                         dC.p + (size_t)order * n, dAct, w0, w1, HW.p, w2, PSI.p);
+      HTRY(hipGetLastError());
+      hpsi += 2;
+      double *t = w0;
+      w0 = w1;
+      w1 = w2;
+      w2 = t;
+      // exit test per system (MARCH:4306-4335)
+      hipLaunchKernelGGL(k_msys_norm, dim3(n), dim3(256), 0, st, nbt, dOff, w1, dNorm.p);
+      HTRY(hipGetLastError());
+      HTRY(hipMemcpyAsync(nrm.data(), dNorm.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+      HTRY(hipStreamSynchronize(st));
+      bool changed = false;
+      for (int k = 0; k < n; ++k) {
+        if (!act[k]) continue;
+        const double crit = std::fabs(nrm[k] * ctab[(size_t)order * n + k]);
+        if (!(crit <= 1e15))
+          return set_error(EVR_ERR_STATE, "Chebyshev series of system " + std::to_string(k) +
+                                              " diverges: widen its interval");
+        if (crit < poly_tol) {
+          act[k] = 0;
+          --left;
+          changed = true;
+          max_order = std::max(max_order, order);
+        }
+      }
+      if (changed && left > 0) HTRY(evr::upload_now(dAct, act.data(), sizeof(int) * n, st));
+    }
+    if (left > 0)
+      return set_error(EVR_ERR_STATE, "Chebyshev expansion did not converge within npoly terms for " +
+                                          std::to_string(left) + " system(s): reduce dt or raise npoly");
+    // phase exp(-i E0_k dt) (MARCH:4386-4388)
+    hipLaunchKernelGGL(k_mphase, dim3(nbk), dim3(256), 0, st, nbt, dSys, dCs.p, dSn.p, PSI.p);
+    HTRY(hipGetLastError());
+  }
+  HTRY(hipMemcpyAsync(psi, PSI.p, sizeof(double) * nblk,
+                      device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+  HTRY(hipStreamSynchronize(st));
+  if (info) {
+    info->hpsi = hpsi;
+    info->max_order = max_order;
+    info->retries = 0;
+    info->Hmin = *std::min_element(Hmin, Hmin + n);
+    info->Hmax = *std::max_element(Hmax, Hmax + n);
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
   return EVR_OK;
 }

@@ -1088,6 +1088,17 @@ __global__ __launch_bounds__(256) void k_to_float(const double *__restrict__ a, 
     b[g] = (float)a[g];
 }
 
+// dst[i] = src[i] + add, entries below zero (pruned) kept at -1
+__global__ __launch_bounds__(256) void k_offset_ints(const int *__restrict__ src, int64_t n, int64_t add,
+                                                     int *__restrict__ dst)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int v = src[g];
+    dst[g] = v >= 0 ? (int)(v + add) : -1;
+  }
+}
+
 // Zero fill with 64-bit indices (the assembled pools exceed 2^34 words; a
 // fill kernel is also safe inside captured graphs, unlike memset nodes).
 template <typename T>

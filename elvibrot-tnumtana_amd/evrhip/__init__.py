@@ -298,3 +298,55 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Multi:
+    """Several assembled plans (evr_plan_assemble with ASM_FP64) as one
+    block-diagonal operator over the concatenation of their packed vectors
+    (evr_multi_create): one H|psi> launch sequence for all systems."""
+
+    def __init__(self, plans, max_nvec=2):
+        arr = (ctypes.c_void_p * len(plans))(*[p.h for p in plans])
+        h = ctypes.c_void_p()
+        check(load().evr_multi_create(arr, len(plans), max_nvec, ctypes.byref(h)))
+        self.h = h
+        self.n = len(plans)
+        off = (ctypes.c_int64 * (self.n + 1))()
+        check(load().evr_multi_offsets(self.h, off))
+        self.offsets = np.array(off[:], dtype=np.int64)
+        self.max_nvec = max_nvec
+
+    @property
+    def nb_total(self):
+        return int(self.offsets[-1])
+
+    def TabOpPsi(self, psi):
+        psi = np.asfortranarray(psi, dtype=np.float64)
+        one = psi.ndim == 1
+        if one:
+            psi = psi.reshape(-1, 1, order="F")
+        out = np.zeros_like(psi, order="F")
+        check(load().evr_multi_oppsi(self.h, psi.shape[1], _d(psi), _d(out), 0))
+        return out[:, 0] if one else out
+
+    def propagate(self, psi, dt, nsteps, Hmin, Hmax, npoly=200, poly_tol=1e-12):
+        """march_cheby for one complex wavepacket per system (psi: the
+        concatenation, length nb_total), each system with its own interval."""
+        r = np.asfortranarray(np.stack([np.real(psi), np.imag(psi)], axis=1), dtype=np.float64)
+        lo = np.ascontiguousarray(Hmin, dtype=np.float64)
+        hi = np.ascontiguousarray(Hmax, dtype=np.float64)
+        info = PropaInfo()
+        check(load().evr_multi_propagate(self.h, dt, nsteps, npoly, poly_tol, _d(lo), _d(hi),
+                                         r.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(info)))
+        return r[:, 0] + 1j * r[:, 1], info
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().evr_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -98,6 +98,16 @@ _SIGS = {
                                             ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                             ctypes.POINTER(ControlInfo)]),
     "evr_bessel_j": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _dp]),
+    "evr_multi_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "evr_multi_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "evr_multi_offsets": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "evr_multi_count": (ctypes.c_int, [ctypes.c_void_p]),
+    "evr_multi_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "evr_multi_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+    "evr_multi_propagate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, _dp, _dp, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.POINTER(PropaInfo)]),
 }
 
 _LIB = None

@@ -74,6 +74,12 @@ class Hamiltonian:
         """a_ijk flattened as i + D*(j + D*k) (symmetric, so order is immaterial)."""
         return None if self.Ga is None else np.ascontiguousarray(self.Ga).ravel(order="F")
 
+    def scaled(self, f):
+        """The same Hamiltonian with every coefficient times f (a change of
+        energy unit, e.g. 1 / AU_TO_CM from cm^-1 to hartree)."""
+        return Hamiltonian(self.D, f * self.omega, f * self.c3, f * self.d22, f * self.e4,
+                           None if self.Ga is None else f * self.Ga)
+
 
 def _upper_cubic(D, rng, scale):
     c3 = np.zeros((D, D, D))

@@ -173,6 +173,25 @@ int evr_plan_max_nvec(const evr_plan *p);
 int evr_plan_assemble(evr_plan *p, int which);
 int evr_plan_assembled(const evr_plan *p, double *bytes);
 int evr_plan_release_assembled(evr_plan *p);
+
+/* Several systems as one block-diagonal operator (no counterpart in the
+ * reference, which treats one system per run): the FP64 assembled operators
+ * of n plans (each assembled with EVR_ASM_FP64, any sizes and dimensions)
+ * are copied into one pool with concatenated tables, so one H|psi> over the
+ * concatenated packed vectors of all systems is one launch sequence (gather,
+ * MFMA apply over every system's terms, scatter).  The plans may be released
+ * or destroyed afterwards.  nb_offsets (n + 1 values) gives where system k's
+ * packed functions start in the concatenation. */
+typedef struct evr_multi evr_multi;
+int evr_multi_create(evr_plan *const *plans, int n, int max_nvec, evr_multi **out);
+int evr_multi_destroy(evr_multi *m);
+int evr_multi_offsets(const evr_multi *m, int64_t *nb_offsets);
+/* Number of systems n. */
+int evr_multi_count(const evr_multi *m);
+void *evr_multi_stream(evr_multi *m);
+/* hpsi = H psi for nvec columns of sum_k nb_k rows (column-major); flags:
+ * EVR_DEVICE_PTRS for device pointers on evr_multi_stream. */
+int evr_multi_oppsi(evr_multi *m, int nvec, const double *psi, double *hpsi, int flags);
 /* Profiling (no counterpart in the reference, whose analogue is the
  * nb_mult_BTOG/GTOB counters, SYS:120): when on, every term-kernel launch is
  * bracketed by HIP events on the plan stream; evr_plan_kernel_stats returns,
@@ -307,6 +326,19 @@ typedef struct evr_propa_info {
  * pairs, Calc_AutoCorr) or NULL. */
 int evr_propagate(evr_plan *H, const evr_propa_opts *opts, int nwp, double *psi,
                   double *autocorr, evr_propa_info *info);
+
+/* march_cheby on a multi-system operator (evr_multi_create), one complex
+ * wavepacket per system (psi: sum_k nb_k x 2, Re and Im columns), every
+ * system with its own interval [Hmin[k], Hmax[k]] (its E0, Esc and Bessel
+ * coefficients) and its own adaptive order per step (the exit test
+ * |c_K| <w_K|w_K> < poly_tol per system: a system whose series has ended takes
+ * no further terms while the others continue); all systems advance by one
+ * H|psi> launch sequence per order.  No counterpart in the reference (one
+ * system per run).  info->max_order is the largest order used; the intervals
+ * must contain the spectra (EVR_ERR_STATE when a series diverges). */
+int evr_multi_propagate(evr_multi *m, double dt, int nsteps, int npoly, double poly_tol,
+                        const double *Hmin, const double *Hmax, double *psi, int device_ptrs,
+                        evr_propa_info *info);
 
 /* ---- optimal control: objective and adjoint gradient ----------------------- */
 /* nf fields, each E_f(t) = sin^2(pi t/T) sum_{k=1..nk} a_kf sin(k pi t/T),
