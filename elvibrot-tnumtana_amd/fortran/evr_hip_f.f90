@@ -232,6 +232,50 @@ module evr_hip_f
       type(c_ptr), value :: p
       integer(c_int) :: evr_plan_release_assembled
     end function
+    function evr_multi_create(plans, n, max_nvec, m) bind(C, name="evr_multi_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: plans(*)
+      integer(c_int), value :: n, max_nvec
+      type(c_ptr), intent(out) :: m
+      integer(c_int) :: evr_multi_create
+    end function
+    function evr_multi_destroy(m) bind(C, name="evr_multi_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int) :: evr_multi_destroy
+    end function
+    function evr_multi_offsets(m, nb_offsets) bind(C, name="evr_multi_offsets")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: m
+      integer(c_int64_t), intent(out) :: nb_offsets(*)
+      integer(c_int) :: evr_multi_offsets
+    end function
+    function evr_multi_count(m) bind(C, name="evr_multi_count")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int) :: evr_multi_count
+    end function
+    function evr_multi_stream(m) bind(C, name="evr_multi_stream")
+      import :: c_ptr
+      type(c_ptr), value :: m
+      type(c_ptr) :: evr_multi_stream
+    end function
+    function evr_multi_oppsi(m, nvec, psi, hpsi, flags) bind(C, name="evr_multi_oppsi")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m, psi, hpsi
+      integer(c_int), value :: nvec, flags
+      integer(c_int) :: evr_multi_oppsi
+    end function
+    function evr_multi_propagate(m, dt, nsteps, npoly, poly_tol, Hmin, Hmax, psi, device_ptrs, info) &
+        bind(C, name="evr_multi_propagate")
+      import :: c_int, c_ptr, c_double, evr_propa_info
+      type(c_ptr), value :: m, psi
+      real(c_double), value :: dt, poly_tol
+      integer(c_int), value :: nsteps, npoly, device_ptrs
+      real(c_double), intent(in) :: Hmin(*), Hmax(*)
+      type(evr_propa_info), intent(out) :: info
+      integer(c_int) :: evr_multi_propagate
+    end function
     function evr_plan_bounds(p, steps, Hmin, Hmax) bind(C, name="evr_plan_bounds")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: p
