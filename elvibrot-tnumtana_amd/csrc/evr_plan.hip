@@ -39,6 +39,7 @@ struct DevBuf {
   size_t n = 0;
   hipError_t alloc(size_t count)
   {
+    release();
     n = count;
     if (count == 0) return hipSuccess;
     return hipMalloc((void **)&ptr, sizeof(X) * count);
@@ -57,6 +58,10 @@ struct DevBuf {
     ptr = nullptr;
     n = 0;
   }
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
 };
 
 // work class of the term kernel: threads per workgroup x elements per thread
@@ -132,6 +137,25 @@ struct evr_plan {
   double ms_acc[kNumClasses + 1] = {};
   double flops_acc[kNumClasses + 1] = {};
   int64_t launches[kNumClasses + 1] = {};
+
+  evr_plan() = default;
+  evr_plan(const evr_plan &) = delete;
+  evr_plan &operator=(const evr_plan &) = delete;
+  ~evr_plan()  // also on a failed evr_plan_create
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    d_terms.release(); d_slots.release(); d_steps.release(); d_bw64.release(); d_keo64.release();
+    d_x.release(); d_bwdiag.release(); d_bw32.release(); d_keo32.release();
+    d_map.release(); d_seg.release(); d_fseg.release(); d_P.release(); d_rowpos.release(); d_V.release();
+    d_Veff.release(); d_Veff32.release(); d_gw.release();
+    d_coefC.release(); d_coefA.release(); d_coefF1.release(); d_keoc.release();
+    for (int c = 0; c < kNumClasses; ++c) d_order[c].release();
+    d_X.release(); d_R.release(); d_psi.release(); d_hpsi.release(); d_G.release();
+    d_A64.release(); d_A32.release(); d_asm_items.release();
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 namespace {
@@ -944,20 +968,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
 
 int evr_plan_destroy(evr_plan *p)
 {
-  if (!p) return EVR_OK;
-  if (p->stream) (void)hipStreamSynchronize(p->stream);
-  p->d_terms.release(); p->d_slots.release(); p->d_steps.release(); p->d_bw64.release(); p->d_keo64.release();
-  p->d_x.release(); p->d_bwdiag.release(); p->d_bw32.release(); p->d_keo32.release();
-  p->d_map.release(); p->d_seg.release(); p->d_fseg.release(); p->d_P.release(); p->d_rowpos.release(); p->d_V.release();
-  p->d_Veff.release(); p->d_Veff32.release(); p->d_gw.release();
-  p->d_coefC.release(); p->d_coefA.release(); p->d_coefF1.release(); p->d_keoc.release();
-  for (int c = 0; c < kNumClasses; ++c) p->d_order[c].release();
-  p->d_X.release(); p->d_R.release(); p->d_psi.release(); p->d_hpsi.release(); p->d_G.release();
-  p->d_A64.release(); p->d_A32.release(); p->d_asm_items.release();
-  for (auto &e : p->ev)
-    if (e) (void)hipEventDestroy(e);
-  if (p->stream) (void)hipStreamDestroy(p->stream);
-  delete p;
+  delete p;  // ~evr_plan releases the device buffers, events and stream
   return EVR_OK;
 }
 
@@ -1319,6 +1330,10 @@ struct evr_multi {
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
+  evr_multi() = default;
+  evr_multi(const evr_multi &) = delete;
+  evr_multi &operator=(const evr_multi &) = delete;
+  ~evr_multi() { release(); }  // also on a failed evr_multi_create
 };
 
 namespace {
@@ -1426,9 +1441,7 @@ int evr_multi_create(evr_plan *const *plans, int n, int max_nvec, evr_multi **ou
 
 int evr_multi_destroy(evr_multi *m)
 {
-  if (!m) return EVR_OK;
-  m->release();
-  delete m;
+  delete m;  // releases its buffers and stream
   return EVR_OK;
 }
 
