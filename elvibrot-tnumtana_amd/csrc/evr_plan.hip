@@ -522,6 +522,16 @@ int evr_sg4_hermite_desc(int D, int LG, int LB, int nbA, int nbB, int nqA, int n
   return wrap_desc(o, out);
 }
 
+int evr_sg4_hermite_desc_modes(int D, int LG, int LB, const int *nbA, const int *nbB, const int *nqA,
+                               const int *nqB, evr_basis_desc **out)
+{
+  if (!out || !nbA || !nbB || !nqA || !nqB) return fail(EVR_ERR_ARG, "NULL argument");
+  std::unique_ptr<evr_desc_owner> o(new evr_desc_owner());
+  std::string err = evr::build_sg4_hermite_modes(D, LG, LB, nbA, nbB, nqA, nqB, o->tab);
+  if (!err.empty()) return fail(EVR_ERR_ARG, err);
+  return wrap_desc(o, out);
+}
+
 int evr_dp_hermite_desc(int D, const int *nb, const int *nq, evr_basis_desc **out)
 {
   if (!out || !nb || !nq) return fail(EVR_ERR_ARG, "NULL argument");
@@ -1478,6 +1488,141 @@ int evr_multi_oppsi(evr_multi *m, int nvec, const double *psi, double *hpsi, int
   if (!dev) {
     HIP_TRY(hipMemcpyAsync(hpsi, dh, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
+  }
+  return EVR_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ coupled states (nb0) --
+// H_ij blocks over one packed basis: (H psi)_i = sum_j H_ij psi_j, each block
+// a plan of its own (the diagonal ones with the kinetic part and V_ii, the
+// off-diagonal ones type 0 with V_ij), applied through its fastest path.
+struct evr_coupled {
+  int nb0 = 0, max_nvec = 0;
+  int64_t nb = 0;
+  std::vector<evr_plan *> blocks;  // nb0 x nb0, row-major, NULL = zero block
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  DevBuf<double> xin, tmp, acc, hin, hout;
+  evr_coupled() = default;
+  evr_coupled(const evr_coupled &) = delete;
+  evr_coupled &operator=(const evr_coupled &) = delete;
+  ~evr_coupled()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// dst (nb x nvec, column stride ldd) = src (nb x nvec, column stride lds) on s
+hipError_t copy_cols(double *dst, int64_t ldd, const double *src, int64_t lds, int64_t nb, int nvec,
+                     hipStream_t s)
+{
+  return hipMemcpy2DAsync(dst, sizeof(double) * ldd, src, sizeof(double) * lds, sizeof(double) * nb, nvec,
+                          hipMemcpyDeviceToDevice, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int evr_coupled_create(evr_plan *const *blocks, int nb0, evr_coupled **out)
+{
+  if (!blocks || !out || nb0 < 1) return fail(EVR_ERR_ARG, "bad argument");
+  *out = nullptr;
+  auto c = std::make_unique<evr_coupled>();
+  c->nb0 = nb0;
+  c->blocks.assign(blocks, blocks + (size_t)nb0 * nb0);
+  for (int i = 0; i < nb0; ++i) {
+    bool row = false;
+    for (int j = 0; j < nb0; ++j) {
+      const evr_plan *p = c->blocks[(size_t)i * nb0 + j];
+      if (!p) continue;
+      row = true;
+      if (c->nb == 0) {
+        c->nb = p->nb_packed;
+        c->max_nvec = p->max_nvec;
+      }
+      if (p->nb_packed != c->nb) return fail(EVR_ERR_ARG, "blocks on different basis sizes");
+      c->max_nvec = std::min(c->max_nvec, p->max_nvec);
+    }
+    if (!row) return fail(EVR_ERR_ARG, "state " + std::to_string(i) + " has no block");
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
+  const size_t blk = (size_t)c->nb * c->max_nvec;
+  HIP_TRY(c->xin.alloc(blk));
+  HIP_TRY(c->tmp.alloc(blk));
+  HIP_TRY(c->acc.alloc(blk));
+  HIP_TRY(c->hin.alloc(blk * nb0));
+  HIP_TRY(c->hout.alloc(blk * nb0));
+  *out = c.release();
+  return EVR_OK;
+}
+
+int evr_coupled_destroy(evr_coupled *c)
+{
+  delete c;
+  return EVR_OK;
+}
+
+int evr_coupled_sizes(const evr_coupled *c, int64_t *nb, int *nb0, int *max_nvec)
+{
+  if (!c) return fail(EVR_ERR_ARG, "NULL argument");
+  if (nb) *nb = c->nb;
+  if (nb0) *nb0 = c->nb0;
+  if (max_nvec) *max_nvec = c->max_nvec;
+  return EVR_OK;
+}
+
+void *evr_coupled_stream(evr_coupled *c) { return c ? (void *)c->stream : nullptr; }
+
+int evr_coupled_oppsi(evr_coupled *c, int nvec, const double *psi, double *hpsi, int flags)
+{
+  if (!c || !psi || !hpsi) return fail(EVR_ERR_ARG, "NULL argument");
+  if (nvec < 1 || nvec > c->max_nvec) return fail(EVR_ERR_ARG, "nvec must be in 1..max_nvec");
+  const bool dev = (flags & EVR_DEVICE_PTRS) != 0;
+  const int nb0 = c->nb0;
+  const int64_t nb = c->nb, ld = nb * nb0, n = ld * nvec;
+  const double *dpsi = psi;
+  double *dh = hpsi;
+  if (!dev) {
+    HIP_TRY(hipMemcpyAsync(c->hin.ptr, psi, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    dpsi = c->hin.ptr;
+    dh = c->hout.ptr;
+  }
+  const int bflags = (flags & (EVR_FP32 | EVR_MATRIX_FREE)) | EVR_DEVICE_PTRS;
+  for (int i = 0; i < nb0; ++i) {
+    bool first = true;
+    for (int j = 0; j < nb0; ++j) {
+      evr_plan *p = c->blocks[(size_t)i * nb0 + j];
+      if (!p) continue;
+      // psi_j (rows j nb .. (j+1) nb - 1 of every column) -> contiguous
+      HIP_TRY(copy_cols(c->xin.ptr, nb, dpsi + j * nb, ld, nb, nvec, c->stream));
+      HIP_TRY(hipEventRecord(c->fork, c->stream));
+      HIP_TRY(hipStreamWaitEvent(p->stream, c->fork, 0));
+      const int rc = evr_oppsi(p, nvec, c->xin.ptr, first ? c->acc.ptr : c->tmp.ptr, bflags);
+      if (rc) return rc;
+      HIP_TRY(hipEventRecord(c->join, p->stream));
+      HIP_TRY(hipStreamWaitEvent(c->stream, c->join, 0));
+      if (!first) {
+        hipLaunchKernelGGL(evr::k_axpy, dim3(blocks_for(nb * nvec)), dim3(256), 0, c->stream, nb * nvec,
+                           1.0, c->tmp.ptr, c->acc.ptr);
+        HIP_TRY(hipGetLastError());
+      }
+      first = false;
+    }
+    HIP_TRY(copy_cols(dh + i * nb, ld, c->acc.ptr, nb, nb, nvec, c->stream));
+  }
+  if (!dev) {
+    HIP_TRY(hipMemcpyAsync(hpsi, dh, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
   }
   return EVR_OK;
 }

@@ -377,12 +377,47 @@ std::vector<cplx> expm_tridiag_e1(int k, const double *alpha, const double *beta
   return u;
 }
 
-int lanczos_bounds(evr_plan *p, int steps, double *lo, double *hi)
+// ---- operator handle ---------------------------------------------------------
+
+int OpRef::rows(int64_t *nb) const
 {
-  int64_t sz[7];
-  RTRY(evr_plan_sizes(p, sz));
-  const int64_t nb = sz[0];
-  hipStream_t st = (hipStream_t)evr_plan_stream(p);
+  if (plan) {
+    int64_t sz[7];
+    RTRY(evr_plan_sizes(plan, sz));
+    *nb = sz[0];
+    return EVR_OK;
+  }
+  int64_t n = 0;
+  int nb0 = 0;
+  RTRY(evr_coupled_sizes(coupled, &n, &nb0, nullptr));
+  *nb = n * nb0;
+  return EVR_OK;
+}
+
+hipStream_t OpRef::stream() const
+{
+  return (hipStream_t)(plan ? evr_plan_stream(plan) : evr_coupled_stream(coupled));
+}
+
+int OpRef::max_nvec() const
+{
+  if (plan) return evr_plan_max_nvec(plan);
+  int mv = 0;
+  if (evr_coupled_sizes(coupled, nullptr, nullptr, &mv) != EVR_OK) return 0;
+  return mv;
+}
+
+int OpRef::apply(int nvec, const double *X, double *Y, int flags) const
+{
+  return plan ? evr_oppsi(plan, nvec, X, Y, flags | EVR_DEVICE_PTRS)
+              : evr_coupled_oppsi(coupled, nvec, X, Y, flags | EVR_DEVICE_PTRS);
+}
+
+int lanczos_bounds(OpRef p, int steps, double *lo, double *hi)
+{
+  int64_t nb = 0;
+  RTRY(p.rows(&nb));
+  hipStream_t st = p.stream();
   DBuf Q, w, red, cf;
   HTRY(Q.alloc(nb * (steps + 1)));
   HTRY(w.alloc(nb));
@@ -402,7 +437,7 @@ int lanczos_bounds(evr_plan *p, int steps, double *lo, double *hi)
   HTRY(upload_now(Q.p, h.data(), sizeof(double) * nb, st));
   std::vector<double> al, be(1, 0.0);
   for (int k = 0; k < steps; ++k) {
-    RTRY(evr_oppsi(p, 1, Q.p + nb * k, w.p, EVR_FP64 | EVR_DEVICE_PTRS));
+    RTRY(p.apply(1, Q.p + nb * k, w.p, EVR_FP64));
     // c_j = <q_j|H q_k>, j <= k; w -= sum_j c_j q_j (Lanczos with full
     // re-orthogonalisation: c_k is alpha_k, c_{k-1} is beta_k; a single pass
     // loses orthogonality within a few tens of steps)
@@ -454,20 +489,18 @@ int lanczos_bounds(evr_plan *p, int steps, double *lo, double *hi)
 
 // ---- batch context ---------------------------------------------------------
 
-int Batch::init(evr_plan *H_, evr_plan *mu_, int nwp_, int flags_)
+int Batch::init(OpRef H_, OpRef mu_, int nwp_, int flags_)
 {
   H = H_;
   mu = mu_;
   nwp = nwp_;
   flags = flags_;
-  int64_t sz[7];
-  RTRY(evr_plan_sizes(H, sz));
-  nb = sz[0];
-  st = (hipStream_t)evr_plan_stream(H);
+  RTRY(H.rows(&nb));
+  st = H.stream();
   if (mu) {
-    int64_t s2[7];
-    RTRY(evr_plan_sizes(mu, s2));
-    if (s2[0] != nb) return set_error(EVR_ERR_ARG, "coupling plan has another basis size");
+    int64_t n2 = 0;
+    RTRY(mu.rows(&n2));
+    if (n2 != nb) return set_error(EVR_ERR_ARG, "coupling plan has another basis size");
   }
   HTRY(dred.alloc(2 * (int64_t)nwp * 64));
   HTRY(dcoef.alloc(2 * (int64_t)nwp * 65));
@@ -483,21 +516,21 @@ Batch::~Batch()
   if (join) (void)hipEventDestroy(join);
 }
 
-int Batch::apply(evr_plan *p, const double *X, double *Y, int fl)
+int Batch::apply(OpRef p, const double *X, double *Y, int fl)
 {
   const int ncol = 2 * nwp;
-  const int mv = std::max(1, evr_plan_max_nvec(p));
-  // evr_oppsi runs on the stream of plan p: ordered after this batch's work
-  // and before what follows by events (no host wait, so a captured step can
-  // take the coupling plan's stream along)
-  hipStream_t ps = (hipStream_t)evr_plan_stream(p);
+  const int mv = std::max(1, p.max_nvec());
+  // the operator runs on its own stream: ordered after this batch's work and
+  // before what follows by events (no host wait, so a captured step can take
+  // the coupling plan's stream along)
+  hipStream_t ps = p.stream();
   if (ps != st) {
     HTRY(hipEventRecord(fork, st));
     HTRY(hipStreamWaitEvent(ps, fork, 0));
   }
   for (int c0 = 0; c0 < ncol; c0 += mv) {
     const int k = std::min(mv, ncol - c0);
-    RTRY(evr_oppsi(p, k, X + nb * c0, Y + nb * c0, fl | EVR_DEVICE_PTRS));
+    RTRY(p.apply(k, X + nb * c0, Y + nb * c0, fl));
   }
   if (ps != st) {
     HTRY(hipEventRecord(join, ps));
@@ -774,8 +807,26 @@ static int replay_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double poly_to
 using namespace evr::propa;
 using evr::set_error;
 
+static int propagate_impl(OpRef H, const evr_propa_opts *o, int nwp, double *psi, double *autocorr,
+                          evr_propa_info *info);
+
 extern "C" int evr_propagate(evr_plan *H, const evr_propa_opts *o, int nwp, double *psi,
                              double *autocorr, evr_propa_info *info)
+{
+  if (!H) return set_error(EVR_ERR_ARG, "NULL plan");
+  return propagate_impl(OpRef(H), o, nwp, psi, autocorr, info);
+}
+
+extern "C" int evr_coupled_propagate(evr_coupled *C, const evr_propa_opts *o, int nwp, double *psi,
+                                     double *autocorr, evr_propa_info *info)
+{
+  if (!C) return set_error(EVR_ERR_ARG, "NULL coupled operator");
+  if (o && (o->field || o->mu)) return set_error(EVR_ERR_ARG, "no field coupling for coupled states");
+  return propagate_impl(OpRef(C), o, nwp, psi, autocorr, info);
+}
+
+static int propagate_impl(OpRef H, const evr_propa_opts *o, int nwp, double *psi, double *autocorr,
+                          evr_propa_info *info)
 {
   if (!H || !o || !psi || nwp < 1) return set_error(EVR_ERR_ARG, "bad argument");
   if (o->method != EVR_PROPA_CHEBY && o->method != EVR_PROPA_SIL)

@@ -16,6 +16,23 @@ namespace propa {
 
 using cplx = std::complex<double>;
 
+// The operator a driver applies: one plan, or coupled states over one basis.
+struct OpRef {
+  evr_plan *plan = nullptr;
+  evr_coupled *coupled = nullptr;
+  OpRef() = default;
+  OpRef(evr_plan *p) : plan(p) {}
+  OpRef(evr_coupled *c) : coupled(c) {}
+  explicit operator bool() const { return plan || coupled; }
+  bool operator==(const OpRef &o) const { return plan == o.plan && coupled == o.coupled; }
+  bool operator!=(const OpRef &o) const { return !(*this == o); }
+  int rows(int64_t *nb) const;   // length of one column
+  hipStream_t stream() const;
+  int max_nvec() const;
+  // Y = H X for nvec columns (device pointers, on stream())
+  int apply(int nvec, const double *X, double *Y, int flags) const;
+};
+
 struct DBuf {
   double *p = nullptr;
   hipError_t alloc(int64_t k)
@@ -43,11 +60,11 @@ void sym_eig(int n, std::vector<double> A, std::vector<double> &ev, std::vector<
 // off-diagonal beta[1..k-1]).
 std::vector<cplx> expm_tridiag_e1(int k, const double *alpha, const double *beta, double dt);
 // Extreme eigenvalues of a symmetric plan from `steps` Lanczos steps.
-int lanczos_bounds(evr_plan *p, int steps, double *lo, double *hi);
+int lanczos_bounds(OpRef p, int steps, double *lo, double *hi);
 
 // nwp complex wavepackets on one plan (and an optional coupling plan).
 struct Batch {
-  evr_plan *H = nullptr, *mu = nullptr;
+  OpRef H, mu;
   hipStream_t st = nullptr;
   int64_t nb = 0;
   int nwp = 0, flags = 0;
@@ -60,9 +77,9 @@ struct Batch {
   Batch &operator=(const Batch &) = delete;
   ~Batch();
 
-  int init(evr_plan *H, evr_plan *mu, int nwp, int flags);
+  int init(OpRef H, OpRef mu, int nwp, int flags);
   // Y = p X on all 2 nwp columns (chunks of the plan's max_nvec)
-  int apply(evr_plan *p, const double *X, double *Y, int flags);
+  int apply(OpRef p, const double *X, double *Y, int flags);
   // HX = H X and, when a field is set, MX = mu X
   int apply_field(const double *X, double *HX, double *MX, const double *dE);
   // out[j*nwp + w] = <X_j,w | Y_w>, X_j = X + j*xs, j < nk <= 64

@@ -228,25 +228,37 @@ std::string build_sg4_hermite(int D, int LG, int LB, int nbA, int nbB, int nqA, 
                               BasisTables &t)
 {
   if (D < 1 || D > kMaxD) return "D out of range (1..16)";
+  const std::vector<int> a(D, nbA), b(D, nbB), qa(D, nqA), qb(D, nqB);
+  return build_sg4_hermite_modes(D, LG, LB, a.data(), b.data(), qa.data(), qb.data(), t);
+}
+
+std::string build_sg4_hermite_modes(int D, int LG, int LB, const int *nbA, const int *nbB,
+                                    const int *nqA, const int *nqB, BasisTables &t)
+{
+  if (D < 1 || D > kMaxD) return "D out of range (1..16)";
   if (LG < 0 || LB < 0 || LB > LG) return "need 0 <= LB <= LG";
-  if (nbA < 1 || nqA < 1 || nbB < 0 || nqB < 0) return "bad L_TO_n law";
-  auto nbL = [&](int l) { return nbA + nbB * l; };
-  auto nqL = [&](int l) { return nqA + nqB * l; };
-  for (int l = 0; l <= LG; ++l)
-    if (nqL(l) < nbL(std::min(l, LB))) return "nq(l) < nb(l): grid too small for the basis";
+  for (int d = 0; d < D; ++d)
+    if (nbA[d] < 1 || nqA[d] < 1 || nbB[d] < 0 || nqB[d] < 0) return "bad L_TO_n law";
+  // per mode d: nb(l) = nbA + nbB l, nq(l) = nqA + nqB l (the basis_nD
+  // L_TO_nq_A/B laws of each mode)
+  auto nbL = [&](int d, int l) { return nbA[d] + nbB[d] * l; };
+  auto nqL = [&](int d, int l) { return nqA[d] + nqB[d] * l; };
+  for (int d = 0; d < D; ++d)
+    for (int l = 0; l <= LG; ++l)
+      if (nqL(d, l) < nbL(d, std::min(l, LB))) return "nq(l) < nb(l): grid too small for the basis";
   t = BasisTables();
   t.D = D;
   t.nlev = LG + 1;
   t.nb.assign((size_t)D * t.nlev, 0);
   t.nq.assign((size_t)D * t.nlev, 0);
-  // primitive bases per level: nq(L), nb(min(L,LB)); identical for every mode
-  std::vector<Slot1D> lev(LG + 1);
-  for (int l = 0; l <= LG; ++l) hermite_slot(nbL(std::min(l, LB)), nqL(l), lev[l]);
+  // primitive bases per mode and level: nq(L), nb(min(L,LB))
   for (int d = 0; d < D; ++d)
     for (int l = 0; l <= LG; ++l) {
-      t.nb[d * t.nlev + l] = lev[l].nb;
-      t.nq[d * t.nlev + l] = lev[l].nq;
-      append_slot(t, lev[l]);
+      Slot1D s;
+      hermite_slot(nbL(d, std::min(l, LB)), nqL(d, l), s);
+      t.nb[d * t.nlev + l] = s.nb;
+      t.nq[d * t.nlev + l] = s.nq;
+      append_slot(t, s);
     }
   // Smolyak terms: levels with Lmin <= |l| <= LG, first mode fastest
   const int Lmin = std::max(0, LG - D + 1);
@@ -275,13 +287,16 @@ std::string build_sg4_hermite(int D, int LG, int LB, int nbA, int nbB, int nqA, 
     if (d == D) break;
   }
   t.nterms = (int64_t)t.weights.size();
-  // level of 1D function i: smallest L with nb(L) > i
-  const int nmax = nbL(LB);
-  std::vector<int> lev_of(nmax);
-  for (int i = 0; i < nmax; ++i) {
-    int L = 0;
-    while (nbL(L) <= i) ++L;
-    lev_of[i] = L;
+  // level of 1D function i of mode d: smallest L with nb_d(L) > i
+  std::vector<std::vector<int>> lev_of(D);
+  for (int d = 0; d < D; ++d) {
+    const int nmax = nbL(d, LB);
+    lev_of[d].resize(nmax);
+    for (int i = 0; i < nmax; ++i) {
+      int L = 0;
+      while (nbL(d, L) <= i) ++L;
+      lev_of[d][i] = L;
+    }
   }
   // cnt[d][b]: tuples of modes d..D-1 whose level sum is <= b
   const int Bp = LB + 1;
@@ -290,8 +305,8 @@ std::string build_sg4_hermite(int D, int LG, int LB, int nbA, int nbB, int nqA, 
   for (int d = D - 1; d >= 0; --d)
     for (int b = 0; b < Bp; ++b) {
       int64_t c = 0;
-      for (int i = 0; i < nmax; ++i)
-        if (lev_of[i] <= b) c += cnt[(size_t)(d + 1) * Bp + b - lev_of[i]];
+      for (int lv : lev_of[d])
+        if (lv <= b) c += cnt[(size_t)(d + 1) * Bp + b - lv];
       cnt[(size_t)d * Bp + b] = c;
     }
   t.nb_packed = cnt[LB];
@@ -315,9 +330,10 @@ std::string build_sg4_hermite(int D, int LG, int LB, int nbA, int nbB, int nqA, 
       int64_t rank = 0;
       bool ok = true;
       for (int d = 0; d < D && ok; ++d) {
+        const std::vector<int> &lv = lev_of[d];
         for (int m = 0; m < n[d]; ++m)
-          if (lev_of[m] <= bud) rank += cnt[(size_t)(d + 1) * Bp + bud - lev_of[m]];
-        bud -= lev_of[n[d]];
+          if (lv[m] <= bud) rank += cnt[(size_t)(d + 1) * Bp + bud - lv[m]];
+        bud -= lv[n[d]];
         if (bud < 0) ok = false;
       }
       t.map[off[k] + e] = ok ? (int)rank : -1;

@@ -54,6 +54,9 @@ struct BasisTables {
 
 std::string build_sg4_hermite(int D, int LG, int LB, int nbA, int nbB, int nqA, int nqB,
                               BasisTables &out);
+// The same with per-mode laws nb_d(l) = nbA[d] + nbB[d] l, nq_d(l) = nqA[d] + nqB[d] l.
+std::string build_sg4_hermite_modes(int D, int LG, int LB, const int *nbA, const int *nbB,
+                                    const int *nqA, const int *nqB, BasisTables &t);
 std::string build_dp_hermite(int D, const int *nb, const int *nq, BasisTables &out);
 
 }  // namespace evr

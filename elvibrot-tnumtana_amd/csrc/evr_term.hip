@@ -1088,6 +1088,15 @@ __global__ __launch_bounds__(256) void k_to_float(const double *__restrict__ a, 
     b[g] = (float)a[g];
 }
 
+// y += a x over n values
+__global__ __launch_bounds__(256) void k_axpy(int64_t n, double a, const double *__restrict__ x,
+                                              double *__restrict__ y)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+       g += (int64_t)gridDim.x * blockDim.x)
+    y[g] += a * x[g];
+}
+
 // dst[i] = src[i] + add, entries below zero (pruned) kept at -1
 __global__ __launch_bounds__(256) void k_offset_ints(const int *__restrict__ src, int64_t n, int64_t add,
                                                      int *__restrict__ dst)

@@ -83,6 +83,18 @@ class Basis:
         return cls(p)
 
     @classmethod
+    def sg4_modes(cls, D, LG, nbA, nbB, nqA, nqB, LB=None):
+        """SG4 Hermite basis with a law per mode: nb_d(l) = nbA[d] + nbB[d] l,
+        nq_d(l) = nqA[d] + nqB[d] l (sequences of length D)."""
+        LB = LG if LB is None else LB
+        arr = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.int32), (D,))) for v in
+               (nbA, nbB, nqA, nqB)]
+        p = ctypes.POINTER(BasisDesc)()
+        check(load().evr_sg4_hermite_desc_modes(D, LG, LB, *[a.ctypes.data_as(_ip) for a in arr],
+                                                ctypes.byref(p)))
+        return cls(p)
+
+    @classmethod
     def dp(cls, nb, nq):
         nb = np.asarray(nb, dtype=np.int32)
         nq = np.asarray(nq, dtype=np.int32)
@@ -343,6 +355,67 @@ class Multi:
     def close(self):
         if getattr(self, "h", None):
             load().evr_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Coupled:
+    """nb0 coupled electronic states over one basis (evr_coupled_create):
+    blocks[i][j] is the plan of H_ij (None for a zero block); a wavepacket is
+    nb0 blocks of nb_packed rows, vibrational index fastest."""
+
+    def __init__(self, blocks):
+        nb0 = len(blocks)
+        self.blocks = [p for row in blocks for p in row]  # keep the plans alive
+        arr = (ctypes.c_void_p * (nb0 * nb0))(*[None if p is None else p.h for p in self.blocks])
+        h = ctypes.c_void_p()
+        check(load().evr_coupled_create(arr, nb0, ctypes.byref(h)))
+        self.h = h
+        nb = ctypes.c_int64()
+        n0 = ctypes.c_int()
+        mv = ctypes.c_int()
+        check(load().evr_coupled_sizes(self.h, ctypes.byref(nb), ctypes.byref(n0), ctypes.byref(mv)))
+        self.nb_packed, self.nb0, self.max_nvec = nb.value, n0.value, mv.value
+
+    def TabOpPsi(self, psi, flags=FP64):
+        psi = np.asfortranarray(psi, dtype=np.float64)
+        one = psi.ndim == 1
+        if one:
+            psi = psi.reshape(-1, 1, order="F")
+        out = np.zeros_like(psi, order="F")
+        for c0 in range(0, psi.shape[1], self.max_nvec):
+            c1 = min(psi.shape[1], c0 + self.max_nvec)
+            x = np.asfortranarray(psi[:, c0:c1])
+            y = np.zeros_like(x, order="F")
+            check(load().evr_coupled_oppsi(self.h, c1 - c0, _d(x), _d(y), flags))
+            out[:, c0:c1] = y
+        return out[:, 0] if one else out
+
+    def propagate(self, psi, dt, nsteps, method=CHEBY, npoly=200, poly_tol=1e-12, Hmin=0.0, Hmax=0.0,
+                  psi0=None, autocorr=False, graph=False):
+        """march_cheby / march_SIL on the coupled states (see Plan.propagate)."""
+        r = _to_pairs(psi)
+        nwp = r.shape[1] // 2
+        p0 = None if psi0 is None else _to_pairs(psi0)
+        o = PropaOpts(method, dt, nsteps, npoly, poly_tol, Hmin, Hmax, None, None,
+                      None if p0 is None else p0.ctypes.data_as(ctypes.c_void_p), FP64, 0, 1 if graph else 0)
+        ac = np.zeros((nsteps, nwp, 2)) if autocorr else None
+        info = PropaInfo()
+        check(load().evr_coupled_propagate(self.h, ctypes.byref(o), nwp, r.ctypes.data_as(ctypes.c_void_p),
+                                           _d(ac), ctypes.byref(info)))
+        out = _from_pairs(r)
+        if np.ndim(psi) == 1:
+            out = out[:, 0]
+        return out, (None if ac is None else ac[..., 0] + 1j * ac[..., 1]), info
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().evr_coupled_destroy(self.h)
             self.h = None
 
     def __del__(self):

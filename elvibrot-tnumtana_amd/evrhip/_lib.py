@@ -66,6 +66,8 @@ class ControlInfo(ctypes.Structure):
 _SIGS = {
     "evr_last_error": (ctypes.c_char_p, []),
     "evr_sg4_hermite_desc": (ctypes.c_int, [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.POINTER(BasisDesc))]),
+    "evr_sg4_hermite_desc_modes": (ctypes.c_int, [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 4 +
+                                   [ctypes.POINTER(ctypes.POINTER(BasisDesc))]),
     "evr_dp_hermite_desc": (ctypes.c_int, [ctypes.c_int, _ip, _ip, ctypes.POINTER(ctypes.POINTER(BasisDesc))]),
     "evr_basis_desc_free": (None, [ctypes.POINTER(BasisDesc)]),
     "evr_basis_desc_sizes": (ctypes.c_int, [ctypes.POINTER(BasisDesc), _i64p]),
@@ -108,6 +110,15 @@ _SIGS = {
     "evr_multi_propagate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_double, _dp, _dp, ctypes.c_void_p, ctypes.c_int,
                                            ctypes.POINTER(PropaInfo)]),
+    "evr_coupled_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "evr_coupled_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "evr_coupled_sizes": (ctypes.c_int, [ctypes.c_void_p, _i64p, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
+    "evr_coupled_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "evr_coupled_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+    "evr_coupled_propagate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PropaOpts), ctypes.c_int,
+                                             ctypes.c_void_p, _dp, ctypes.POINTER(PropaInfo)]),
 }
 
 _LIB = None

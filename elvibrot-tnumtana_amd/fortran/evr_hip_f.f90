@@ -93,6 +93,14 @@ module evr_hip_f
       type(c_ptr), intent(out) :: out
       integer(c_int) :: evr_sg4_hermite_desc
     end function
+    function evr_sg4_hermite_desc_modes(D, LG, LB, nbA, nbB, nqA, nqB, out) &
+        bind(C, name="evr_sg4_hermite_desc_modes")
+      import :: c_int, c_ptr
+      integer(c_int), value :: D, LG, LB
+      integer(c_int), intent(in) :: nbA(*), nbB(*), nqA(*), nqB(*)
+      type(c_ptr), intent(out) :: out
+      integer(c_int) :: evr_sg4_hermite_desc_modes
+    end function
     function evr_dp_hermite_desc(D, nb, nq, out) bind(C, name="evr_dp_hermite_desc")
       import :: c_int, c_ptr
       integer(c_int), value :: D
@@ -275,6 +283,47 @@ module evr_hip_f
       real(c_double), intent(in) :: Hmin(*), Hmax(*)
       type(evr_propa_info), intent(out) :: info
       integer(c_int) :: evr_multi_propagate
+    end function
+    ! nb0 coupled electronic states.  blocks is row-major: blocks((i-1)*nb0 + j)
+    ! holds the plan of H_ij (C_NULL_PTR for a zero block); from a Fortran
+    ! matrix B(i, j) of plans pass reshape(transpose(B), [nb0*nb0])
+    function evr_coupled_create(blocks, nb0, c) bind(C, name="evr_coupled_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: blocks(*)
+      integer(c_int), value :: nb0
+      type(c_ptr), intent(out) :: c
+      integer(c_int) :: evr_coupled_create
+    end function
+    function evr_coupled_destroy(c) bind(C, name="evr_coupled_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int) :: evr_coupled_destroy
+    end function
+    function evr_coupled_sizes(c, nb, nb0, max_nvec) bind(C, name="evr_coupled_sizes")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: c
+      integer(c_int64_t), intent(out) :: nb
+      integer(c_int), intent(out) :: nb0, max_nvec
+      integer(c_int) :: evr_coupled_sizes
+    end function
+    function evr_coupled_stream(c) bind(C, name="evr_coupled_stream")
+      import :: c_ptr
+      type(c_ptr), value :: c
+      type(c_ptr) :: evr_coupled_stream
+    end function
+    function evr_coupled_oppsi(c, nvec, psi, hpsi, flags) bind(C, name="evr_coupled_oppsi")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, psi, hpsi
+      integer(c_int), value :: nvec, flags
+      integer(c_int) :: evr_coupled_oppsi
+    end function
+    function evr_coupled_propagate(c, opts, nwp, psi, autocorr, info) bind(C, name="evr_coupled_propagate")
+      import :: c_int, c_ptr, evr_propa_opts, evr_propa_info
+      type(c_ptr), value :: c, psi, autocorr
+      type(evr_propa_opts), intent(in) :: opts
+      integer(c_int), value :: nwp
+      type(evr_propa_info), intent(out) :: info
+      integer(c_int) :: evr_coupled_propagate
     end function
     function evr_plan_bounds(p, steps, Hmin, Hmax) bind(C, name="evr_plan_bounds")
       import :: c_int, c_ptr, c_double

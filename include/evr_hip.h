@@ -128,6 +128,10 @@ const char *evr_last_error(void);
  * returned descriptor owns its arrays; free with evr_basis_desc_free. */
 int evr_sg4_hermite_desc(int D, int LG, int LB, int nbA, int nbB, int nqA, int nqB,
                          evr_basis_desc **out);
+/* The same with a law per mode (the L_TO_nq_A / L_TO_nq_B of each basis_nD
+ * block): nb_d(l) = nbA[d] + nbB[d] l, nq_d(l) = nqA[d] + nqB[d] l. */
+int evr_sg4_hermite_desc_modes(int D, int LG, int LB, const int *nbA, const int *nbB, const int *nqA,
+                               const int *nqB, evr_basis_desc **out);
 /* Hermite direct-product basis, nb[d] functions on nq[d] points per mode,
  * packed order last-dimension-fastest (BG:730-798). */
 int evr_dp_hermite_desc(int D, const int *nb, const int *nq, evr_basis_desc **out);
@@ -192,6 +196,25 @@ void *evr_multi_stream(evr_multi *m);
 /* hpsi = H psi for nvec columns of sum_k nb_k rows (column-major); flags:
  * EVR_DEVICE_PTRS for device pointers on evr_multi_stream. */
 int evr_multi_oppsi(evr_multi *m, int nvec, const double *psi, double *hpsi, int flags);
+
+/* nb0 coupled electronic states over one packed basis (the reference's
+ * nb_be > 1 operators, V(:,ib0,jb0) per grid point, OPSG4:1566-1576): a
+ * wavepacket holds nb0 blocks of nb_packed rows, state-major within each
+ * column (vibrational index fastest), and (H psi)_i = sum_j H_ij psi_j.  The
+ * blocks are plans on the same basis, row-major in blocks[i*nb0 + j] (NULL
+ * for a zero block): a diagonal block carries the kinetic part and V_ii, an
+ * off-diagonal block is a type-0 plan with V_ij.  The plans must outlive the
+ * coupled operator; each block runs through its own fastest path (assembled
+ * when present). */
+typedef struct evr_coupled evr_coupled;
+int evr_coupled_create(evr_plan *const *blocks, int nb0, evr_coupled **out);
+int evr_coupled_destroy(evr_coupled *c);
+/* nb_packed of the basis, nb0, and the largest nvec of one call. */
+int evr_coupled_sizes(const evr_coupled *c, int64_t *nb, int *nb0, int *max_nvec);
+void *evr_coupled_stream(evr_coupled *c);
+/* hpsi = H psi for nvec columns of nb0 x nb_packed rows; flags as evr_oppsi
+ * (EVR_DEVICE_PTRS: device pointers on evr_coupled_stream). */
+int evr_coupled_oppsi(evr_coupled *c, int nvec, const double *psi, double *hpsi, int flags);
 /* Profiling (no counterpart in the reference, whose analogue is the
  * nb_mult_BTOG/GTOB counters, SYS:120): when on, every term-kernel launch is
  * bracketed by HIP events on the plan stream; evr_plan_kernel_stats returns,
@@ -326,6 +349,10 @@ typedef struct evr_propa_info {
  * pairs, Calc_AutoCorr) or NULL. */
 int evr_propagate(evr_plan *H, const evr_propa_opts *opts, int nwp, double *psi,
                   double *autocorr, evr_propa_info *info);
+/* The same on coupled electronic states (evr_coupled_create): psi and psi0
+ * columns hold nb0 x nb_packed rows; no field coupling (field and mu NULL). */
+int evr_coupled_propagate(evr_coupled *H, const evr_propa_opts *opts, int nwp, double *psi,
+                          double *autocorr, evr_propa_info *info);
 
 /* march_cheby on a multi-system operator (evr_multi_create), one complex
  * wavepacket per system (psi: sum_k nb_k x 2, Re and Im columns), every

@@ -12,6 +12,7 @@
  *   SGT2  Source_ElVibRot/sub_Basis/sub_module_param_SGType2.f90
  *   NDI   Source_Lib/sub_nDindex/sub_module_nDindex.f90
  *   LTON  Source_ElVibRot/sub_Basis/sub_module_Basis_LTO_n.f90
+ *   RD    Source_ElVibRot/sub_Basis/sub_read_data.f90
  *   BGSG4 Source_ElVibRot/sub_Basis/sub_Basis_SG4/sub_module_basis_BtoG_GtoB_SG4.f90
  *   BG    Source_ElVibRot/sub_Basis/sub_module_basis_BtoG_GtoB.f90
  *   OPSG4 Source_ElVibRot/sub_Operator/sub_OpPsi_SG4.f90
@@ -248,8 +249,10 @@ struct orc_sg4 {
   int *packed;          /* nb_packed x D */
   int *map;             /* sum_nb */
   orc_1d *bas;          /* (LG+1) x D, index l + (LG+1)*d */
-  int nmax;             /* number of 1D functions in the packed basis per dim */
-  int *lev_of_fn;       /* level of 1D function i (0-based), size nmax */
+  int *nmax;            /* D: number of 1D functions of mode d in the packed basis */
+  int nmaxall;          /* max_d nmax[d] */
+  int *lev_of_fn;       /* D x nmaxall: level of 1D function i (0-based) of mode d
+                           at lev_of_fn[d * nmaxall + i] */
 };
 
 static int law(int A, int B, int l) { return A + B * l; }
@@ -285,14 +288,16 @@ static long long enumerate_terms(int D, int Lmin, int Lmax, int *out)
 }
 
 /* Packed basis: type 5 nDindex (NDI:2396-2461): last index fastest,
- * 1D function indices 0..nmax-1, kept when sum of levels <= LB. */
-static long long enumerate_packed(int D, int nmax, const int *lev, int LB, int *out)
+ * 1D function indices 0..nmax[d]-1 of mode d (level lev[d*stride + i]),
+ * kept when the sum of levels is <= LB. */
+static long long enumerate_packed(int D, const int *nmax, const int *lev, int stride, int LB,
+                                  int *out)
 {
   int *n = (int *)calloc(D, sizeof(int));
   long long cnt = 0;
   for (;;) {
     int s = 0;
-    for (int d = 0; d < D; ++d) s += lev[n[d]];
+    for (int d = 0; d < D; ++d) s += lev[d * stride + n[d]];
     if (s <= LB) {
       if (out) memcpy(out + (size_t)cnt * D, n, sizeof(int) * D);
       ++cnt;
@@ -301,8 +306,8 @@ static long long enumerate_packed(int D, int nmax, const int *lev, int LB, int *
     for (;;) {
       n[d]++;
       int s2 = 0;
-      for (int e = 0; e < D; ++e) s2 += lev[n[e] < nmax ? n[e] : 0];
-      if (n[d] < nmax && s2 <= LB) break;
+      for (int e = 0; e < D; ++e) s2 += lev[e * stride + (n[e] < nmax[e] ? n[e] : 0)];
+      if (n[d] < nmax[d] && s2 <= LB) break;
       n[d] = 0;
       --d;
       if (d < 0) { free(n); return cnt; }
@@ -344,26 +349,45 @@ static void free_1d(orc_1d *b)
 
 orc_sg4 *orc_sg4_create(int D, int LG, int LB, int nbA, int nbB, int nqA, int nqB)
 {
-  if (D < 1 || LG < 0 || LB < 0 || LB > LG) return NULL;
+  if (D < 1 || D > 64) return NULL;
+  int a[4][64];
+  for (int d = 0; d < D; ++d) { a[0][d] = nbA; a[1][d] = nbB; a[2][d] = nqA; a[3][d] = nqB; }
+  return orc_sg4_create_modes(D, LG, LB, a[0], a[1], a[2], a[3]);
+}
+
+orc_sg4 *orc_sg4_create_modes(int D, int LG, int LB, const int *nbA, const int *nbB,
+                              const int *nqA, const int *nqB)
+{
+  if (D < 1 || D > 64 || LG < 0 || LB < 0 || LB > LG) return NULL;
+  for (int d = 0; d < D; ++d)
+    if (nbA[d] < 1 || nbB[d] < 0 || nqA[d] < 1 || nqB[d] < 0 ||
+        nbA[d] > nqA[d] || nbB[d] > nqB[d]) return NULL;
   orc_sg4 *s = (orc_sg4 *)calloc(1, sizeof(orc_sg4));
   s->D = D; s->LG = LG; s->LB = LB;
   s->Lmin = LG - D + 1 > 0 ? LG - D + 1 : 0;          /* SPB:994 */
-  /* primitive bases per level: nq(L), nb(min(L,LB)) (SPB:1059-1064) */
+  /* primitive bases per level: nq(L), nb(min(L,LB)) (SPB:1059-1064), each
+   * mode with its own law (the basis_nD items of the input, RD:511-512) */
   s->bas = (orc_1d *)calloc((size_t)(LG + 1) * D, sizeof(orc_1d));
   for (int d = 0; d < D; ++d)
     for (int l = 0; l <= LG; ++l) {
       int lb = l < LB ? l : LB;
-      build_1d(&s->bas[l + (LG + 1) * d], law(nbA, nbB, lb), law(nqA, nqB, l));
+      build_1d(&s->bas[l + (LG + 1) * d], law(nbA[d], nbB[d], lb), law(nqA[d], nqB[d], l));
     }
   /* level of a 1D function: smallest L with nb(L) >= n, n 1-based
    * (Get_L_FROM_Basis_L_TO_n, LTON:508-564; used at SPB:1162) */
-  s->nmax = law(nbA, nbB, LB);
-  s->lev_of_fn = (int *)malloc(sizeof(int) * s->nmax);
-  for (int i = 0; i < s->nmax; ++i) {
-    int L = 0;
-    while (law(nbA, nbB, L) < i + 1) ++L;
-    s->lev_of_fn[i] = L;
+  s->nmax = (int *)malloc(sizeof(int) * D);
+  s->nmaxall = 0;
+  for (int d = 0; d < D; ++d) {
+    s->nmax[d] = law(nbA[d], nbB[d], LB);
+    if (s->nmax[d] > s->nmaxall) s->nmaxall = s->nmax[d];
   }
+  s->lev_of_fn = (int *)calloc((size_t)D * s->nmaxall, sizeof(int));
+  for (int d = 0; d < D; ++d)
+    for (int i = 0; i < s->nmax[d]; ++i) {
+      int L = 0;
+      while (law(nbA[d], nbB[d], L) < i + 1) ++L;
+      s->lev_of_fn[d * s->nmaxall + i] = L;
+    }
   s->nterms = enumerate_terms(D, s->Lmin, LG, NULL);
   s->levels = (int *)malloc(sizeof(int) * s->nterms * D);
   enumerate_terms(D, s->Lmin, LG, s->levels);
@@ -391,9 +415,9 @@ orc_sg4 *orc_sg4_create(int D, int LG, int LB, int nbA, int nbB, int nqA, int nq
   }
   s->sum_nb = s->off_nb[s->nterms];
   s->sum_nq = s->off_nq[s->nterms];
-  s->nb_packed = enumerate_packed(D, s->nmax, s->lev_of_fn, LB, NULL);
+  s->nb_packed = enumerate_packed(D, s->nmax, s->lev_of_fn, s->nmaxall, LB, NULL);
   s->packed = (int *)malloc(sizeof(int) * s->nb_packed * D);
-  enumerate_packed(D, s->nmax, s->lev_of_fn, LB, s->packed);
+  enumerate_packed(D, s->nmax, s->lev_of_fn, s->nmaxall, LB, s->packed);
 
   /* Map term-local basis index -> packed index (BGSG4:811-892, calc_nDI
    * NDI:2648).  The rank of a tuple in the type-5 order is computed with
@@ -404,8 +428,9 @@ orc_sg4 *orc_sg4_create(int D, int LG, int LB, int nbA, int nbB, int nqA, int nq
   for (int d = D - 1; d >= 0; --d)
     for (int b = 0; b < LBp; ++b) {
       long long c = 0;
-      for (int i = 0; i < s->nmax; ++i)
-        if (s->lev_of_fn[i] <= b) c += cnt[(d + 1) * LBp + b - s->lev_of_fn[i]];
+      const int *lv = s->lev_of_fn + (size_t)d * s->nmaxall;
+      for (int i = 0; i < s->nmax[d]; ++i)
+        if (lv[i] <= b) c += cnt[(d + 1) * LBp + b - lv[i]];
       cnt[d * LBp + b] = c;
     }
   s->map = (int *)malloc(sizeof(int) * s->sum_nb);
@@ -422,9 +447,10 @@ orc_sg4 *orc_sg4_create(int D, int LG, int LB, int nbA, int nbB, int nqA, int nq
       int bud = LB, ok = 1;
       long long rank = 0;
       for (int d = 0; d < D && ok; ++d) {
+        const int *lv = s->lev_of_fn + (size_t)d * s->nmaxall;
         for (int m = 0; m < n[d]; ++m)
-          if (s->lev_of_fn[m] <= bud) rank += cnt[(d + 1) * LBp + bud - s->lev_of_fn[m]];
-        bud -= s->lev_of_fn[n[d]];
+          if (lv[m] <= bud) rank += cnt[(d + 1) * LBp + bud - lv[m]];
+        bud -= lv[n[d]];
         if (bud < 0) ok = 0;
       }
       s->map[s->off_nb[t] + e] = ok ? (int)rank : -1;
@@ -439,7 +465,7 @@ void orc_sg4_destroy(orc_sg4 *s)
   if (!s) return;
   for (int i = 0; i < (s->LG + 1) * s->D; ++i) free_1d(&s->bas[i]);
   free(s->bas); free(s->levels); free(s->weights); free(s->off_nb); free(s->off_nq);
-  free(s->packed); free(s->map); free(s->lev_of_fn); free(s);
+  free(s->packed); free(s->map); free(s->lev_of_fn); free(s->nmax); free(s);
 }
 
 void orc_sg4_sizes(const orc_sg4 *s, long long *z)

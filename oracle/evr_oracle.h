@@ -52,6 +52,13 @@ typedef struct orc_sg4 orc_sg4;
  * (NDI:2396-2461), level of a 1D function from L_TO_nb (SPB:1060-1075).
  * Map: BGSG4:625-949. Weights: SGT2:754-854. */
 orc_sg4 *orc_sg4_create(int D, int LG, int LB, int nbA, int nbB, int nqA, int nqB);
+/* The same with a law per mode: nb_d(l) = nbA[d] + nbB[d]*l, nq_d(l) =
+ * nqA[d] + nqB[d]*l (one basis_nD item per mode, each with its own
+ * L_TO_nb/L_TO_nq, Source_ElVibRot/sub_Basis/sub_read_data.f90:511-512 and
+ * :738-740, nb = nq when L_TO_nb is not given); the level of a 1D function
+ * of mode d follows that mode's law. */
+orc_sg4 *orc_sg4_create_modes(int D, int LG, int LB, const int *nbA, const int *nbB,
+                              const int *nqA, const int *nqB);
 void orc_sg4_destroy(orc_sg4 *s);
 
 /* sizes[0]=nterms, [1]=nb_packed, [2]=sum nb_t, [3]=sum nq_t,

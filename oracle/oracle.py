@@ -26,6 +26,8 @@ def lib():
         L = ctypes.CDLL(path)
         L.orc_sg4_create.restype = ctypes.c_void_p
         L.orc_sg4_create.argtypes = [ctypes.c_int] * 7
+        L.orc_sg4_create_modes.restype = ctypes.c_void_p
+        L.orc_sg4_create_modes.argtypes = [ctypes.c_int] * 3 + [_ip] * 4
         L.orc_sg4_destroy.argtypes = [ctypes.c_void_p]
         L.orc_sg4_sizes.argtypes = [ctypes.c_void_p, _llp]
         L.orc_sg4_terms.argtypes = [ctypes.c_void_p, _ip, _dp, _llp, _llp]
@@ -114,8 +116,14 @@ class SG4:
     """Smolyak SG4 basis of the oracle: nb(l)=nbA+nbB*l, nq(l)=nqA+nqB*l."""
 
     def __init__(self, D, LG, LB=None, nbA=1, nbB=2, nqA=2, nqB=2):
+        """nbA..nqB: integers, or sequences of D integers (a law per mode)."""
         LB = LG if LB is None else LB
-        self.h = lib().orc_sg4_create(D, LG, LB, nbA, nbB, nqA, nqB)
+        if all(np.ndim(v) == 0 for v in (nbA, nbB, nqA, nqB)):
+            self.h = lib().orc_sg4_create(D, LG, LB, nbA, nbB, nqA, nqB)
+        else:
+            arr = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.int32), (D,)))
+                   for v in (nbA, nbB, nqA, nqB)]
+            self.h = lib().orc_sg4_create_modes(D, LG, LB, *[_i(a) for a in arr])
         if not self.h:
             raise ValueError("orc_sg4_create failed")
         z = np.zeros(8, dtype=np.int64)

@@ -3,7 +3,9 @@
 evr_f_host builds a D=4 SG4 Henon-Heiles type-1 operator through evr_hip_f,
 applies it to real and complex vectors, runs Davidson and 5 Chebyshev steps,
 builds the assembled per-term operators and applies them again,
-and writes the results; the oracle recomputes all of them.
+and writes the results; the oracle recomputes all of them.  evr_f_pyrazine
+propagates the two-state 12D pyrazine model and writes the autocorrelation
+file the reference's benchmark holds.
 """
 import os
 import shutil
@@ -66,3 +68,27 @@ def test_fortran_host(oracle_mod, tmp_path):
     off += 2 * nb
     Hasm = x[off:off + 4 * nb].reshape(4, nb).T
     assert np.abs(Hasm - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+PYRAZINE = os.path.join(ROOT, "elvibrot-tnumtana_amd", "fortran", "evr_f_pyrazine")
+
+
+def test_fortran_pyrazine_benchmark(tmp_path):
+    """evr_f_pyrazine (two coupled states through evr_coupled_* from Fortran)
+    writes the reference's 12D pyrazine autocorrelation file; it must agree
+    with the reference's benchmark to 1e-9."""
+    import pyrazine as P
+    if not os.path.exists(PYRAZINE):
+        if shutil.which("flang") is None and not os.path.exists("/opt/rocm/lib/llvm/bin/flang"):
+            pytest.skip("no Fortran compiler: the Fortran host is not built")
+        raise RuntimeError("evr_f_pyrazine is not built (run __graft_entry__.build())")
+    out = tmp_path / "pyrazine.txt"
+    r = subprocess.run([PYRAZINE, str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    got = np.loadtxt(out)
+    t, ref = P.autocorr_ref()
+    assert got.shape == (P.NSTEPS + 1, 4)
+    assert np.abs(got[:, 0] - t).max() < 1e-12
+    assert np.abs(got[:, 1] + 1j * got[:, 2] - ref).max() < 1e-9
+    assert np.abs(got[:, 3] - np.abs(ref)).max() < 1e-9
