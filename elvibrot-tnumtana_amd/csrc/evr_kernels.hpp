@@ -36,6 +36,12 @@ struct TermDev {
   // Assembled operator: start of the term's tiled nb_t x nb_t matrix (MFMA
   // A-fragment order, see asm_index) in the assembled pools.
   int64_t off_asm;
+  // Coefficient set of the term (index into the per-set KEO coefficients:
+  // keoc + cix * keoc_stride, coefC + cix * D * D, coefF1 + cix * D).  All
+  // terms share set 0 unless the plan compacts modes (more modes than the
+  // kernel's positions): then each set holds the coefficients of one
+  // position -> mode assignment.
+  int cix;
 };
 
 // Position of element (row, col) of a term's assembled matrix: fragments of
@@ -93,7 +99,8 @@ struct TermParams {
   const double *coefC;        // D x D: c[i*D+j]
   const double *coefA;        // D^3: a[(i*D+j)*D+k]
   const double *coefF1;       // D
-  const double *keoc;         // pre-zeroed coefficient strips (see plan)
+  const double *keoc;         // pre-zeroed coefficient strips (see plan), one block per set
+  int64_t keoc_stride;        // doubles per coefficient set in keoc
   const T *X;                 // packed input, [nb_packed][nvec]
   T *R;                       // term results in scatter order, [rows][nvec]
   T *G;                       // grid output (B2G), column v at G + v*gld

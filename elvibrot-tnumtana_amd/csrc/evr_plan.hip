@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -86,7 +87,9 @@ struct evr_desc_owner {
 };
 
 struct evr_plan {
-  int D = 0, type = 0, max_nvec = 0;
+  int D = 0, type = 0, max_nvec = 0;  // D: the term kernel's mode positions
+  int D_full = 0;                     // modes of the basis (> D when compacted)
+  int64_t keoc_stride = 0;            // doubles per KEO coefficient set
   bool dp_order = false;
   int64_t nterms = 0, nb_packed = 0, sum_nb = 0, sum_nq = 0, max_nb = 0, max_nq = 0;
   int64_t sum_vq = 0;  // reduced grid points (V_eff) of all terms
@@ -212,6 +215,7 @@ evr::TermParams<T> base_params(evr_plan *p, int mode, int nvec)
   prm.coefA = p->d_coefA.ptr;
   prm.coefF1 = p->d_coefF1.ptr;
   prm.keoc = p->d_keoc.ptr;
+  prm.keoc_stride = p->keoc_stride;
   prm.gld = p->sum_nq;
   return prm;
 }
@@ -626,8 +630,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
                     evr_plan **out)
 {
   if (!bd || !op || !out) return fail(EVR_ERR_ARG, "NULL argument");
-  if (bd->D < 1 || bd->D > evr::kMaxDev)
-    return fail(EVR_ERR_ARG, "D must be in 1..12 for the term kernel");
+  if (bd->D < 1 || bd->D > evr::kMaxD) return fail(EVR_ERR_ARG, "D must be in 1..64");
   if (op->type != 0 && op->type != 1 && op->type != 10)
     return fail(EVR_ERR_ARG, "operator type must be 0, 1 or 10");
   if ((op->type == 10 && !op->G0) || (op->type == 1 && !op->F2))
@@ -650,6 +653,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   std::unique_ptr<evr_plan> p(new evr_plan());
   const int D = bd->D, nlev = bd->nlev;
   p->D = D;
+  p->D_full = D;
   p->type = op->type;
   p->max_nvec = max_nvec;
   p->dp_order = bd->grid_last_fastest != 0;
@@ -731,6 +735,118 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     }
   }
 
+  // ---- kernel mode positions of every term.  Up to kMaxDev modes the
+  // positions are the modes.  Beyond (SG4 bases of many modes at a low level,
+  // where most modes of a term have one function on one grid point), a term
+  // keeps its live modes in mode order, merges its other modes into one
+  // composite slot and pads with a unit slot.  A merged mode contributes
+  // scalars only: its B and W (1 x 1) multiply into the composite's, its
+  // P, Q, R vanish (no cross terms), and its diagonal KEO c_dd Td_d sums into
+  // the composite's Td with coefficient 1; the term grid and basis orders are
+  // unchanged (a merged mode has extent 1).  The KEO coefficients then come
+  // per position -> mode assignment (coefficient sets).
+  const int K = evr::kMaxDev;
+  const bool compact = D > K;
+  int Dk = D;
+  std::vector<int> tslot((size_t)p->nterms * K, 0), tmode((size_t)p->nterms * K, -2);
+  if (!compact) {
+    for (int64_t t = 0; t < p->nterms; ++t)
+      for (int d = 0; d < D; ++d) {
+        const int lv = bd->term_levels[t * D + d];
+        if (lv < 0 || lv >= nlev) return fail(EVR_ERR_ARG, "term level out of range");
+        tslot[t * K + d] = d * nlev + lv;
+        tmode[t * K + d] = d;
+      }
+  } else {
+    if (p->dp_order) return fail(EVR_ERR_ARG, "a direct-product basis needs D <= 12");
+    if (p->hasA) return fail(EVR_ERR_ARG, "G(q) = G0 + Ga q needs D <= 12 (G0 alone is fine)");
+    const int Td = p->diagT2 ? evr::kT2 : evr::kT;
+    auto merged = [&](int s) { return bd->nb[s] == 1 && bd->nq[s] == 1 && slot_fold[s]; };
+    Dk = 1;
+    for (int64_t t = 0; t < p->nterms; ++t) {
+      int live = 0, dead = 0;
+      for (int d = 0; d < D; ++d) {
+        const int lv = bd->term_levels[t * D + d];
+        if (lv < 0 || lv >= nlev) return fail(EVR_ERR_ARG, "term level out of range");
+        (merged(d * nlev + lv) ? dead : live)++;
+      }
+      Dk = std::max(Dk, live + (dead > 0));
+    }
+    if (Dk > K)
+      return fail(EVR_ERR_SIZE, "with more than 12 modes a term may have at most 11 modes with "
+                                "more than one function or grid point");
+    // a slot of one function on one point: B, W, the W o B^T of the plan
+    // diagonal, and the diagonal KEO factor Td (the other factors zero)
+    auto add_slot = [&](double B, double W, double fTd) -> int {
+      evr::SlotDev sd{};
+      sd.nb = sd.nq = 1;
+      sd.offB = (int64_t)bw.size();
+      bw.push_back(B);
+      bwdiag.push_back(B);
+      sd.offW = (int64_t)bw.size();
+      bw.push_back(W);
+      bwdiag.push_back(W * B);
+      sd.offX = (int64_t)xs.size();
+      xs.push_back(0.0);
+      sd.offK = (int64_t)keo.size();
+      keo.insert(keo.end(), evr::kNumFactors, 0.0);
+      keo[sd.offK + Td] = fTd;
+      p->slots.push_back(sd);
+      slot_fold.push_back(1);
+      return (int)p->slots.size() - 1;
+    };
+    const int unit = add_slot(1.0, 1.0, 0.0);
+    std::map<std::vector<int>, int> composite;  // merged slots -> composite slot
+    for (int64_t t = 0; t < p->nterms; ++t) {
+      std::vector<int> dead;
+      int k = 0;
+      for (int d = 0; d < D; ++d) {
+        const int s = d * nlev + bd->term_levels[t * D + d];
+        if (merged(s)) {
+          dead.push_back(s);
+        } else {
+          tslot[t * K + k] = s;
+          tmode[t * K + k++] = d;
+        }
+      }
+      if (!dead.empty()) {
+        auto it = composite.find(dead);
+        int cs;
+        if (it != composite.end()) {
+          cs = it->second;
+        } else {
+          double B = 1.0, W = 1.0, f = 0.0;
+          for (int s : dead) {
+            const evr::SlotDev &sd = p->slots[s];
+            const int d = s / nlev;
+            B *= bw[sd.offB];
+            W *= bw[sd.offW];
+            f += cC[(size_t)d * D + d] * keo[sd.offK + Td];
+          }
+          cs = add_slot(B, W, f);
+          composite.emplace(dead, cs);
+        }
+        tslot[t * K + k] = cs;
+        tmode[t * K + k++] = -1;
+      }
+      for (; k < Dk; ++k) tslot[t * K + k] = unit;  // tmode stays -2
+    }
+    p->D = Dk;
+  }
+  // coefficient sets: one per distinct position -> mode assignment
+  std::map<std::vector<int>, int> set_of;
+  std::vector<std::vector<int>> sets;
+  std::vector<int> cix(p->nterms, 0);
+  for (int64_t t = 0; t < p->nterms; ++t) {
+    std::vector<int> key(tmode.begin() + t * K, tmode.begin() + t * K + Dk);
+    auto it = set_of.find(key);
+    if (it == set_of.end()) {
+      it = set_of.emplace(key, (int)sets.size()).first;
+      sets.push_back(key);
+    }
+    cix[t] = it->second;
+  }
+
   // ---- terms
   p->terms.resize(p->nterms);
   int64_t onb = 0, onq = 0, onvq = 0;
@@ -741,13 +857,13 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     int nbd[evr::kMaxDev], nqd[evr::kMaxDev];
     int64_t nbt = 1, nqt = 1;
     int mats = 0;
+    const int D = Dk;  // the kernel's positions from here on
+    td.cix = cix[t];
     for (int d = 0; d < D; ++d) {
-      const int lv = bd->term_levels[t * D + d];
-      if (lv < 0 || lv >= nlev) return fail(EVR_ERR_ARG, "term level out of range");
-      const int s = d * nlev + lv;
+      const int s = tslot[t * K + d];
       td.slot[d] = s;
-      nbd[d] = bd->nb[s];
-      nqd[d] = bd->nq[s];
+      nbd[d] = p->slots[s].nb;
+      nqd[d] = p->slots[s].nq;
       nbt *= nbd[d];
       nqt *= nqd[d];
       mats += 2 * nbd[d] * nqd[d] + (p->hasKeo ? evr::kKeoFactors * nbd[d] * nbd[d] : 0);
@@ -929,33 +1045,52 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     HIP_TRY(p->d_P.alloc((size_t)std::max<int64_t>(p->nseg, 1) * max_nvec));
   }
   HIP_TRY(p->d_rowpos.upload(rowpos.data(), rowpos.size(), p->stream));
-  HIP_TRY(p->d_coefC.upload(cC.data(), cC.size(), p->stream));
-  if (!cA.empty()) HIP_TRY(p->d_coefA.upload(cA.data(), cA.size(), p->stream));
-  if (!cF1.empty()) HIP_TRY(p->d_coefF1.upload(cF1.data(), cF1.size(), p->stream));
   {
-    // pre-zeroed strips of the basis-space KEO coefficients (kernel layout):
-    //  A[(j*Dp+k)*Dp+i] = a_ijk for i,j,k distinct; A[(j*Dp+j)*Dp+i] = c_ij, i != j
-    //  S[j*Dp+i] = a_ijj (i != j); N[j*Dp+i] = a_iji (j != i); O[k*Dp+i] = a_iik (k != i)
-    //  Cd[i] = c_ii; U[i] = a_iii; F[i] = F1_i
+    // per coefficient set (positions a, b with modes m[a], m[b]): c_ab from
+    // the operator, 1 on the diagonal of a composite position, 0 for padding;
+    // Ga only without compaction (positions = modes).  Kernel layout of the
+    // pre-zeroed strips of one set:
+    //  S[j*Dp+i] = a_ijj, N[j*Dp+i] = a_iji, O[k*Dp+i] = a_iik, C[j*Dp+i] = c_ij
+    //  (i != j); Cd[i] = c_ii; U[i] = a_iii; F[i] = F1_i;
+    //  A[(j*Dp+k)*Dp+i] = a_ijk for i, j, k distinct (with Ga only)
     const int Dp = evr::kDp;
-    std::vector<double> kc((size_t)Dp * Dp * Dp + 3 * Dp * Dp + 3 * Dp, 0.0);
-    double *kA = kc.data(), *kS = kA + Dp * Dp * Dp, *kN = kS + Dp * Dp, *kO = kN + Dp * Dp,
-           *kCd = kO + Dp * Dp, *kU = kCd + Dp, *kF = kU + Dp;
-    auto a = [&](int i, int j, int k) { return cA.empty() ? 0.0 : cA[(i * D + j) * D + k]; };
-    for (int i = 0; i < D; ++i) {
-      kCd[i] = cC[i * D + i];
-      kU[i] = a(i, i, i);
-      kF[i] = cF1.empty() ? 0.0 : cF1[i];
-      for (int j = 0; j < D; ++j) {
-        if (j == i) continue;
-        kA[(j * Dp + j) * Dp + i] = cC[i * D + j];
-        kS[j * Dp + i] = a(i, j, j);
-        kN[j * Dp + i] = a(i, j, i);
-        kO[j * Dp + i] = a(i, i, j);
-        for (int k = 0; k < D; ++k)
-          if (k != i && k != j) kA[(j * Dp + k) * Dp + i] = a(i, j, k);
+    const size_t nset = sets.size();
+    auto cpos = [&](const std::vector<int> &m, int a, int b) -> double {
+      if (m[a] >= 0 && m[b] >= 0) return cC[(size_t)m[a] * D + m[b]];
+      return (a == b && m[a] == -1) ? 1.0 : 0.0;
+    };
+    auto apos = [&](int i, int j, int k) { return cA.empty() ? 0.0 : cA[((size_t)i * D + j) * D + k]; };
+    auto fpos = [&](const std::vector<int> &m, int a) {
+      return (!cF1.empty() && m[a] >= 0) ? cF1[m[a]] : 0.0;
+    };
+    p->keoc_stride = 4 * Dp * Dp + 3 * Dp + (cA.empty() ? 0 : Dp * Dp * Dp);
+    std::vector<double> kc(nset * p->keoc_stride, 0.0), pc(nset * Dk * Dk),
+        pf(cF1.empty() ? 0 : nset * Dk);
+    for (size_t c = 0; c < nset; ++c) {
+      const std::vector<int> &m = sets[c];
+      double *kS = kc.data() + c * p->keoc_stride, *kN = kS + Dp * Dp, *kO = kN + Dp * Dp,
+             *kC = kO + Dp * Dp, *kCd = kC + Dp * Dp, *kU = kCd + Dp, *kF = kU + Dp, *kA = kF + Dp;
+      for (int i = 0; i < Dk; ++i) {
+        for (int j = 0; j < Dk; ++j) pc[(c * Dk + i) * Dk + j] = cpos(m, i, j);
+        if (!cF1.empty()) pf[c * Dk + i] = fpos(m, i);
+        kCd[i] = cpos(m, i, i);
+        kU[i] = apos(i, i, i);
+        kF[i] = fpos(m, i);
+        for (int j = 0; j < Dk; ++j) {
+          if (j == i) continue;
+          kC[j * Dp + i] = cpos(m, i, j);
+          kS[j * Dp + i] = apos(i, j, j);
+          kN[j * Dp + i] = apos(i, j, i);
+          kO[j * Dp + i] = apos(i, i, j);
+          if (!cA.empty())
+            for (int k = 0; k < Dk; ++k)
+              if (k != i && k != j) kA[(j * Dp + k) * Dp + i] = apos(i, j, k);
+        }
       }
     }
+    HIP_TRY(p->d_coefC.upload(pc.data(), pc.size(), p->stream));
+    if (!cA.empty()) HIP_TRY(p->d_coefA.upload(cA.data(), cA.size(), p->stream));
+    if (!pf.empty()) HIP_TRY(p->d_coefF1.upload(pf.data(), pf.size(), p->stream));
     HIP_TRY(p->d_keoc.upload(kc.data(), kc.size(), p->stream));
   }
   HIP_TRY(p->d_V.alloc(onq));
@@ -1012,6 +1147,8 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
                                 const double *d22, const double *e4)
 {
   if (!p) return fail(EVR_ERR_ARG, "NULL plan");
+  if (p->D != p->D_full)
+    return fail(EVR_ERR_ARG, "the polynomial generator needs D <= 12 (set V on the grid instead)");
   const int D = p->D;
   std::vector<double> coef((size_t)D + D * D * D + D * D + D, 0.0);
   if (v2) std::copy(v2, v2 + D, coef.begin());
@@ -1041,7 +1178,7 @@ int evr_plan_sizes(const evr_plan *p, int64_t *out)
 {
   if (!p || !out) return fail(EVR_ERR_ARG, "NULL argument");
   out[0] = p->nb_packed; out[1] = p->nterms; out[2] = p->sum_nb; out[3] = p->sum_nq;
-  out[4] = p->max_nb; out[5] = p->max_nq; out[6] = p->D;
+  out[4] = p->max_nb; out[5] = p->max_nq; out[6] = p->D_full;
   return EVR_OK;
 }
 

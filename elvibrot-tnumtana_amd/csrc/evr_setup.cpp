@@ -227,7 +227,7 @@ static void append_slot(BasisTables &t, const Slot1D &s)
 std::string build_sg4_hermite(int D, int LG, int LB, int nbA, int nbB, int nqA, int nqB,
                               BasisTables &t)
 {
-  if (D < 1 || D > kMaxD) return "D out of range (1..16)";
+  if (D < 1 || D > kMaxD) return "D out of range (1..64)";
   const std::vector<int> a(D, nbA), b(D, nbB), qa(D, nqA), qb(D, nqB);
   return build_sg4_hermite_modes(D, LG, LB, a.data(), b.data(), qa.data(), qb.data(), t);
 }
@@ -235,7 +235,7 @@ std::string build_sg4_hermite(int D, int LG, int LB, int nbA, int nbB, int nqA, 
 std::string build_sg4_hermite_modes(int D, int LG, int LB, const int *nbA, const int *nbB,
                                     const int *nqA, const int *nqB, BasisTables &t)
 {
-  if (D < 1 || D > kMaxD) return "D out of range (1..16)";
+  if (D < 1 || D > kMaxD) return "D out of range (1..64)";
   if (LG < 0 || LB < 0 || LB > LG) return "need 0 <= LB <= LG";
   for (int d = 0; d < D; ++d)
     if (nbA[d] < 1 || nqA[d] < 1 || nbB[d] < 0 || nqB[d] < 0) return "bad L_TO_n law";
@@ -348,7 +348,7 @@ std::string build_sg4_hermite_modes(int D, int LG, int LB, const int *nbA, const
 
 std::string build_dp_hermite(int D, const int *nb, const int *nq, BasisTables &t)
 {
-  if (D < 1 || D > kMaxD) return "D out of range (1..16)";
+  if (D < 1 || D > kMaxD) return "D out of range (1..64)";
   t = BasisTables();
   t.D = D;
   t.nlev = 1;

@@ -8,7 +8,7 @@
 
 namespace evr {
 
-constexpr int kMaxD = 16;
+constexpr int kMaxD = 64;
 
 // One 1D primitive basis (a "slot": one mode at one Smolyak level).
 // Matrices are column-major.

@@ -271,13 +271,15 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       T *zb = work;
       T *zb2 = work + nunit;
       T *zb3 = work + 2 * nunit;
-      const double *kA = p.keoc;                    // [(j*Dp + k)*Dp + i]
-      const double *kS = kA + kDp * kDp * kDp;      // [j*Dp + i]
+      // this term's coefficient set (strips in its mode positions)
+      const double *kS = p.keoc + (int64_t)td.cix * p.keoc_stride;  // [j*Dp + i]
       const double *kN = kS + kDp * kDp;            // [j*Dp + i]
       const double *kO = kN + kDp * kDp;            // [k*Dp + i]
-      const double *kCd = kO + kDp * kDp;           // [i]
+      const double *kC = kO + kDp * kDp;            // [j*Dp + i]
+      const double *kCd = kC + kDp * kDp;           // [i]
       const double *kU = kCd + kDp;                 // [i]
       const double *kF = kU + kDp;                  // [i]
+      const double *kA = kF + kDp;                  // [(j*Dp + k)*Dp + i], with Ga only
       int base[EPT][kMaxDev];
 #pragma unroll
       for (int r = 0; r < EPT; ++r) {
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
             const T fq = s_fq[j];
 #pragma unroll
             for (int ii = 0; ii < IG; ++ii) {
-              cz[ii] = T(kA[(j * kDp + j) * kDp + icol[ii]]);
+              cz[ii] = T(kC[j * kDp + icol[ii]]);
               cs[ii] = T(kS[j * kDp + icol[ii]]);
               cn[ii] = T(kN[j * kDp + icol[ii]]);
               co[ii] = T(kO[j * kDp + icol[ii]]);
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   if (hmode && p.hasKeo) {
     constexpr int IG = IGroup<EPT>::value;
     T *zb = work;
-    const double *cC = p.coefC;
+    const double *cC = p.coefC + (int64_t)td.cix * D * D;  // this term's coefficient set
     const double *cA = p.coefA;
     for (int i0 = 0; i0 < D; i0 += IG) {
       T accM[IG][EPT], accN[IG][EPT], accO[IG][EPT];
@@ -580,7 +582,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         if (u < nunit)
           for (int i = 0; i < D; ++i) {
             const int nbi = s_nb[i];
-            yown[r] += T(p.coefF1[i]) *
+            yown[r] += T(p.coefF1[(int64_t)td.cix * D + i]) *
                        apply_mode(smem + s_offK[i], sx, u, (int)((dig[r] >> (5 * i)) & 31), nbi,
                                   s_st[i]);
           }
@@ -728,7 +730,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   if (p.mode == kModeG2B && p.diagKeo && p.hasKeo) {
     // diagonal of the kinetic part: sum_i c_ii Tdiag_i[b_i,b_i] + a_iii U_i[b_i,b_i]
     // (the other factor products vanish on the diagonal: P and Q have a zero diagonal)
-    const double *kCd = p.keoc + kDp * kDp * kDp + 3 * kDp * kDp, *kU = kCd + kDp;
+    const double *kCd = p.keoc + (int64_t)td.cix * p.keoc_stride + 4 * kDp * kDp, *kU = kCd + kDp;
     const int fdiag = p.diagT2 ? 6 : 2;
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {

@@ -144,7 +144,12 @@ int evr_basis_desc_grid(const evr_basis_desc *d, double *q);
 /* ---- plans -------------------------------------------------------------- */
 /* Replaces the per-call setup of sub_TabOpPsi_FOR_SGtype4 (OPSG4:678-979):
  * uploads tables, 1D matrices, the basis-space KEO factors and V.
- * max_nvec bounds the vectors per evr_oppsi call. */
+ * max_nvec bounds the vectors per evr_oppsi call.  D <= 12: every mode is a
+ * position of the term kernel.  12 < D <= 64 (SG4 at low levels, e.g. the
+ * reference's 21D and 24D working tests): the modes of a term with one
+ * function on one grid point merge into one scalar position, so each term
+ * may have at most 11 other modes; not for a DP basis, a Ga, or
+ * evr_plan_gen_potential_poly. */
 int evr_plan_create(const evr_basis_desc *basis, const evr_op_desc *op, int max_nvec,
                     evr_plan **out);
 int evr_plan_destroy(evr_plan *p);

@@ -1,13 +1,14 @@
-"""Coupled electronic states (nb0 = 2) through the C ABI, on the 12D pyrazine
-model of the reference's propagation test: H psi against the oracle's
-coupled operator, and the Chebyshev autocorrelation against the reference's
-benchmark (Working_tests/MPI_tests/12D_propagation_S0/benchmark)."""
+"""Coupled electronic states (nb0 = 2) through the C ABI, on the 12D and 24D
+pyrazine models of the reference's propagation tests: H psi against the
+oracle's coupled operator, and the Chebyshev autocorrelation against the
+reference's benchmarks
+(Working_tests/MPI_tests/{12D,24D}_propagation_S0/benchmark)."""
 import numpy as np
 import pytest
 import scipy.linalg as sl
 
-import pyrazine as P
-from test_oracle_pyrazine import coupled_dense, psi0
+from pyrazine import P12, P24
+from test_oracle_pyrazine import MODELS, coupled_dense, psi0
 
 pytestmark = pytest.mark.gpu
 
@@ -16,8 +17,9 @@ def relerr(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-def _coupled(evr, L, max_nvec=4, assemble=False):
-    b = evr.Basis.sg4_modes(P.D, L, P.NQA, P.NQB, P.NQA, P.NQB)
+def _coupled(evr, P, L=None, max_nvec=4, assemble=False):
+    D, LG, LB, nbA, nbB, nqA, nqB = P.sg4_args(L)
+    b = evr.Basis.sg4_modes(D, LG, nbA, nbB, nqA, nqB, LB=LB)
     V11, V22, V12 = P.potentials(b.grid())
     p11 = evr.Plan(b, type=1, V=V11, F2=P.F2(), max_nvec=max_nvec)
     p22 = evr.Plan(b, type=1, V=V22, F2=P.F2(), max_nvec=max_nvec)
@@ -28,48 +30,51 @@ def _coupled(evr, L, max_nvec=4, assemble=False):
     return b, evr.Coupled([[p11, w], [w, p22]])
 
 
-@pytest.mark.parametrize("L", [1, 2])
-def test_coupled_oppsi_vs_oracle(evr, oracle_mod, L):
-    b, H = _coupled(evr, L)
-    s, Hd = coupled_dense(oracle_mod, L)
+@pytest.mark.parametrize("name,L", [("12d", 1), ("12d", 2), ("24d", 2)])
+def test_coupled_oppsi_vs_oracle(evr, oracle_mod, name, L):
+    P = MODELS[name]
+    b, H = _coupled(evr, P, L)
+    s, Hd = coupled_dense(oracle_mod, P, L)
     assert H.nb_packed == s.nb_packed and H.nb0 == 2
     X = evr.hamiltonian.test_vectors(2 * s.nb_packed, 6, 5)   # 6 columns: two calls of 4 + 2
     got = H.TabOpPsi(X)
     err = relerr(got, Hd @ X)
-    print("pyrazine L=%d coupled H psi rel err %.3e" % (L, err))
+    print("pyrazine %s L=%d coupled H psi rel err %.3e" % (name, L, err))
     assert err < 1e-12
     H.close()
 
 
 def test_coupled_assembled_matches_matrix_free(evr):
-    _, Hm = _coupled(evr, 2)
-    _, Ha = _coupled(evr, 2, assemble=True)
+    _, Hm = _coupled(evr, P24)
+    _, Ha = _coupled(evr, P24, assemble=True)
     X = evr.hamiltonian.test_vectors(2 * Hm.nb_packed, 4, 9)
     assert relerr(Ha.TabOpPsi(X), Hm.TabOpPsi(X)) < 1e-12
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_pyrazine_benchmark_autocorr(evr, oracle_mod, graph):
+@pytest.mark.parametrize("name,graph", [("12d", False), ("12d", True), ("24d", False), ("24d", True)])
+def test_pyrazine_benchmark_autocorr(evr, oracle_mod, name, graph):
     """march_cheby on the GPU at the reference's npoly = 300, poly_tol = 1e-10,
     60 steps of 0.1 fs: <psi0|psi(t)> within 1e-9 of the reference's output."""
-    b, H = _coupled(evr, P.L)
-    s, _ = coupled_dense(oracle_mod)
+    P = MODELS[name]
+    b, H = _coupled(evr, P)
+    s, _ = coupled_dense(oracle_mod, P)
     p0 = psi0(s)
     t, ref = P.autocorr_ref()
     psi, ac, info = H.propagate(p0, P.DT, P.NSTEPS, npoly=P.NPOLY, poly_tol=P.POLY_TOL, autocorr=True,
                                 graph=graph)
     got = np.r_[1.0, ac[:, 0]]
     err = np.abs(got - ref).max()
-    print("pyrazine GPU (graph=%s) vs benchmark %.3e, norm %.15f" % (graph, err, np.linalg.norm(psi)))
+    print("pyrazine %s GPU (graph=%s) vs benchmark %.3e, norm %.15f, max order %d"
+          % (name, graph, err, np.linalg.norm(psi), info.max_order))
     assert err < 1e-9
     assert abs(np.linalg.norm(psi) - 1.0) < 1e-9
 
 
 def test_pyrazine_L2_vs_exact(evr, oracle_mod):
-    """A larger basis (362 functions per state): 20 steps against exp(-i H t)
-    of the oracle's coupled operator, two wavepackets at once."""
-    b, H = _coupled(evr, 2, max_nvec=4)
-    s, Hd = coupled_dense(oracle_mod, 2)
+    """A larger 12D basis (362 functions per state): 20 steps against
+    exp(-i H t) of the oracle's coupled operator, two wavepackets at once."""
+    b, H = _coupled(evr, P12, 2, max_nvec=4)
+    s, Hd = coupled_dense(oracle_mod, P12, 2)
     n = s.nb_packed
     rng = np.random.default_rng(3)
     p = np.zeros((2 * n, 2), complex)
@@ -77,8 +82,8 @@ def test_pyrazine_L2_vs_exact(evr, oracle_mod):
     p[:, 1] = rng.standard_normal(2 * n) + 1j * rng.standard_normal(2 * n)
     p[:, 1] /= np.linalg.norm(p[:, 1])
     nsteps = 20
-    got, _, _ = H.propagate(p, P.DT, nsteps, npoly=P.NPOLY, poly_tol=1e-12)
-    ref = sl.expm(-1j * Hd * P.DT * nsteps) @ p
+    got, _, _ = H.propagate(p, P12.DT, nsteps, npoly=P12.NPOLY, poly_tol=1e-12)
+    ref = sl.expm(-1j * Hd * P12.DT * nsteps) @ p
     err = np.abs(got - ref).max()
     print("pyrazine L=2 GPU vs exact %.3e" % err)
     assert err < 1e-9

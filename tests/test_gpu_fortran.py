@@ -77,7 +77,7 @@ def test_fortran_pyrazine_benchmark(tmp_path):
     """evr_f_pyrazine (two coupled states through evr_coupled_* from Fortran)
     writes the reference's 12D pyrazine autocorrelation file; it must agree
     with the reference's benchmark to 1e-9."""
-    import pyrazine as P
+    from pyrazine import P12 as P
     if not os.path.exists(PYRAZINE):
         if shutil.which("flang") is None and not os.path.exists("/opt/rocm/lib/llvm/bin/flang"):
             pytest.skip("no Fortran compiler: the Fortran host is not built")
