@@ -8,6 +8,7 @@ sub_PsiGridRep_TO_BasisRep (Source_ElVibRot/sub_WP/sub_module_psi_B_TO_G.f90:60,
 library is missing, loading raises.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -117,6 +118,28 @@ class Basis:
 
     def map(self):
         return np.ctypeslib.as_array(self.ptr.contents.map, shape=(self.sum_nb,)).copy()
+
+    def save_psi(self, path, psi, nb0=1, version=0, formatted=True):
+        """Write wavepackets (nb0*nb_packed,) or (nb0*nb_packed, nwp), real or
+        complex, in the reference's save-file format (sub_save_psi)."""
+        psi = np.asarray(psi)
+        cplx = np.iscomplexobj(psi)
+        if psi.ndim == 1:
+            psi = psi[:, None]
+        arr = _to_pairs(psi) if cplx else np.asfortranarray(psi, dtype=np.float64)
+        check(load().evr_psi_save(self.ptr, nb0, psi.shape[1], 1 if cplx else 0, _d(arr), version,
+                                  1 if formatted else 0, os.fsencode(path)))
+
+    def read_psi(self, path, nb0=1, cplx=True, max_wp=1):
+        """Wavepackets of a save file projected onto this basis:
+        (nb0*nb_packed, nwp) complex (or real) array."""
+        n = self.nb_packed * nb0
+        arr = np.zeros((n, (2 if cplx else 1) * max_wp), order="F")
+        got = ctypes.c_int()
+        check(load().evr_psi_read(self.ptr, nb0, 1 if cplx else 0, max_wp, _d(arr), ctypes.byref(got),
+                                  os.fsencode(path)))
+        out = _from_pairs(arr) if cplx else arr
+        return out[:, :got.value]
 
     def __del__(self):
         try:

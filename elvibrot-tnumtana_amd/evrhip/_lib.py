@@ -72,6 +72,10 @@ _SIGS = {
     "evr_basis_desc_free": (None, [ctypes.POINTER(BasisDesc)]),
     "evr_basis_desc_sizes": (ctypes.c_int, [ctypes.POINTER(BasisDesc), _i64p]),
     "evr_basis_desc_grid": (ctypes.c_int, [ctypes.POINTER(BasisDesc), _dp]),
+    "evr_psi_save": (ctypes.c_int, [ctypes.POINTER(BasisDesc), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
+    "evr_psi_read": (ctypes.c_int, [ctypes.POINTER(BasisDesc), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                    _ip, ctypes.c_char_p]),
     "evr_plan_create": (ctypes.c_int, [ctypes.POINTER(BasisDesc), ctypes.POINTER(OpDesc), ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_void_p)]),
     "evr_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),

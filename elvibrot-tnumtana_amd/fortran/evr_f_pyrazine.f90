@@ -10,6 +10,7 @@
 ! by march_cheby (npoly 300, poly_tol 1e-10) for 60 steps of 0.1 fs; the
 ! autocorrelation <psi0|psi(t)> is written to the file named by the first
 ! argument as lines "t Re Im |.|", as the reference's benchmark file.
+! The final wavepacket goes to <file>.psi in the reference's save-file format.
 !===============================================================================
 program evr_f_pyrazine
   use, intrinsic :: iso_c_binding
@@ -92,6 +93,9 @@ program evr_f_pyrazine
     write(u, '(4es25.17)') n * dt, ac(1, n), ac(2, n), sqrt(ac(1, n)**2 + ac(2, n)**2)
   end do
   close(u)
+  ! the final wavepacket in the reference's save-file format (version 0)
+  call evr_check(evr_psi_save(desc, 2, 1, 1, c_loc(psi), 0, 1, trim(out)//'.psi'//c_null_char), &
+                 'evr_psi_save')
   write(*, '(a,i0,a,i0,a,i0,a)') 'evr_f_pyrazine: ', nb, ' functions x 2 states, ', info%hpsi, &
       ' H psi, max Chebyshev order ', info%max_order, ', done'
   call evr_check(evr_coupled_destroy(hc), 'evr_coupled_destroy')

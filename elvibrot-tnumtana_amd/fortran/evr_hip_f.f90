@@ -101,6 +101,24 @@ module evr_hip_f
       type(c_ptr), intent(out) :: out
       integer(c_int) :: evr_sg4_hermite_desc_modes
     end function
+    ! wavepacket files (sub_save_psi / sub_read_psi0 formats); path is a
+    ! C string: trim(name)//c_null_char
+    function evr_psi_save(basis, nb0, nwp, cplx, psi, version, formatted, path) &
+        bind(C, name="evr_psi_save")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: basis, psi
+      integer(c_int), value :: nb0, nwp, cplx, version, formatted
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: evr_psi_save
+    end function
+    function evr_psi_read(basis, nb0, cplx, max_wp, psi, nwp, path) bind(C, name="evr_psi_read")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: basis, psi
+      integer(c_int), value :: nb0, cplx, max_wp
+      integer(c_int), intent(out) :: nwp
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: evr_psi_read
+    end function
     function evr_dp_hermite_desc(D, nb, nq, out) bind(C, name="evr_dp_hermite_desc")
       import :: c_int, c_ptr
       integer(c_int), value :: D

@@ -48,6 +48,7 @@ extern "C" {
 #define EVR_ERR_HIP -2
 #define EVR_ERR_SIZE -3
 #define EVR_ERR_STATE -4
+#define EVR_ERR_IO -5
 
 /* evr_oppsi flags */
 #define EVR_FP64 0          /* all contractions in FP64                         */
@@ -140,6 +141,30 @@ void evr_basis_desc_free(evr_basis_desc *d);
 int evr_basis_desc_sizes(const evr_basis_desc *d, int64_t *out);
 /* Coordinates of every grid point (sum nq_t x D, point-major, host). */
 int evr_basis_desc_grid(const evr_basis_desc *d, double *q);
+
+/* ---- wavepacket files (host) ------------------------------------------- */
+/* Write nwp wavepackets of nb0 x nb_packed rows (state-major: row
+ * (i_e - 1) nb_packed + packed index; column-major; complex as (Re, Im)
+ * column pairs as evr_propagate) to `path` in the reference's save-file
+ * format, sub_save_psi (Source_ElVibRot/sub_WP/sub_module_psi_io.f90:365-424):
+ *  version 0, formatted (the reference's default, FilePsiVersion = 0):
+ *    "nb_psi nb_tot", then per wavepacket one line "n_1 .. n_D i_h i_e Re [Im]"
+ *    per function and "end wp k" (Write_Psi_nDBasis, :1660-1686);
+ *  version 1, formatted: namelist headerFile, the index list, then each
+ *    wavepacket list-directed (:1441-1604, :1750-1762);
+ *  version 1, unformatted: Fortran sequential records with 4-byte markers
+ *    and 4-byte integers (:1480-1487).
+ * n_d is the 1-based 1D function index of mode d (Rec_ndim_index), i_h = 1. */
+int evr_psi_save(const evr_basis_desc *basis, int nb0, int nwp, int cplx, const double *psi, int version,
+                 int formatted, const char *path);
+/* Read a file in any of those forms (detected from the header, as
+ * Read_header_saveFile_psi :1294-1439) into psi (nb0 x nb_packed x max_wp
+ * columns, x2 for complex; zeroed first): each value goes to the function
+ * with the same index tuple (n_1..n_D, i_h, i_e), so a file written on
+ * another basis is projected onto this one (Read_list_nDindBasis1_TO_
+ * nDindBasis2 :1143-1292).  *nwp = wavepackets read (at most max_wp). */
+int evr_psi_read(const evr_basis_desc *basis, int nb0, int cplx, int max_wp, double *psi, int *nwp,
+                 const char *path);
 
 /* ---- plans -------------------------------------------------------------- */
 /* Replaces the per-call setup of sub_TabOpPsi_FOR_SGtype4 (OPSG4:678-979):

@@ -64,6 +64,8 @@ def fixture(test, out_name):
     D = int(re.search(r"integer,\s*parameter\s*::\s*ndim=(\d+)", src).group(1))
     L = int(re.search(r"shell_run\s+\d+\s+(\d+)", jobs).group(1))
     B = [shell_int("B1"), shell_int("B2"), shell_int("B3"), shell_int("B4")] + [shell_int("B")] * (D - 4)
+    tail = shell[shell.index("New_Read_WP0"):]
+    wp0_line = re.search(r"^ *((?:\d+ +){%d}[-\d.]+ +[-\d.]+) *$" % (D + 2), tail, re.M).group(1)
     out = {
         "time_au": [r[0] for r in rows], "re": [r[1] for r in rows], "im": [r[2] for r in rows],
         "abs": [r[3] for r in rows],
@@ -73,6 +75,9 @@ def fixture(test, out_name):
         "basis": {"D": D, "L": L, "nqA": 1, "nqB": B, "nb_equals_nq": True},
         "propa": {"dt_fs": 0.1, "nsteps": 60, "npoly": 300, "poly_tol": 1e-10},
         "psi0": {"state": 2, "vibrational": "ground (all modes n=0)"},
+        # the initial wavepacket as the input gives it (New_Read_WP0): D
+        # 1-based function indices, i_h, i_e, Re, Im (lect_psiBasisRepnotall_nD)
+        "wp0_line": wp0_line,
     }
     assert len(out["w_eV"]) == len(out["k1_eV"]) == len(out["k2_eV"]) == D
     json.dump(out, open(os.path.join(HERE, out_name), "w"), indent=0)

@@ -92,3 +92,11 @@ def test_fortran_pyrazine_benchmark(tmp_path):
     assert np.abs(got[:, 0] - t).max() < 1e-12
     assert np.abs(got[:, 1] + 1j * got[:, 2] - ref).max() < 1e-9
     assert np.abs(got[:, 3] - np.abs(ref)).max() < 1e-9
+    # the final wavepacket, saved from Fortran in the reference's version-0
+    # format: unit norm and <psi0|psi(T)> = the last autocorrelation value
+    import evrhip
+    D, LG, LB, nbA, nbB, nqA, nqB = P.sg4_args()
+    b = evrhip.Basis.sg4_modes(D, LG, nbA, nbB, nqA, nqB, LB=LB)
+    psi = b.read_psi(str(out) + ".psi", nb0=2, cplx=True, max_wp=1)[:, 0]
+    assert abs(np.linalg.norm(psi) - 1.0) < 1e-9
+    assert abs(psi[b.nb_packed] - (got[-1, 1] + 1j * got[-1, 2])) < 1e-15
