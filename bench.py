@@ -235,6 +235,83 @@ def multi_run(args):
     print(json.dumps(line), flush=True)
 
 
+def pyrazine_run(args):
+    """The reference's 24D pyrazine propagation test (two coupled states, SG4
+    L=2 with a law per mode, 1,502 functions per state; 60 Chebyshev steps of
+    0.1 fs, npoly 300, poly_tol 1e-10): GPU wall time adaptive, captured and
+    matrix-free, the autocorrelation's distance to the reference's benchmark,
+    and the oracle's Chebyshev propagation (C/OpenMP H|psi> per order) on the
+    host cores over the first steps, scaled to 60."""
+    import numpy as np
+    import evrhip
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyrazine import P24 as P
+    D, LG, LB, nbA, nbB, nqA, nqB = P.sg4_args()
+    b = evrhip.Basis.sg4_modes(D, LG, nbA, nbB, nqA, nqB, LB=LB)
+    V11, V22, V12 = P.potentials(b.grid())
+
+    def coupled(assemble):
+        p11 = evrhip.Plan(b, type=1, V=V11, F2=P.F2(), max_nvec=2)
+        p22 = evrhip.Plan(b, type=1, V=V22, F2=P.F2(), max_nvec=2)
+        w = evrhip.Plan(b, type=0, V=V12, max_nvec=2)
+        if assemble:
+            for p in (p11, p22, w):
+                p.assemble(evrhip.ASM_FP64)
+        return evrhip.Coupled([[p11, w], [w, p22]])
+
+    n = b.nb_packed
+    p0 = np.zeros(2 * n, complex)
+    p0[n] = 1.0
+    t_ref, ac_ref = P.autocorr_ref()
+    out = {}
+    for name, assemble, graph in (("assembled", True, False), ("assembled_graph", True, True),
+                                  ("matrix_free", False, False)):
+        H = coupled(assemble)
+        H.propagate(p0, P.DT, 2, npoly=P.NPOLY, poly_tol=P.POLY_TOL, graph=graph)  # warm-up
+        t0 = time.perf_counter()
+        psi, ac, info = H.propagate(p0, P.DT, P.NSTEPS, npoly=P.NPOLY, poly_tol=P.POLY_TOL,
+                                    autocorr=True, graph=graph)
+        dt = time.perf_counter() - t0
+        out[name] = {"seconds": round(dt, 4), "hpsi_columns": info.hpsi, "max_order": info.max_order,
+                     "max_abs_autocorr_diff_vs_reference": float(np.abs(np.r_[1.0, ac[:, 0]] - ac_ref).max())}
+        bounds = (info.Hmin, info.Hmax)
+        H.close()
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        import propa
+        threads = args.cpu_threads or os.cpu_count()
+        s = oracle.SG4(D, LG, LB, nbA=nbA, nbB=nbB, nqA=nqA, nqB=nqB)
+        ops = [oracle.Op(1, V=V11, F2=P.F2()), oracle.Op(1, V=V22, F2=P.F2()), oracle.Op(0, V=V12)]
+
+        class CoupledOracle:
+            def __matmul__(self, w):
+                w = np.asarray(w)
+                r = np.concatenate([w.real, w.imag], axis=1)
+                a = s.oppsi(ops[0], r[:n], nthreads=threads) + s.oppsi(ops[2], r[n:], nthreads=threads)
+                c = s.oppsi(ops[2], r[:n], nthreads=threads) + s.oppsi(ops[1], r[n:], nthreads=threads)
+                y = np.concatenate([a, c])
+                k = w.shape[1]
+                return y[:, :k] + 1j * y[:, k:]
+        ksteps = 5
+        p = p0.reshape(-1, 1).copy()
+        t0 = time.perf_counter()
+        for _ in range(ksteps):
+            p, _, _ = propa.cheby_step(CoupledOracle(), p, P.DT, bounds[0], bounds[1], P.NPOLY, P.POLY_TOL)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(dt / ksteps * P.NSTEPS, 3), "unit": "s for 60 steps", "cores": threads,
+               "kind": "port",
+               "sample": "oracle march_cheby with the oracle's C/OpenMP coupled H|psi> per order, "
+                         "first %d of 60 steps (%.2f s wall), scaled" % (ksteps, dt)}
+    line = {"workload": "24D pyrazine, 2 coupled states, SG4 L=2 (1502 functions per state), "
+                        "60 Chebyshev steps of 0.1 fs, npoly 300, poly_tol 1e-10",
+            "gpu": out, "cpu_baseline": cpu}
+    if cpu:
+        line["gpu_speedup_assembled"] = round(cpu["value"] / out["assembled"]["seconds"], 1)
+    print(json.dumps(line), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this workload (profiles/traffic_sg4_d9l8.md), or None."""
@@ -273,6 +350,8 @@ def main():
                     help="only run the optimal-control objective, gradient and field-driven propagation")
     ap.add_argument("--multi", action="store_true",
                     help="only run the 64-system batched Chebyshev propagation against one system at a time")
+    ap.add_argument("--pyrazine", action="store_true",
+                    help="the reference's 24D pyrazine propagation test (two coupled states)")
     ap.add_argument("--nev", type=int, default=200)
     ap.add_argument("--conv-resi", type=float, default=1e-3)
     ap.add_argument("--refine-resi", type=float, default=1e-2)
@@ -293,6 +372,9 @@ def main():
         return
     if args.multi:
         multi_run(args)
+        return
+    if args.pyrazine:
+        pyrazine_run(args)
         return
 
     import numpy as np

@@ -73,6 +73,7 @@ struct SlotDev {
   int64_t offW;    // W: nb x nq  (pool "bw")
   int64_t offK;    // 7 factors of nb x nb (pool "keo")
   int64_t offX;    // nq grid points (pool "x")
+  int64_t offD1;   // dnRGG%d1: nq x nq (pool "d1", grid form of type 10 with G per point)
 };
 
 // kModeAssemble: H|psi> of the term's own unit vectors e_c, c = c0 .. c0+NVW-1,

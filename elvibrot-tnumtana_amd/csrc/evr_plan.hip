@@ -104,8 +104,16 @@ struct evr_plan {
   DevBuf<evr::SlotDev> d_slots;
   std::vector<evr::StepDev> steps;
   DevBuf<evr::StepDev> d_steps;
-  DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag;
+  DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag, d_D1;
   DevBuf<float> d_bw32, d_keo32;
+  // type 10 with G, J, s per grid point (evr_plan_set_keo_grid): the tables,
+  // the (term, first point) items of the grid kernels, and work grids for
+  // gvec vectors at a time (psi_G, the D derivative columns, the KEO output)
+  bool hermitian = false, keo_grid = false;
+  int gvec = 0;
+  int64_t n_gitems = 0;
+  DevBuf<double> d_GG, d_Jg, d_Sg, d_Gg, d_Pg, d_Fg, d_tmpb;
+  DevBuf<int2> d_gitems;
   DevBuf<int> d_map, d_rowpos;
   // reduction segments: rows seg[s] .. seg[s+1]-1 of R; the segments of packed
   // function pk are fseg[pk] .. fseg[pk+1]-1
@@ -473,6 +481,42 @@ hipError_t launch_items(int c, const evr::TermParams<double> &prm, int64_t nitem
   }
 }
 
+// The kinetic part of a plan with G, J, s per grid point, added to the
+// packed result dh (device, nb_packed x nvec): B->G, the grid form of the
+// term (k_gk_deriv, k_gk_combine, k_gk_out), then the weighted G->B and the
+// deterministic scatter per vector, gvec vectors at a time.
+int add_keo_grid(evr_plan *p, int nvec, const double *dpsi, double *dh)
+{
+  const int D = p->D;
+  const int64_t nq = p->sum_nq, nb = p->nb_packed;
+  for (int c0 = 0; c0 < nvec; c0 += p->gvec) {
+    const int k = std::min(p->gvec, nvec - c0);
+    int rc = evr::plan_b2g_device(p, k, dpsi + nb * c0, p->d_Gg.ptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(evr::k_gk_deriv, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                       p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Sg.ptr,
+                       p->d_Gg.ptr, nq, k, p->d_Pg.ptr);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(evr::k_gk_combine, dim3(blocks_for(nq)), dim3(256), 0, p->stream, D, nq, k,
+                       p->d_GG.ptr, p->d_Jg.ptr, p->d_Pg.ptr);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(evr::k_gk_out, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                       p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
+                       p->d_Sg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
+    HIP_TRY(hipGetLastError());
+    for (int v = 0; v < k; ++v) {
+      rc = run_terms<double>(p, evr::kModeG2B, 1, nullptr, p->d_R.ptr, nullptr, p->d_Fg.ptr + nq * v,
+                             p->d_bw64.ptr, p->d_keo64.ptr, p->d_V.ptr);
+      if (rc) return rc;
+      HIP_TRY(launch_reduce<double>(p, p->d_R.ptr, 1, p->d_tmpb.ptr));
+      hipLaunchKernelGGL(evr::k_axpy, dim3(blocks_for(nb)), dim3(256), 0, p->stream, nb, 1.0,
+                         p->d_tmpb.ptr, dh + nb * (c0 + v));
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  return EVR_OK;
+}
+
 // V_eff = V summed over the one-function modes of every term (k_fold_v),
 // queued on the plan stream after the V it reads.
 int fold_v(evr_plan *p)
@@ -654,6 +698,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   const int D = bd->D, nlev = bd->nlev;
   p->D = D;
   p->D_full = D;
+  p->hermitian = op->hermitian != 0;
   p->type = op->type;
   p->max_nvec = max_nvec;
   p->dp_order = bd->grid_last_fastest != 0;
@@ -665,7 +710,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   const int ns = D * nlev;
   std::vector<int64_t> ox, oB, oD;
   slot_offsets(bd, ox, oB, oD);
-  std::vector<double> bw, keo, xs, bwdiag;
+  std::vector<double> bw, keo, xs, bwdiag, d1;
   std::vector<char> slot_fold(ns, 0);  // one function with vanishing P, Q, R
   p->slots.resize(ns);
   for (int s = 0; s < ns; ++s) {
@@ -686,6 +731,8 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
         bwdiag.push_back(bd->W[oB[s] + b + (int64_t)nb * q] * bd->B[oB[s] + q + (int64_t)nq * b]);
     sd.offX = (int64_t)xs.size();
     xs.insert(xs.end(), bd->x + ox[s], bd->x + ox[s + 1]);
+    sd.offD1 = (int64_t)d1.size();
+    d1.insert(d1.end(), bd->D1 + oD[s], bd->D1 + oD[s + 1]);
     evr::Slot1D s1;
     s1.nb = nb;
     s1.nq = nq;
@@ -788,6 +835,8 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
       bwdiag.push_back(W * B);
       sd.offX = (int64_t)xs.size();
       xs.push_back(0.0);
+      sd.offD1 = (int64_t)d1.size();
+      d1.push_back(0.0);
       sd.offK = (int64_t)keo.size();
       keo.insert(keo.end(), evr::kNumFactors, 0.0);
       keo[sd.offK + Td] = fTd;
@@ -1028,6 +1077,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   HIP_TRY(p->d_bw32.upload(bw32.data(), bw32.size(), p->stream));
   HIP_TRY(p->d_keo32.upload(keo32.data(), keo32.size(), p->stream));
   HIP_TRY(p->d_x.upload(xs.data(), xs.size(), p->stream));
+  HIP_TRY(p->d_D1.upload(d1.data(), d1.size(), p->stream));
   HIP_TRY(p->d_map.upload(bd->map, onb, p->stream));
   {
     std::vector<int> seg, fseg(p->nb_packed + 1);
@@ -1174,6 +1224,45 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
   return EVR_OK;
 }
 
+int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const double *sqRhoOVERJac)
+{
+  if (!p || !G) return fail(EVR_ERR_ARG, "NULL argument");
+  if (p->type != 10) return fail(EVR_ERR_ARG, "G per grid point is a type-10 operator");
+  if (p->D != p->D_full) return fail(EVR_ERR_ARG, "G per grid point needs D <= 12");
+  if (p->dp_order) return fail(EVR_ERR_ARG, "G per grid point is for SG4 bases");
+  if (p->hermitian) return fail(EVR_ERR_ARG, "G per grid point has no symmetric form (hermitian = 0)");
+  const int D = p->D;
+  const int64_t nq = p->sum_nq;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->d_A64.release();
+  p->d_A32.release();
+  HIP_TRY(p->d_GG.upload(G, (size_t)D * D * nq, p->stream));
+  std::vector<double> one;
+  if (!Jac || !sqRhoOVERJac) one.assign(nq, 1.0);
+  HIP_TRY(p->d_Jg.upload(Jac ? Jac : one.data(), nq, p->stream));
+  HIP_TRY(p->d_Sg.upload(sqRhoOVERJac ? sqRhoOVERJac : one.data(), nq, p->stream));
+  if (!p->keo_grid) {
+    std::vector<int2> items;
+    for (int64_t t = 0; t < p->nterms; ++t)
+      for (int q0 = 0; q0 < p->terms[t].nqt; q0 += 256) items.push_back(make_int2((int)t, q0));
+    p->n_gitems = (int64_t)items.size();
+    HIP_TRY(p->d_gitems.upload(items.data(), items.size(), p->stream));
+    // work grids of gvec vectors: (D + 2) x sum_nq doubles each
+    p->gvec = (int)std::max<int64_t>(1, std::min<int64_t>(p->max_nvec, (int64_t)(8e9 / (8.0 * (D + 2) * nq))));
+    HIP_TRY(p->d_Gg.alloc((size_t)nq * p->gvec));
+    HIP_TRY(p->d_Pg.alloc((size_t)nq * D * p->gvec));
+    HIP_TRY(p->d_Fg.alloc((size_t)nq * p->gvec));
+    HIP_TRY(p->d_tmpb.alloc((size_t)p->nb_packed));
+  }
+  // the kinetic part leaves the term kernels (they keep the V part)
+  p->hasKeo = false;
+  p->hasA = false;
+  p->hasF1 = false;
+  p->keo_grid = true;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return EVR_OK;
+}
+
 int evr_plan_sizes(const evr_plan *p, int64_t *out)
 {
   if (!p || !out) return fail(EVR_ERR_ARG, "NULL argument");
@@ -1191,6 +1280,8 @@ int evr_plan_assemble(evr_plan *p, int which)
   if (!p) return fail(EVR_ERR_ARG, "NULL plan");
   if (which < 1 || which > (EVR_ASM_FP64 | EVR_ASM_FP32))
     return fail(EVR_ERR_ARG, "which must be EVR_ASM_FP64 and/or EVR_ASM_FP32");
+  if (p->keo_grid)
+    return fail(EVR_ERR_ARG, "a plan with G, J, s per grid point runs the grid form (not assembled)");
   const size_t n = (size_t)p->asm_elems;
   if ((which & EVR_ASM_FP64) && !p->d_A64.ptr) {
     if (p->d_A64.alloc(n) != hipSuccess) {
@@ -1337,6 +1428,7 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
     dh = p->d_hpsi.ptr;
   }
   int rc;
+  if (p->keo_grid && fp32) return fail(EVR_ERR_ARG, "a plan with G, J, s per grid point is FP64 only");
   const bool use_asm =
       !(flags & EVR_MATRIX_FREE) && (fp32 ? p->d_A32.ptr != nullptr : p->d_A64.ptr != nullptr);
   if (use_asm) {
@@ -1364,6 +1456,10 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
                            p->d_keo64.ptr, p->d_Veff.ptr);
     if (rc) return rc;
     HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
+    if (p->keo_grid) {
+      rc = add_keo_grid(p, nvec, dpsi, dh);
+      if (rc) return rc;
+    }
   } else {
     rc = ensure_v32(p);
     if (rc) return rc;

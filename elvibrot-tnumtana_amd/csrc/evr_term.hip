@@ -1120,5 +1120,95 @@ __global__ __launch_bounds__(256) void k_zero(T *__restrict__ a, int64_t n)
     a[g] = T(0);
 }
 
+// ------------------------------------------- type 10 with G, J, s per point --
+// The reference's grid form of one term (OPSG4:1566-1632) over the
+// term-concatenated grids, k vectors at a time, column v of a grid array at
+// v * sum_nq:  P_j = d_j (s psi_G);  h_i = J sum_j G(:, j, i) P_j;
+// out = -1/2 (sum_i d_i h_i) / (J s).  A derivative along mode j is the slot's
+// dnRGG%d1 (nq x nq, column-major) applied along that mode of the term grid
+// (first mode fastest).  One thread per grid point; items = (term, first point).
+
+// mode-j derivative of column `in` (optionally scaled by sc) at point p of a term
+__device__ __forceinline__ double grid_d1(const double *__restrict__ Dm, int nq, int qj, int64_t st,
+                                          const double *__restrict__ in, const double *__restrict__ sc,
+                                          int64_t p0)
+{
+  double acc = 0.0;
+  for (int k = 0; k < nq; ++k) {
+    const int64_t g = p0 + k * st;
+    acc += Dm[qj + nq * k] * (sc ? sc[g] * in[g] : in[g]);
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_gk_deriv(int D, const TermDev *__restrict__ terms,
+                                                  const SlotDev *__restrict__ slots,
+                                                  const double *__restrict__ D1, const int2 *__restrict__ items,
+                                                  const double *__restrict__ s, const double *__restrict__ G,
+                                                  int64_t sum_nq, int k, double *__restrict__ P)
+{
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int p = it.y + threadIdx.x;
+  if (p >= td.nqt) return;
+  int64_t st = 1;
+  for (int j = 0; j < D; ++j) {
+    const SlotDev &sl = slots[td.slot[j]];
+    const int nq = sl.nq;
+    const int qj = (int)((p / st) % nq);
+    const int64_t p0 = td.off_nq + p - qj * st;
+    for (int v = 0; v < k; ++v)
+      P[((int64_t)v * D + j) * sum_nq + td.off_nq + p] = grid_d1(D1 + sl.offD1, nq, qj, st, G + v * sum_nq, s, p0);
+    st *= nq;
+  }
+}
+
+// h_i = J sum_j G[(i D + j) sum_nq + g] P_j, in place (all P_j of the point
+// are read before any h_i is written)
+__global__ __launch_bounds__(256) void k_gk_combine(int D, int64_t sum_nq, int k, const double *__restrict__ GG,
+                                                    const double *__restrict__ J, double *__restrict__ P)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < sum_nq;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const double jg = J[g];
+    for (int v = 0; v < k; ++v) {
+      double pj[kMaxDev];
+      for (int j = 0; j < D; ++j) pj[j] = P[((int64_t)v * D + j) * sum_nq + g];
+      for (int i = 0; i < D; ++i) {
+        double h = 0.0;
+        for (int j = 0; j < D; ++j) h += GG[((int64_t)i * D + j) * sum_nq + g] * pj[j];
+        P[((int64_t)v * D + i) * sum_nq + g] = jg * h;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict__ terms,
+                                                const SlotDev *__restrict__ slots, const double *__restrict__ D1,
+                                                const int2 *__restrict__ items, const double *__restrict__ J,
+                                                const double *__restrict__ s, const double *__restrict__ P,
+                                                int64_t sum_nq, int k, double *__restrict__ F)
+{
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int p = it.y + threadIdx.x;
+  if (p >= td.nqt) return;
+  const int64_t g = td.off_nq + p;
+  const double scale = -0.5 / (J[g] * s[g]);
+  for (int v = 0; v < k; ++v) {
+    double o = 0.0;
+    int64_t st = 1;
+    for (int i = 0; i < D; ++i) {
+      const SlotDev &sl = slots[td.slot[i]];
+      const int nq = sl.nq;
+      const int qi = (int)((p / st) % nq);
+      o += grid_d1(D1 + sl.offD1, nq, qi, st, P + ((int64_t)v * D + i) * sum_nq, nullptr,
+                   td.off_nq + p - qi * st);
+      st *= nq;
+    }
+    F[v * sum_nq + g] = scale * o;
+  }
+}
+
 // explicit instantiations are produced by the launch table in evr_plan.hip
 }  // namespace evr

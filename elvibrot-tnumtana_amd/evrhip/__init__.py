@@ -171,6 +171,14 @@ class Plan:
         V = np.ascontiguousarray(V, dtype=np.float64)
         check(load().evr_plan_set_potential(self.h, _d(V)))
 
+    def set_keo_grid(self, Gq, Jac=None, s=None):
+        """Type 10 with G, J and s = sqrt(rho/J) per grid point: Gq is
+        (sum_nq, D, D) with Gq[g, j, i] = GGiq(g, j, i) of the reference."""
+        G = np.asarray(Gq, dtype=np.float64).ravel(order="F").copy()
+        J = None if Jac is None else np.ascontiguousarray(Jac, dtype=np.float64)
+        S = None if s is None else np.ascontiguousarray(s, dtype=np.float64)
+        check(load().evr_plan_set_keo_grid(self.h, _d(G), _d(J), _d(S)))
+
     def gen_potential_poly(self, ham):
         D = ham.D
         c3 = np.ascontiguousarray(ham.c3.reshape(D * D * D))   # (i*D + j)*D + k

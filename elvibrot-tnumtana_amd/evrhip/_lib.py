@@ -80,6 +80,7 @@ _SIGS = {
                                        ctypes.POINTER(ctypes.c_void_p)]),
     "evr_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "evr_plan_set_potential": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "evr_plan_set_keo_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "evr_plan_gen_potential_poly": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "evr_plan_sizes": (ctypes.c_int, [ctypes.c_void_p, _i64p]),
     "evr_plan_stream": (ctypes.c_void_p, [ctypes.c_void_p]),

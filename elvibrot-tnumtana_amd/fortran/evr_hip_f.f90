@@ -161,6 +161,14 @@ module evr_hip_f
       real(c_double), intent(in) :: V(*)
       integer(c_int) :: evr_plan_set_potential
     end function
+    ! G(sum_nq, D, D) = GGiq per grid point; Jac, sqRhoOVERJac: c_loc of
+    ! sum_nq arrays or c_null_ptr (= 1)
+    function evr_plan_set_keo_grid(p, G, Jac, sqRhoOVERJac) bind(C, name="evr_plan_set_keo_grid")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: p, Jac, sqRhoOVERJac
+      real(c_double), intent(in) :: G(*)
+      integer(c_int) :: evr_plan_set_keo_grid
+    end function
     function evr_plan_gen_potential_poly(p, v2, c3, d22, e4) bind(C, name="evr_plan_gen_potential_poly")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: p

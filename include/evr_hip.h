@@ -180,6 +180,18 @@ int evr_plan_create(const evr_basis_desc *basis, const evr_op_desc *op, int max_
 int evr_plan_destroy(evr_plan *p);
 /* Replace V (host array, sum nq_t values). */
 int evr_plan_set_potential(evr_plan *p, const double *V);
+/* Type 10 with the kinetic tables per grid point, as the reference's
+ * get_OpGrid_type10_OF_ONEDP_FOR_SG4 (OPSG4:2538-2816) gets them from Tnum
+ * (get_d0GG): G is Fortran (sum_nq, D, D) over the term-concatenated grid,
+ * G(g, j, i) = GGiq(g, j, i); Jac and sqRhoOVERJac are sum_nq values or NULL
+ * (= 1).  From then on the kinetic part is the reference's grid form
+ * (OPSG4:1566-1632): -1/2 (1/(J s)) sum_i d_i [J sum_j G(:, j, i) d_j (s psi)]
+ * with d = dnRGG%d1, on every term grid; G0/Ga are no longer used.  SG4 bases
+ * with D <= 12, hermitian = 0, FP64 and matrix-free only (EVR_ERR_ARG for an
+ * assembly or FP32); the plan diagonal then holds the V part only.  Device
+ * memory: (D^2 + 2) sum_nq tables and (D + 2) sum_nq work doubles per vector
+ * processed at once. */
+int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const double *sqRhoOVERJac);
 /* Generate V on the device: V(q) = sum_i v2_i q_i^2 + sum_{i<=j<=k} c_ijk
  * q_i q_j q_k + sum_{i<j} d_ij q_i^2 q_j^2 + sum_i e_i q_i^4, with c given as a
  * dense D^3 array (only i<=j<=k read), d as D x D (only i<j read). */

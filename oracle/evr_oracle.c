@@ -611,9 +611,9 @@ static void term_deriv(int D, const orc_1d *const *b, int dim, const double *M,
 /* grid combine of one term, restating OPSG4:1448-1650 (nb0 = 1).
  * psiG is overwritten by (H psi)_G.  qpt: grid coordinates (nq x D). */
 static void term_grid_op(int D, const orc_1d *const *b, const orc_op *op, long long nq,
-                         const double *V, const double *J, const double *s,
-                         const double *qpt, double *psiG, double *w1, double *w2,
-                         double *PsiRj, double *PsiRi, double *OpPsiR)
+                         const double *V, const double *J, const double *s, const double *Gq,
+                         long long gstride, const double *qpt, double *psiG, double *w1,
+                         double *w2, double *PsiRj, double *PsiRi, double *OpPsiR)
 {
   if (op->type == 0) {
     /* OPSG4:1448-1488 */
@@ -661,9 +661,14 @@ static void term_grid_op(int D, const orc_1d *const *b, const orc_op *op, long l
     for (long long p = 0; p < nq; ++p) {
       double acc = 0.0;
       for (int j = 0; j < D; ++j) {
-        double g = op->G0[j + D * i];
-        if (op->Ga)
-          for (int k = 0; k < D; ++k) g += op->Ga[j + D * (i + D * k)] * qpt[p * D + k];
+        double g;
+        if (Gq) {
+          g = Gq[p + gstride * (j + (long long)D * i)];   /* GGiq(p, j, i) */
+        } else {
+          g = op->G0[j + D * i];
+          if (op->Ga)
+            for (int k = 0; k < D; ++k) g += op->Ga[j + D * (i + D * k)] * qpt[p * D + k];
+        }
         acc += g * PsiRj[(size_t)nq * j + p];
       }
       PsiRi[p] = acc * (J ? J[p] : 1.0);
@@ -735,7 +740,7 @@ static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const do
       long long nbt = s->off_nb[t + 1] - s->off_nb[t];
       long long nqt = s->off_nq[t + 1] - s->off_nq[t];
       const int *map = s->map + s->off_nb[t];
-      if (op->type == 10 && op->Ga) {
+      if (op->type == 10 && op->Ga && !op->Gq) {
         for (long long p = 0; p < nqt; ++p) {
           long long r = p;
           for (int d = 0; d < D; ++d) { qpt[p * D + d] = b[d]->x[r % b[d]->nq]; r /= b[d]->nq; }
@@ -749,7 +754,8 @@ static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const do
         /* gather (BGSG4:1220-1238) */
         for (long long e = 0; e < nbt; ++e) x[e] = map[e] >= 0 ? pv[map[e]] : 0.0;
         term_b2g(D, b, x, G, T);
-        term_grid_op(D, b, op, nqt, V, J, sr, qpt, G, w1, w2, PsiRj, PsiRi, OpPsiR);
+        term_grid_op(D, b, op, nqt, V, J, sr, op->Gq ? op->Gq + s->off_nq[t] : NULL, s->sum_nq, qpt, G,
+                     w1, w2, PsiRj, PsiRi, OpPsiR);
         term_g2b(D, b, G, y, T, w2);
         /* weighted scatter (BGSG4:1273-1287) with OMP ATOMIC */
         double wt = s->weights[t];

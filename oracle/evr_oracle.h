@@ -93,6 +93,10 @@ typedef struct {
   const double *F1;    /* D (type 1), may be NULL                            */
   const double *Jac;   /* per grid point Jacobian J, NULL = 1 (type 10)      */
   const double *sqRhoOVERJac; /* per grid point s = sqrt(rho/J), NULL = 1    */
+  const double *Gq;    /* SG4 type 10: G per grid point in place of G0/Ga,
+                          Fortran (sum_nq, D, D): Gq(g, j, i) = GGiq(:,j,i) of
+                          get_OpGrid_type10_OF_ONEDP_FOR_SG4 (OPSG4:2538-2816),
+                          the term-concatenated grid; NULL: G0 + Ga q        */
 } orc_op;
 
 /* H|psi> on the SG4 basis: sub_TabOpPsi_FOR_SGtype4 (OPSG4:678-979) with

@@ -70,19 +70,20 @@ def _i(a):
 
 class _Op(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("V", _dp), ("G0", _dp), ("Ga", _dp), ("F2", _dp),
-                ("F1", _dp), ("Jac", _dp), ("sqRhoOVERJac", _dp)]
+                ("F1", _dp), ("Jac", _dp), ("sqRhoOVERJac", _dp), ("Gq", _dp)]
 
 
 class Op:
     """Operator description (type 0 / 1 / 10), arrays kept alive here."""
 
-    def __init__(self, type, V=None, G0=None, Ga=None, F2=None, F1=None, Jac=None, s=None):
+    def __init__(self, type, V=None, G0=None, Ga=None, F2=None, F1=None, Jac=None, s=None, Gq=None):
+        """Gq: SG4 type 10 G per grid point, (sum_nq, D, D) with Gq[g, j, i] = GGiq(g, j, i)."""
         def f(a):
-            return None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel(order="F")
-        self.arrays = dict(V=f(V), G0=f(G0), Ga=f(Ga), F2=f(F2), F1=f(F1), Jac=f(Jac), s=f(s))
+            return None if a is None else np.asarray(a, dtype=np.float64).ravel(order="F").copy()
+        self.arrays = dict(V=f(V), G0=f(G0), Ga=f(Ga), F2=f(F2), F1=f(F1), Jac=f(Jac), s=f(s), Gq=f(Gq))
         a = self.arrays
         self.st = _Op(type, _d(a["V"]), _d(a["G0"]), _d(a["Ga"]), _d(a["F2"]), _d(a["F1"]),
-                      _d(a["Jac"]), _d(a["s"]))
+                      _d(a["Jac"]), _d(a["s"]), _d(a["Gq"]))
 
 
 def herm_quadrature(nq):

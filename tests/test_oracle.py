@@ -147,6 +147,68 @@ def test_smolyak_matches_dense_kronecker(oracle_mod):
     assert np.abs(H - Hd).max() < 1e-11 * np.abs(Hd).max()
 
 
+def tabulated_keo(q, seed):
+    """A per-point G (not symmetric in i, j, so a transposed index shows), a
+    Jacobian and s = sqrt(rho/J) on the grid points q."""
+    rng = np.random.default_rng(seed)
+    npt, D = q.shape
+    G0 = np.diag(rng.uniform(0.8, 1.4, D)) + 0.05 * rng.standard_normal((D, D))
+    A = 0.03 * rng.standard_normal((D, D, D))
+    Gq = G0[None] + np.einsum("pk,ijk->pij", q, A) + 0.01 * (q ** 2).sum(1)[:, None, None]
+    J = 1.0 + 0.1 * np.tanh(q @ rng.standard_normal(D))
+    s = 1.0 / np.sqrt(J) * (1.0 + 0.05 * np.cos(q[:, 0]))
+    return Gq, J, s   # Gq[p, j, i] multiplies d_j in the i-th flux
+
+
+def test_smolyak_tabulated_keo_matches_dense_kronecker(oracle_mod):
+    """D=2, type 10 with G, J and s per grid point: -1/2 (1/(J s)) sum_i d_i
+    [J sum_j G(:,j,i) d_j (s psi)] + V psi per term against dense Kronecker
+    operators; the same with G(:,i,j) in place of G(:,j,i) differs."""
+    D, L = 2, 3
+    s = oracle_mod.SG4(D, L)
+    q = s.grid()
+    Gq, J, sq = tabulated_keo(q, 11)
+    V = 0.5 * (q ** 2).sum(1)
+    H = s.oppsi(oracle_mod.Op(10, V=V, Jac=J, s=sq, Gq=Gq), np.eye(s.nb_packed))
+    lv, w, onb, onq = s.terms()
+    m = s.map()
+    Hd = np.zeros((s.nb_packed, s.nb_packed))
+    Hx = np.zeros_like(Hd)
+    for t in range(s.nterms):
+        one = [s.one_d(d, lv[t, d]) for d in range(D)]
+        K = lambda A0, A1: np.kron(A1, A0)
+        Bt = K(one[0]["B"], one[1]["B"]); Wt = K(one[0]["W"], one[1]["W"])
+        I0, I1 = np.eye(one[0]["nq"]), np.eye(one[1]["nq"])
+        Dm = [K(one[0]["D1"], I1), K(I0, one[1]["D1"])]
+        sl = slice(onq[t], onq[t + 1])
+        Jt, st = J[sl], sq[sl]
+        kin = sum(Dm[i] @ np.diag(Jt * Gq[sl, j, i]) @ Dm[j] for i in range(D) for j in range(D))
+        kinx = sum(Dm[i] @ np.diag(Jt * Gq[sl, i, j]) @ Dm[j] for i in range(D) for j in range(D))
+        op = np.diag(V[sl]) - 0.5 * np.diag(1.0 / (Jt * st)) @ kin @ np.diag(st)
+        opx = np.diag(V[sl]) - 0.5 * np.diag(1.0 / (Jt * st)) @ kinx @ np.diag(st)
+        rows = m[onb[t]:onb[t + 1]]
+        Hd[np.ix_(rows, rows)] += w[t] * (Wt @ op @ Bt)
+        Hx[np.ix_(rows, rows)] += w[t] * (Wt @ opx @ Bt)
+    assert np.abs(H - Hd).max() < 1e-11 * np.abs(Hd).max()
+    assert np.abs(H - Hx).max() > 1e-4 * np.abs(Hd).max()
+
+
+def test_tabulated_g_equals_polynomial(oracle_mod):
+    """G0 + Ga q tabulated on the grid gives the polynomial operator to rounding."""
+    D, L = 4, 3
+    s = oracle_mod.SG4(D, L)
+    q = s.grid()
+    rng = np.random.default_rng(2)
+    G0 = np.diag([1.0, 1.2, 0.9, 1.1])
+    Ga = rng.uniform(-0.02, 0.02, (D, D, D))
+    Gq = G0[None] + np.einsum("pk,ijk->pij", q, Ga)
+    X = rng.standard_normal((s.nb_packed, 3))
+    V = 0.5 * (q ** 2).sum(1)
+    a = s.oppsi(oracle_mod.Op(10, V=V, G0=G0, Ga=Ga.ravel(order="F")), X)
+    b = s.oppsi(oracle_mod.Op(10, V=V, Gq=Gq), X)
+    assert np.abs(a - b).max() < 1e-13 * np.abs(a).max()
+
+
 def _hh_levels(oracle_mod, D, L):
     s = oracle_mod.SG4(D, L, L, nbA=1, nbB=2, nqA=1, nqB=2)
     V = hh_potential(s.grid())
