@@ -294,7 +294,7 @@ def pyrazine_run(args):
                 y = np.concatenate([a, c])
                 k = w.shape[1]
                 return y[:, :k] + 1j * y[:, k:]
-        ksteps = 5
+        ksteps = P.NSTEPS
         p = p0.reshape(-1, 1).copy()
         t0 = time.perf_counter()
         for _ in range(ksteps):
@@ -309,6 +309,72 @@ def pyrazine_run(args):
             "gpu": out, "cpu_baseline": cpu}
     if cpu:
         line["gpu_speedup_assembled"] = round(cpu["value"] / out["assembled"]["seconds"], 1)
+    print(json.dumps(line), flush=True)
+
+
+def keo_grid_run(args):
+    """Type 10 with G, J and s tabulated per grid point (the reference's grid
+    form) on D=6 SG4 L=5: H|psi> of 8 vectors per step on the GPU, the same
+    operator through the oracle on the host cores.  G = G0 + Ga q of the
+    D=6 L=5 Hamiltonian plus a q^2 term, J and s smooth positive functions."""
+    import numpy as np
+    import torch
+    import evrhip
+    from evrhip import hamiltonian as H
+    D, L, nvec = 6, 5, 8
+    ham = H.smolyak_config(1, D)
+    b = evrhip.Basis.sg4(D, L)
+    q = b.grid()
+    Ga = ham.Ga_fortran().reshape(D, D, D, order="F")
+    Gq = ham.G0[None] + np.einsum("pk,ijk->pji", q, Ga) + 5.0 * (q ** 2).sum(1)[:, None, None] * np.eye(D)[None]
+    J = 1.0 + 0.1 * np.tanh(q @ np.linspace(-0.3, 0.3, D))
+    s = 1.0 / np.sqrt(J)
+    V = ham.potential(q)
+    plan = evrhip.Plan(b, type=10, V=V, G0=ham.G0, max_nvec=nvec)
+    plan.set_keo_grid(Gq, J, s)
+    X = torch.from_numpy(np.ascontiguousarray(H.test_vectors(b.nb_packed, nvec, 1).T)).cuda()
+    Hx = torch.empty_like(X)
+    stream = torch.cuda.ExternalStream(plan.stream())
+    for _ in range(max(1, args.warmup)):
+        plan.TabOpPsi_device(X.data_ptr(), Hx.data_ptr(), nvec)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.TabOpPsi_device(X.data_ptr(), Hx.data_ptr(), nvec)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    # algorithmic HBM bytes of the three grid kernels per vector and grid point:
+    # deriv reads psi_G and s, writes D columns; combine reads D columns, D^2
+    # G values and J, writes D; out reads D columns, J and s, writes 1
+    per_point = 8.0 * ((2 + D) + (D + D * D + 1 + D) + (D + 2 + 1))
+    line = {"workload": "type 10 with G, J, s per grid point, SG4 D=6 L=5 (%d functions, %d grid points), "
+                        "%d vectors" % (b.nb_packed, b.sum_nq, nvec),
+            "ms_per_hpsi_block": round(ms, 3), "hpsi_per_s_per_vector": round(nvec * 1e3 / ms, 1),
+            "grid_kernel_bytes_per_block_gb": round(per_point * b.sum_nq * nvec / 1e9, 3)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = args.cpu_threads or os.cpu_count()
+        so = oracle.SG4(D, L)
+        op = oracle.Op(10, V=V, Jac=J, s=s, Gq=Gq)
+        Xh = X.cpu().numpy().T.copy(order="F")
+        ncalls = 0
+        t0 = time.perf_counter()
+        while True:   # the 8 vectors, repeated for at least 5 s of CPU work
+            ref = so.oppsi(op, Xh, nthreads=threads)
+            ncalls += 1
+            dt = time.perf_counter() - t0
+            if dt > 5.0:
+                break
+        got = Hx.cpu().numpy().T
+        line["max_rel_diff_vs_oracle"] = float(np.abs(got - ref).max() / np.abs(ref).max())
+        line["cpu_baseline"] = {"value": round(nvec * ncalls / dt, 3), "unit": "Hpsi/s per vector",
+                                "cores": threads, "kind": "port",
+                                "sample": "oracle grid form (C/OpenMP), %d calls of %d vectors, %.1f s"
+                                          % (ncalls, nvec, dt)}
+        line["gpu_over_cpu"] = round(line["hpsi_per_s_per_vector"] / line["cpu_baseline"]["value"], 1)
     print(json.dumps(line), flush=True)
 
 
@@ -352,6 +418,8 @@ def main():
                     help="only run the 64-system batched Chebyshev propagation against one system at a time")
     ap.add_argument("--pyrazine", action="store_true",
                     help="the reference's 24D pyrazine propagation test (two coupled states)")
+    ap.add_argument("--keo-grid", action="store_true",
+                    help="type 10 with G, J, s per grid point on D=6 SG4 L=5")
     ap.add_argument("--nev", type=int, default=200)
     ap.add_argument("--conv-resi", type=float, default=1e-3)
     ap.add_argument("--refine-resi", type=float, default=1e-2)
@@ -375,6 +443,9 @@ def main():
         return
     if args.pyrazine:
         pyrazine_run(args)
+        return
+    if args.keo_grid:
+        keo_grid_run(args)
         return
 
     import numpy as np

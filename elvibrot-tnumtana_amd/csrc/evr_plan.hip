@@ -112,7 +112,7 @@ struct evr_plan {
   bool hermitian = false, keo_grid = false;
   int gvec = 0;
   int64_t n_gitems = 0;
-  DevBuf<double> d_GG, d_Jg, d_Sg, d_Gg, d_Pg, d_Fg, d_tmpb;
+  DevBuf<double> d_GG, d_Jg, d_Sg, d_Gg, d_Pg, d_Fg;
   DevBuf<int2> d_gitems;
   DevBuf<int> d_map, d_rowpos;
   // reduction segments: rows seg[s] .. seg[s+1]-1 of R; the segments of packed
@@ -481,11 +481,11 @@ hipError_t launch_items(int c, const evr::TermParams<double> &prm, int64_t nitem
   }
 }
 
-// The kinetic part of a plan with G, J, s per grid point, added to the
-// packed result dh (device, nb_packed x nvec): B->G, the grid form of the
-// term (k_gk_deriv, k_gk_combine, k_gk_out), then the weighted G->B and the
-// deterministic scatter per vector, gvec vectors at a time.
-int add_keo_grid(evr_plan *p, int nvec, const double *dpsi, double *dh)
+// H|psi> of a plan with G, J, s per grid point into dh (device, nb_packed x
+// nvec): B->G, the reference's grid form of every term (k_gk_deriv,
+// k_gk_combine, k_gk_out, V included), then the weighted G->B and the
+// deterministic scatter, gvec vectors at a time.
+int keo_grid_hpsi(evr_plan *p, int nvec, const double *dpsi, double *dh)
 {
   const int D = p->D;
   const int64_t nq = p->sum_nq, nb = p->nb_packed;
@@ -502,17 +502,12 @@ int add_keo_grid(evr_plan *p, int nvec, const double *dpsi, double *dh)
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(evr::k_gk_out, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
                        p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
-                       p->d_Sg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
+                       p->d_Sg.ptr, p->d_V.ptr, p->d_Gg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
     HIP_TRY(hipGetLastError());
-    for (int v = 0; v < k; ++v) {
-      rc = run_terms<double>(p, evr::kModeG2B, 1, nullptr, p->d_R.ptr, nullptr, p->d_Fg.ptr + nq * v,
-                             p->d_bw64.ptr, p->d_keo64.ptr, p->d_V.ptr);
-      if (rc) return rc;
-      HIP_TRY(launch_reduce<double>(p, p->d_R.ptr, 1, p->d_tmpb.ptr));
-      hipLaunchKernelGGL(evr::k_axpy, dim3(blocks_for(nb)), dim3(256), 0, p->stream, nb, 1.0,
-                         p->d_tmpb.ptr, dh + nb * (c0 + v));
-      HIP_TRY(hipGetLastError());
-    }
+    rc = run_terms<double>(p, evr::kModeG2B, k, nullptr, p->d_R.ptr, nullptr, p->d_Fg.ptr, p->d_bw64.ptr,
+                           p->d_keo64.ptr, p->d_V.ptr);
+    if (rc) return rc;
+    HIP_TRY(launch_reduce<double>(p, p->d_R.ptr, k, dh + nb * c0));
   }
   return EVR_OK;
 }
@@ -1252,9 +1247,9 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
     HIP_TRY(p->d_Gg.alloc((size_t)nq * p->gvec));
     HIP_TRY(p->d_Pg.alloc((size_t)nq * D * p->gvec));
     HIP_TRY(p->d_Fg.alloc((size_t)nq * p->gvec));
-    HIP_TRY(p->d_tmpb.alloc((size_t)p->nb_packed));
   }
-  // the kinetic part leaves the term kernels (they keep the V part)
+  // the term kernels keep the B->G and G->B transforms; the operator itself
+  // is the grid form
   p->hasKeo = false;
   p->hasA = false;
   p->hasF1 = false;
@@ -1448,17 +1443,18 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
       HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
     }
   } else if (!fp32) {
-    double *X = p->d_X.ptr, *R = p->d_R.ptr;
-    hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
-                       p->nb_packed, nvec, X);
-    HIP_TRY(hipGetLastError());
-    rc = run_terms<double>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw64.ptr,
-                           p->d_keo64.ptr, p->d_Veff.ptr);
-    if (rc) return rc;
-    HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
     if (p->keo_grid) {
-      rc = add_keo_grid(p, nvec, dpsi, dh);
+      rc = keo_grid_hpsi(p, nvec, dpsi, dh);
       if (rc) return rc;
+    } else {
+      double *X = p->d_X.ptr, *R = p->d_R.ptr;
+      hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                         p->nb_packed, nvec, X);
+      HIP_TRY(hipGetLastError());
+      rc = run_terms<double>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw64.ptr,
+                             p->d_keo64.ptr, p->d_Veff.ptr);
+      if (rc) return rc;
+      HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
     }
   } else {
     rc = ensure_v32(p);

@@ -699,7 +699,11 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         }
         __syncthreads();
       } else {
-        for (int i2 = tid; i2 < QA * NVW; i2 += NT) cur[i2] = p.Gin[gbase + i2 / NVW];
+        // grid input: column v0 + c at Gin + (v0 + c) * gld (lanes past nvec read 0)
+        for (int i2 = tid; i2 < QA * NVW; i2 += NT) {
+          const int i = i2 / NVW, c = i2 - i * NVW;
+          cur[i2] = v0 + c < nvec ? p.Gin[(int64_t)(v0 + c) * p.gld + gbase + i] : T(0);
+        }
         __syncthreads();
       }
       if (p.mode != kModeB2G) {
@@ -1183,10 +1187,12 @@ __global__ __launch_bounds__(256) void k_gk_combine(int D, int64_t sum_nq, int k
   }
 }
 
+// out = -1/2 (sum_i d_i h_i) / (J s) + V psi_G (V on the unscaled psi_G, OPSG4:1566-1576)
 __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict__ terms,
                                                 const SlotDev *__restrict__ slots, const double *__restrict__ D1,
                                                 const int2 *__restrict__ items, const double *__restrict__ J,
-                                                const double *__restrict__ s, const double *__restrict__ P,
+                                                const double *__restrict__ s, const double *__restrict__ V,
+                                                const double *__restrict__ G, const double *__restrict__ P,
                                                 int64_t sum_nq, int k, double *__restrict__ F)
 {
   const int2 it = items[blockIdx.x];
@@ -1206,7 +1212,7 @@ __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict
                    td.off_nq + p - qi * st);
       st *= nq;
     }
-    F[v * sum_nq + g] = scale * o;
+    F[v * sum_nq + g] = scale * o + V[g] * G[v * sum_nq + g];
   }
 }
 
