@@ -1408,6 +1408,25 @@ static int ensure_v32(evr_plan *p)
   return EVR_OK;
 }
 
+// The assembled operator streams its whole pool once per call whatever the
+// number of vectors; the sum-factorised term kernels cost flops per vector.
+// With EVR_AUTO_PATH the cheaper of the two by this model is taken (rates
+// measured on MI355X: pool streaming about 5.5 TB/s, term kernels about 2
+// TFLOP/s FP64 on small blocks, about 8 us per launch).
+static bool matrix_free_cheaper(const evr_plan *p, int nvec, bool fp32)
+{
+  double flops = 0.0;
+  int launches = 2;  // gather-free pack + reduction
+  for (int c = 0; c < kNumClasses; ++c)
+    if (p->nunits[c]) {
+      flops += p->flops_class[c];
+      ++launches;
+    }
+  const double t_mf = nvec * flops / (fp32 ? 2.5e12 : 2.0e12) + 8e-6 * launches;
+  const double t_asm = (fp32 ? 0.5 : 1.0) * p->asm_bytes64 / 5.5e12 + 8e-6 * 3;
+  return t_mf < t_asm;
+}
+
 int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
 {
   if (!p || !psi || !hpsi) return fail(EVR_ERR_ARG, "NULL argument");
@@ -1424,8 +1443,9 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
   }
   int rc;
   if (p->keo_grid && fp32) return fail(EVR_ERR_ARG, "a plan with G, J, s per grid point is FP64 only");
-  const bool use_asm =
-      !(flags & EVR_MATRIX_FREE) && (fp32 ? p->d_A32.ptr != nullptr : p->d_A64.ptr != nullptr);
+  const bool use_asm = !(flags & EVR_MATRIX_FREE) &&
+                       (fp32 ? p->d_A32.ptr != nullptr : p->d_A64.ptr != nullptr) &&
+                       !((flags & EVR_AUTO_PATH) && matrix_free_cheaper(p, nvec, fp32));
   if (use_asm) {
     if (fp32) {
       float *X = reinterpret_cast<float *>(p->d_X.ptr), *R = reinterpret_cast<float *>(p->d_R.ptr);
@@ -1826,7 +1846,7 @@ int evr_coupled_oppsi(evr_coupled *c, int nvec, const double *psi, double *hpsi,
     dpsi = c->hin.ptr;
     dh = c->hout.ptr;
   }
-  const int bflags = (flags & (EVR_FP32 | EVR_MATRIX_FREE)) | EVR_DEVICE_PTRS;
+  const int bflags = (flags & (EVR_FP32 | EVR_MATRIX_FREE | EVR_AUTO_PATH)) | EVR_DEVICE_PTRS;
   for (int i = 0; i < nb0; ++i) {
     bool first = true;
     for (int j = 0; j < nb0; ++j) {

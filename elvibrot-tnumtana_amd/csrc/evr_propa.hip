@@ -407,10 +407,12 @@ int OpRef::max_nvec() const
   return mv;
 }
 
+// The propagation drivers let each H|psi> take the cheaper of the assembled
+// and sum-factorised paths for its number of columns (EVR_AUTO_PATH).
 int OpRef::apply(int nvec, const double *X, double *Y, int flags) const
 {
-  return plan ? evr_oppsi(plan, nvec, X, Y, flags | EVR_DEVICE_PTRS)
-              : evr_coupled_oppsi(coupled, nvec, X, Y, flags | EVR_DEVICE_PTRS);
+  const int f = flags | EVR_DEVICE_PTRS | ((flags & EVR_MATRIX_FREE) ? 0 : EVR_AUTO_PATH);
+  return plan ? evr_oppsi(plan, nvec, X, Y, f) : evr_coupled_oppsi(coupled, nvec, X, Y, f);
 }
 
 int lanczos_bounds(OpRef p, int steps, double *lo, double *hi)

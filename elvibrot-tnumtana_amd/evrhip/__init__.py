@@ -23,6 +23,7 @@ FP64 = 0
 FP32 = 1
 DEVICE_PTRS = 2
 MATRIX_FREE = 4
+AUTO_PATH = 8
 ASM_FP64 = 1
 ASM_FP32 = 2
 CHEBY = 1

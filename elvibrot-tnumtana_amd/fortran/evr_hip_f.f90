@@ -21,7 +21,7 @@ module evr_hip_f
 
   integer(c_int), parameter :: EVR_OK = 0, EVR_FP64 = 0, EVR_FP32 = 1, EVR_DEVICE_PTRS = 2
   integer(c_int), parameter :: EVR_PROPA_CHEBY = 1, EVR_PROPA_SIL = 2
-  integer(c_int), parameter :: EVR_MATRIX_FREE = 4, EVR_ASM_FP64 = 1, EVR_ASM_FP32 = 2
+  integer(c_int), parameter :: EVR_MATRIX_FREE = 4, EVR_AUTO_PATH = 8, EVR_ASM_FP64 = 1, EVR_ASM_FP32 = 2
 
   type, bind(C) :: evr_basis_desc
     integer(c_int)     :: D, nlev

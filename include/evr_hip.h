@@ -55,6 +55,11 @@ extern "C" {
 #define EVR_FP32 1          /* contractions in FP32, FP32 accumulation          */
 #define EVR_DEVICE_PTRS 2   /* psi/hpsi are device pointers on the plan stream  */
 #define EVR_MATRIX_FREE 4   /* ignore an assembled operator (sum-factorised path) */
+#define EVR_AUTO_PATH 8     /* with an assembled operator, take whichever of the
+                               assembled and sum-factorised paths a cost model
+                               says is faster for this nvec (the assembled pool
+                               is streamed once per call whatever nvec); the
+                               propagation drivers always pass it              */
 
 /* evr_plan_assemble precisions */
 #define EVR_ASM_FP64 1

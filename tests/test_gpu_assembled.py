@@ -39,6 +39,9 @@ def test_c1_type10_assembled(evr, oracle_mod, c1, nvec):
     assert relerr(plan.TabOpPsi(X), ref) < 1e-12
     assert relerr(plan.TabOpPsi(X, flags=evr.FP32), ref) < 2e-5
     assert relerr(plan.TabOpPsi(X, flags=evr.MATRIX_FREE), ref) < 1e-12
+    # whichever path the cost model picks, the operator is the same
+    assert relerr(plan.TabOpPsi(X, flags=evr.AUTO_PATH), ref) < 1e-12
+    assert relerr(plan.TabOpPsi(X, flags=evr.AUTO_PATH | evr.FP32), ref) < 2e-5
 
 
 def test_hermitian_assembled_matches_matrix_free(evr, c1):
