@@ -52,6 +52,16 @@ __host__ __device__ __forceinline__ int64_t asm_index(int row, int col, int ks)
   return ((int64_t)(row >> 4) * ks + (col >> 2)) * 64 + (col & 3) * 16 + (row & 15);
 }
 
+// FP32 pool: the same fragments with two consecutive K-steps interleaved per
+// lane (K-step pairs in order), so a lane reads the A operands of two MFMAs
+// with one 8-byte load; same size and tile offsets as the FP64 pool.
+__host__ __device__ __forceinline__ int64_t asm_index32(int row, int col, int ks)
+{
+  const int kstep = col >> 2;
+  return (((int64_t)(row >> 4) * (ks >> 1) + (kstep >> 1)) * 64 + (col & 3) * 16 + (row & 15)) * 2 +
+         (kstep & 1);
+}
+
 // K-steps (4 columns each) stored per 16-row tile of an nb_t x nb_t matrix:
 // rounded up to a multiple of 4 (zero columns), so the apply kernel reads
 // whole groups of four fragments without bounds tests.
