@@ -34,7 +34,7 @@ struct TermDev {
   int na;
   int amode[kMaxDev];
   // Assembled operator: start of the term's tiled nb_t x nb_t matrix (MFMA
-  // A-fragment order, see asm_index) in the assembled pools.
+  // A-fragment order, see asm_index2) in the assembled pools.
   int64_t off_asm;
   // Coefficient set of the term (index into the per-set KEO coefficients:
   // keoc + cix * keoc_stride, coefC + cix * D * D, coefF1 + cix * D).  All
@@ -44,18 +44,13 @@ struct TermDev {
   int cix;
 };
 
-// Position of element (row, col) of a term's assembled matrix: fragments of
-// 16 rows x 4 columns, each 64 values in the MFMA A-operand lane order
-// (lane = 16 * (col % 4) + row % 16), fragments ordered [row / 16][col / 4].
-__host__ __device__ __forceinline__ int64_t asm_index(int row, int col, int ks)
-{
-  return ((int64_t)(row >> 4) * ks + (col >> 2)) * 64 + (col & 3) * 16 + (row & 15);
-}
-
-// FP32 pool: the same fragments with two consecutive K-steps interleaved per
-// lane (K-step pairs in order), so a lane reads the A operands of two MFMAs
-// with one 8-byte load; same size and tile offsets as the FP64 pool.
-__host__ __device__ __forceinline__ int64_t asm_index32(int row, int col, int ks)
+// Position of element (row, col) of a term's assembled matrix (both pools):
+// fragments of 16 rows x 4 columns (one K-step), each 64 values in the MFMA
+// A-operand lane order (lane = 16 * (col % 4) + row % 16), with two
+// consecutive K-steps interleaved per lane, K-step pairs ordered
+// [row / 16][col / 8]: a lane reads the A operands of two MFMAs with one
+// 16-byte (FP64) or 8-byte (FP32) load.
+__host__ __device__ __forceinline__ int64_t asm_index2(int row, int col, int ks)
 {
   const int kstep = col >> 2;
   return (((int64_t)(row >> 4) * (ks >> 1) + (kstep >> 1)) * 64 + (col & 3) * 16 + (row & 15)) * 2 +
