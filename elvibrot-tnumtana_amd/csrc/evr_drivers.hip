@@ -69,19 +69,6 @@ struct Blas {
 
 unsigned nblocks(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
 
-__global__ void k_symmetrize(int m, int lda, double *S)
-{
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < (int64_t)m * m;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(g % m), j = (int)(g / m);
-    if (i < j) {
-      const double v = 0.5 * (S[i + (int64_t)lda * j] + S[j + (int64_t)lda * i]);
-      S[i + (int64_t)lda * j] = v;
-      S[j + (int64_t)lda * i] = v;
-    }
-  }
-}
-
 // R[:, c] = XH[:, sel[c]] - theta[sel[c]] XV[:, sel[c]]
 __global__ void k_residual(int64_t nb, int nc, const int *sel, const double *theta, const double *XH,
                            const double *XV, double *R)
@@ -95,20 +82,63 @@ __global__ void k_residual(int64_t nb, int nc, const int *sel, const double *the
   }
 }
 
-// squared column norms, one block per column, fixed reduction order
-__global__ __launch_bounds__(256) void k_colnorm2(int64_t nb, const double *A, double *out)
+// squared column norms in two passes of fixed order: blocks (segment s,
+// column c) write partial sums part[c * kNormSeg + s], then one thread per
+// column adds its segments in order (a long column spread over many blocks)
+constexpr int kNormSeg = 64;
+__global__ __launch_bounds__(256) void k_colnorm2_part(int64_t nb, const double *A, double *part)
 {
   __shared__ double red[256];
-  const double *a = A + nb * blockIdx.x;
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < nb; i += 256) s = fma(a[i], a[i], s);
-  red[threadIdx.x] = s;
+  const int s = blockIdx.x, c = blockIdx.y;
+  const int64_t len = (nb + kNormSeg - 1) / kNormSeg;
+  const int64_t i0 = s * len, i1 = i0 + len < nb ? i0 + len : nb;
+  const double *a = A + nb * c;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc = fma(a[i], a[i], acc);
+  red[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) part[(int64_t)c * kNormSeg + s] = red[0];
+}
+
+__global__ void k_colnorm2_fin(int nc, const double *part, double *out)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double acc = 0.0;
+  for (int s = 0; s < kNormSeg; ++s) acc += part[(int64_t)c * kNormSeg + s];
+  out[c] = acc;
+}
+
+// C[i + ldc j] = sum over b (in order) of P[b m n + i + m j]: the partial
+// products of a split inner-product gemm
+__global__ void k_sum_parts(int m, int n, int nbat, const double *P, double *C, int ldc)
+{
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < (int64_t)m * n;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int b = 0; b < nbat; ++b) acc += P[(int64_t)b * m * n + g];
+    const int i = (int)(g % m), j = (int)(g / m);
+    C[i + (int64_t)ldc * j] = acc;
+  }
+}
+
+// Rayleigh-Ritz matrix, columns m0..m-1 (and the matching rows) from
+// C1 = V^T HV[:, m0:m] (m x nn) and C2 = V[:, m0:m]^T HV (nn x m):
+// S(i, j) = S(j, i) = (C1(i, j - m0) + C2(j - m0, i)) / 2, the symmetric part
+__global__ void k_place_sym(int m, int m0, const double *C1, const double *C2, double *S, int lds)
+{
+  const int nn = m - m0;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < (int64_t)m * nn;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(g % m), jj = (int)(g / m), j = m0 + jj;
+    const double v = 0.5 * (C1[i + (int64_t)m * jj] + C2[jj + (int64_t)nn * i]);
+    S[i + (int64_t)lds * j] = v;
+    S[j + (int64_t)lds * i] = v;
+  }
 }
 
 // Davidson correction t = r / (theta - diag), guarded near zero
@@ -202,12 +232,52 @@ struct Dav {
     return EVR_OK;
   }
 
-  // C (m x n) = A^T B with A nb x m, B nb x n
+  // C (m x n) = A^T B with A nb x m, B nb x n.  The result is small and the
+  // reduction long (nb rows), so the rows are split into nbat chunks
+  // (strided-batched gemm, enough workgroups for the whole GPU) and the
+  // partial products summed in a fixed order.
+  DBuf part;
   int gemm_tn(int m, int n, const double *A, const double *B, double *C, int ldc)
   {
     const double one = 1.0, zero = 0.0;
-    BTRY(rocblas_dgemm(bh, rocblas_operation_transpose, rocblas_operation_none, m, n, (int)nb, &one, A,
-                       (int)nb, B, (int)nb, &zero, C, ldc));
+    const int64_t tiles = (int64_t)((m + 127) / 128) * ((n + 127) / 128);
+    int nbat = (int)std::min<int64_t>(64, std::max<int64_t>(1, 1024 / tiles));
+    while (nbat > 1 && nb / nbat < 4096) nbat /= 2;
+    if (nbat == 1) {
+      BTRY(rocblas_dgemm(bh, rocblas_operation_transpose, rocblas_operation_none, m, n, (int)nb, &one, A,
+                         (int)nb, B, (int)nb, &zero, C, ldc));
+      return EVR_OK;
+    }
+    const size_t need = (size_t)nbat * m * n;
+    if (part.n < need) {
+      if (part.p) HTRY(hipFree(part.p));
+      part.p = nullptr;
+      HTRY(part.alloc(need));
+    }
+    const int64_t chunk = nb / nbat, rem = nb - chunk * nbat;
+    BTRY(rocblas_dgemm_strided_batched(bh, rocblas_operation_transpose, rocblas_operation_none, m, n,
+                                       (int)chunk, &one, A, (int)nb, chunk, B, (int)nb, chunk, &zero,
+                                       part.p, m, (int64_t)m * n, nbat));
+    if (rem)  // the last rows, into the first partial product
+      BTRY(rocblas_dgemm(bh, rocblas_operation_transpose, rocblas_operation_none, m, n, (int)rem, &one,
+                         A + chunk * nbat, (int)nb, B + chunk * nbat, (int)nb, &one, part.p, m));
+    hipLaunchKernelGGL(k_sum_parts, dim3(nblocks((int64_t)m * n)), dim3(256), 0, st, m, n, nbat, part.p, C,
+                       ldc);
+    HTRY(hipGetLastError());
+    return EVR_OK;
+  }
+  // out[c] = |A[:, c]|^2 for nc columns of nb rows
+  DBuf npart;
+  int colnorm2(const double *A, int nc, double *out)
+  {
+    if (npart.n < (size_t)nc * kNormSeg) {
+      if (npart.p) HTRY(hipFree(npart.p));
+      npart.p = nullptr;
+      HTRY(npart.alloc((size_t)nc * kNormSeg));
+    }
+    hipLaunchKernelGGL(k_colnorm2_part, dim3(kNormSeg, nc), dim3(256), 0, st, nb, A, npart.p);
+    hipLaunchKernelGGL(k_colnorm2_fin, dim3((nc + 63) / 64), dim3(64), 0, st, nc, npart.p, out);
+    HTRY(hipGetLastError());
     return EVR_OK;
   }
   // C (nb x n) = alpha A (nb x m) B (m x n) + beta C
@@ -291,7 +361,7 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
 
   const int64_t nb = d.nb;
   const int kmax = std::min(mmax, nev + blk);
-  DBuf V, HV, XV, XH, T, HT, S, theta, E, tmpC, tmpT, nrm, dsel, dshift, diag, didx;
+  DBuf V, HV, XV, XH, T, HT, S, Sk, C2, theta, E, tmpC, tmpT, nrm, dsel, dshift, diag, didx;
   int *dinfo = nullptr;
   HTRY(V.alloc(nb * mmax));
   HTRY(HV.alloc(nb * mmax));
@@ -301,6 +371,8 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
   HTRY(HT.alloc(nb * blk));
   HTRY(tmpT.alloc(nb * std::max(blk, 1)));
   HTRY(S.alloc((size_t)mmax * mmax));
+  HTRY(Sk.alloc((size_t)mmax * mmax));
+  HTRY(C2.alloc((size_t)mmax * blk));
   HTRY(theta.alloc(mmax));
   HTRY(E.alloc(mmax));
   HTRY(tmpC.alloc((size_t)mmax * std::max(blk, kmax)));
@@ -361,6 +433,7 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
   bool refined = flags == EVR_FP64;
   RTRY(d.apply(V.p, HV.p, m, flags));
 
+  int m_s = 0;  // columns of V whose rows and columns are in Sk
   std::vector<double> th(mmax), th_old(nev, 1e300), rn(nev, 1e300);
   std::vector<int> conv(nev, 0);
   int it = 0, nconv = 0;
@@ -371,10 +444,25 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
   double best_res = 1e300;
   int stall = 0;
   for (it = 1; it <= max_it; ++it) {
-    // Rayleigh-Ritz in FP64
-    RTRY(d.gemm_tn(m, m, V.p, HV.p, S.p, m));
-    hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)m * m)), dim3(256), 0, d.st, m, m, S.p);
+    // Rayleigh-Ritz in FP64.  Sk keeps the symmetric part of V^T H V over
+    // the steps: only the columns appended since the last step (and their
+    // rows) are computed, all of them after a collapse or the FP64 rebuild
+    // of H V; the eigensolver works on a copy.
+    if (m_s == 0) {
+      RTRY(d.gemm_tn(m, m, V.p, HV.p, S.p, m));
+      hipLaunchKernelGGL(k_place_sym, dim3(nblocks((int64_t)m * m)), dim3(256), 0, d.st, m, 0, S.p, S.p,
+                         Sk.p, mmax);
+    } else if (m > m_s) {
+      const int nn = m - m_s;
+      RTRY(d.gemm_tn(m, nn, V.p, HV.p + nb * m_s, tmpC.p, m));
+      RTRY(d.gemm_tn(nn, m, V.p + nb * m_s, HV.p, C2.p, nn));
+      hipLaunchKernelGGL(k_place_sym, dim3(nblocks((int64_t)m * nn)), dim3(256), 0, d.st, m, m_s, tmpC.p,
+                         C2.p, Sk.p, mmax);
+    }
     HTRY(hipGetLastError());
+    m_s = m;
+    HTRY(hipMemcpy2DAsync(S.p, sizeof(double) * m, Sk.p, sizeof(double) * mmax, sizeof(double) * m, m,
+                          hipMemcpyDeviceToDevice, d.st));
     BTRY(rocsolver_dsyevd(d.bh, rocblas_evect_original, rocblas_fill_upper, m, S.p, m, theta.p, E.p,
                           dinfo));
     HTRY(hipMemcpyAsync(th.data(), theta.p, sizeof(double) * m, hipMemcpyDeviceToHost, d.st));
@@ -395,8 +483,8 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
       HTRY(evr::upload_now(dsel.p, sel.data(), sizeof(int) * nc, d.st));
       hipLaunchKernelGGL(k_residual, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc,
                          (const int *)dsel.p, theta.p, XH.p, XV.p, T.p);
-      hipLaunchKernelGGL(k_colnorm2, dim3(nc), dim3(256), 0, d.st, nb, T.p, nrm.p);
       HTRY(hipGetLastError());
+      RTRY(d.colnorm2(T.p, nc, nrm.p));
       std::vector<double> r2(nc);
       HTRY(hipMemcpyAsync(r2.data(), nrm.p, sizeof(double) * nc, hipMemcpyDeviceToHost, d.st));
       HTRY(hipStreamSynchronize(d.st));
@@ -430,6 +518,7 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
       flags = EVR_FP64;
       refined = true;
       RTRY(d.apply(V.p, HV.p, m, flags));
+      m_s = 0;
       for (int j = 0; j < nev; ++j) th_old[j] = 1e300;
       continue;
     }
@@ -453,6 +542,7 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
       HTRY(hipMemcpyAsync(V.p, XV.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
       HTRY(hipMemcpyAsync(HV.p, XH.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
       m = keep;
+      m_s = 0;
     }
     HTRY(evr::upload_now(dsel.p, sel.data(), sizeof(int) * nc, d.st));
     hipLaunchKernelGGL(k_residual, dim3(nblocks(nb * nc)), dim3(256), 0, d.st, nb, nc,
