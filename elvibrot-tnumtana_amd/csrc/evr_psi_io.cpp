@@ -24,6 +24,7 @@
 // (functions absent here are dropped, functions absent there stay zero).
 #include <algorithm>
 #include <cctype>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -213,6 +214,9 @@ int evr_psi_save(const evr_basis_desc *bd, int nb0, int nwp, int cplx, const dou
     if (!f) return fail_io(EVR_ERR_IO, std::string("write failed: ") + path);
     return EVR_OK;
   }
+  // default 4-byte integers and record markers, as a reference build writes them
+  if (ntot > INT32_MAX || 8.0 * ntot * ncol > 2147483647.0)
+    return fail_io(EVR_ERR_SIZE, "wavepacket too large for an unformatted file with 4-byte markers");
   std::ofstream f(path, std::ios::binary);
   if (!f) return fail_io(EVR_ERR_IO, std::string("cannot open ") + path);
   const char X = 'X';
