@@ -50,8 +50,10 @@ struct DBuf {
   size_t n = 0;
   hipError_t alloc(size_t k)
   {
-    n = k;
-    return k ? hipMalloc((void **)&p, sizeof(double) * k) : hipSuccess;
+    const hipError_t e = k ? hipMalloc((void **)&p, sizeof(double) * k) : hipSuccess;
+    n = e == hipSuccess ? k : 0;
+    if (e != hipSuccess) p = nullptr;
+    return e;
   }
   ~DBuf()
   {
