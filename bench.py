@@ -378,6 +378,52 @@ def keo_grid_run(args):
     print(json.dumps(line), flush=True)
 
 
+def transforms_run(args):
+    """Basis -> grid and grid -> basis of one vector on the D=9 SG4 L=8 grids
+    (sub_PsiBasisRep_TO_GridRep / sub_PsiGridRep_TO_BasisRep): the term
+    kernels' B->G and weighted G->B modes, timed by the plan's HIP events
+    (the host copies of the 6.4 GB grid vector are outside the kernel times)."""
+    import numpy as np
+    import evrhip
+    from evrhip import hamiltonian as H
+    D, L = args.D, args.L
+    ham = H.smolyak_config(config_index(D, L), D)
+    b = evrhip.Basis.sg4(D, L)
+    plan = evrhip.Plan(b, type=10, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=1)
+    plan.gen_potential_poly(ham)
+    x = H.test_vectors(b.nb_packed, 1, 3)[:, 0]
+    g = plan.BasisRep_TO_GridRep(x)          # warm-up (allocates the grid buffer)
+    plan.GridRep_TO_BasisRep(g)
+    plan.set_profiling(True)
+    ms = lambda: sum(s[1] for s in plan.kernel_stats() if s[0].startswith("k_term"))
+    m0 = ms()
+    g = plan.BasisRep_TO_GridRep(x)
+    m1 = ms()
+    y = plan.GridRep_TO_BasisRep(g)
+    m2 = ms()
+    # algorithmic work: B->G reads the term's nb_t values and writes nq_t grid
+    # values (G->B the reverse); flops of the mode-by-mode contractions
+    d = b.ptr.contents
+    nbs = np.ctypeslib.as_array(d.nb, shape=(D * b.nlev,))
+    nqs = np.ctypeslib.as_array(d.nq, shape=(D * b.nlev,))
+    lev = b.term_levels()
+    slot = lev + b.nlev * np.arange(D)[None, :]
+    tnb, tnq = nbs[slot].astype(float), nqs[slot].astype(float)
+    fl = 0.0
+    for dd in range(D):
+        fl += (2.0 * np.prod(tnq[:, :dd], axis=1) * tnq[:, dd] * tnb[:, dd] * np.prod(tnb[:, dd + 1:], axis=1)).sum()
+    nbytes = 8.0 * (b.sum_nb + b.sum_nq)
+    line = {"workload": "SG4 D=%d L=%d basis <-> grid, one vector (%d packed functions, %d grid points)"
+                        % (D, L, b.nb_packed, b.sum_nq)}
+    for tag, t in (("basis_to_grid", m1 - m0), ("grid_to_basis", m2 - m1)):
+        line[tag] = {"kernel_ms": round(t, 3), "gbytes": round(nbytes / 1e9, 3),
+                     "achieved_gbs": round(nbytes / t / 1e6, 1),
+                     "frac_of_hbm_peak": round(nbytes / t / 1e6 / PEAK_HBM, 3),
+                     "gflop": round(fl / 1e9, 2), "tflops": round(fl / t / 1e9, 2)}
+    line["roundtrip_max_rel"] = float(np.abs(y - plan.GridRep_TO_BasisRep(g)).max() / np.abs(y).max())
+    print(json.dumps(line), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this workload (profiles/traffic_sg4_d9l8.md), or None."""
@@ -420,6 +466,8 @@ def main():
                     help="the reference's 24D pyrazine propagation test (two coupled states)")
     ap.add_argument("--keo-grid", action="store_true",
                     help="type 10 with G, J, s per grid point on D=6 SG4 L=5")
+    ap.add_argument("--transforms", action="store_true",
+                    help="basis <-> grid of one vector on the --D/--L grids, kernel times")
     ap.add_argument("--nev", type=int, default=200)
     ap.add_argument("--conv-resi", type=float, default=1e-3)
     ap.add_argument("--refine-resi", type=float, default=1e-2)
@@ -446,6 +494,9 @@ def main():
         return
     if args.keo_grid:
         keo_grid_run(args)
+        return
+    if args.transforms:
+        transforms_run(args)
         return
 
     import numpy as np
