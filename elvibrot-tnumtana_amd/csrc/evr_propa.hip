@@ -809,25 +809,34 @@ static int replay_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double poly_to
 using namespace evr::propa;
 using evr::set_error;
 
-static int propagate_impl(OpRef H, const evr_propa_opts *o, int nwp, double *psi, double *autocorr,
+static int propagate_impl(OpRef H, OpRef mu, const evr_propa_opts *o, int nwp, double *psi, double *autocorr,
                           evr_propa_info *info);
 
 extern "C" int evr_propagate(evr_plan *H, const evr_propa_opts *o, int nwp, double *psi,
                              double *autocorr, evr_propa_info *info)
 {
   if (!H) return set_error(EVR_ERR_ARG, "NULL plan");
-  return propagate_impl(OpRef(H), o, nwp, psi, autocorr, info);
+  return propagate_impl(OpRef(H), OpRef(o ? o->mu : nullptr), o, nwp, psi, autocorr, info);
 }
 
 extern "C" int evr_coupled_propagate(evr_coupled *C, const evr_propa_opts *o, int nwp, double *psi,
                                      double *autocorr, evr_propa_info *info)
 {
   if (!C) return set_error(EVR_ERR_ARG, "NULL coupled operator");
-  if (o && (o->field || o->mu)) return set_error(EVR_ERR_ARG, "no field coupling for coupled states");
-  return propagate_impl(OpRef(C), o, nwp, psi, autocorr, info);
+  if (o && (o->field || o->mu))
+    return set_error(EVR_ERR_ARG, "field coupling on coupled states: evr_coupled_propagate_field");
+  return propagate_impl(OpRef(C), OpRef(), o, nwp, psi, autocorr, info);
 }
 
-static int propagate_impl(OpRef H, const evr_propa_opts *o, int nwp, double *psi, double *autocorr,
+extern "C" int evr_coupled_propagate_field(evr_coupled *C, evr_coupled *mu, const evr_propa_opts *o, int nwp,
+                                           double *psi, double *autocorr, evr_propa_info *info)
+{
+  if (!C || !mu) return set_error(EVR_ERR_ARG, "NULL coupled operator");
+  if (o && o->mu) return set_error(EVR_ERR_ARG, "the coupling is the coupled operator mu (opts->mu NULL)");
+  return propagate_impl(OpRef(C), OpRef(mu), o, nwp, psi, autocorr, info);
+}
+
+static int propagate_impl(OpRef H, OpRef mu, const evr_propa_opts *o, int nwp, double *psi, double *autocorr,
                           evr_propa_info *info)
 {
   if (!H || !o || !psi || nwp < 1) return set_error(EVR_ERR_ARG, "bad argument");
@@ -838,17 +847,17 @@ static int propagate_impl(OpRef H, const evr_propa_opts *o, int nwp, double *psi
   if (o->method == EVR_PROPA_SIL && (o->npoly < 2 || o->npoly > 63))
     return set_error(EVR_ERR_ARG, "SIL needs 2 <= npoly <= 63");
   if (o->nsteps < 0 || !(o->dt > 0.0)) return set_error(EVR_ERR_ARG, "bad dt or nsteps");
-  if ((o->field != nullptr) != (o->mu != nullptr))
+  if ((o->field != nullptr) != (bool)mu)
     return set_error(EVR_ERR_ARG, "field values and the coupling plan go together");
   const auto t_start = std::chrono::steady_clock::now();
   Batch P;
-  RTRY(P.init(H, o->mu, nwp, o->precision == EVR_FP32 ? EVR_FP32 : EVR_FP64));
+  RTRY(P.init(H, mu, nwp, o->precision == EVR_FP32 ? EVR_FP32 : EVR_FP64));
   const int64_t nb = P.nb, nblk = nb * 2 * nwp;
   const hipMemcpyKind h2d = o->device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   DBuf PSI, HW, MW, PSI0;
   HTRY(PSI.alloc(nblk));
   HTRY(HW.alloc(nblk));
-  if (o->mu) HTRY(MW.alloc(nblk));
+  if (mu) HTRY(MW.alloc(nblk));
   HTRY(hipMemcpyAsync(PSI.p, psi, sizeof(double) * nblk, h2d, P.st));
   if (autocorr) {
     HTRY(PSI0.alloc(nblk));
@@ -869,9 +878,9 @@ static int propagate_impl(OpRef H, const evr_propa_opts *o, int nwp, double *psi
       hmin -= pad;
       hmax += pad;
     }
-    if (o->mu && fmax > 0.0) {
+    if (mu && fmax > 0.0) {
       double mlo, mhi;
-      RTRY(lanczos_bounds(o->mu, 60, &mlo, &mhi));
+      RTRY(lanczos_bounds(mu, 60, &mlo, &mhi));
       const double mb = 1.05 * std::max(std::fabs(mlo), std::fabs(mhi));
       hmin -= fmax * mb;
       hmax += fmax * mb;

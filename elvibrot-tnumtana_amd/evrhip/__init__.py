@@ -429,17 +429,27 @@ class Coupled:
         return out[:, 0] if one else out
 
     def propagate(self, psi, dt, nsteps, method=CHEBY, npoly=200, poly_tol=1e-12, Hmin=0.0, Hmax=0.0,
-                  psi0=None, autocorr=False, graph=False):
-        """march_cheby / march_SIL on the coupled states (see Plan.propagate)."""
+                  psi0=None, autocorr=False, graph=False, field=None, mu=None):
+        """march_cheby / march_SIL on the coupled states (see Plan.propagate);
+        field (nsteps, nwp) with mu a Coupled dipole operator on the same
+        states: step n uses H - field[n] mu."""
         r = _to_pairs(psi)
         nwp = r.shape[1] // 2
         p0 = None if psi0 is None else _to_pairs(psi0)
-        o = PropaOpts(method, dt, nsteps, npoly, poly_tol, Hmin, Hmax, None, None,
+        fl = None
+        if field is not None:
+            fl = np.ascontiguousarray(np.asarray(field, dtype=np.float64).reshape(nsteps, nwp))
+        o = PropaOpts(method, dt, nsteps, npoly, poly_tol, Hmin, Hmax, _d(fl), None,
                       None if p0 is None else p0.ctypes.data_as(ctypes.c_void_p), FP64, 0, 1 if graph else 0)
         ac = np.zeros((nsteps, nwp, 2)) if autocorr else None
         info = PropaInfo()
-        check(load().evr_coupled_propagate(self.h, ctypes.byref(o), nwp, r.ctypes.data_as(ctypes.c_void_p),
-                                           _d(ac), ctypes.byref(info)))
+        if mu is None:
+            check(load().evr_coupled_propagate(self.h, ctypes.byref(o), nwp, r.ctypes.data_as(ctypes.c_void_p),
+                                               _d(ac), ctypes.byref(info)))
+        else:
+            check(load().evr_coupled_propagate_field(self.h, mu.h, ctypes.byref(o), nwp,
+                                                     r.ctypes.data_as(ctypes.c_void_p), _d(ac),
+                                                     ctypes.byref(info)))
         out = _from_pairs(r)
         if np.ndim(psi) == 1:
             out = out[:, 0]

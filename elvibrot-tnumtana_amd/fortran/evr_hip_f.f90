@@ -351,6 +351,15 @@ module evr_hip_f
       type(evr_propa_info), intent(out) :: info
       integer(c_int) :: evr_coupled_propagate
     end function
+    function evr_coupled_propagate_field(c, mu, opts, nwp, psi, autocorr, info) &
+        bind(C, name="evr_coupled_propagate_field")
+      import :: c_int, c_ptr, evr_propa_opts, evr_propa_info
+      type(c_ptr), value :: c, mu, psi, autocorr
+      type(evr_propa_opts), intent(in) :: opts
+      integer(c_int), value :: nwp
+      type(evr_propa_info), intent(out) :: info
+      integer(c_int) :: evr_coupled_propagate_field
+    end function
     function evr_plan_bounds(p, steps, Hmin, Hmax) bind(C, name="evr_plan_bounds")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: p

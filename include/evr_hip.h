@@ -400,6 +400,13 @@ int evr_propagate(evr_plan *H, const evr_propa_opts *opts, int nwp, double *psi,
  * columns hold nb0 x nb_packed rows; no field coupling (field and mu NULL). */
 int evr_coupled_propagate(evr_coupled *H, const evr_propa_opts *opts, int nwp, double *psi,
                           double *autocorr, evr_propa_info *info);
+/* Coupled states driven by a field (the reference's field term with the
+ * dipole operator of the electronic states, MARCH:1434-1836): step n uses
+ * H - field[n*nwp + w] mu, mu a coupled operator on the same states (its
+ * blocks mu_ij typically type-0 plans of the transition and permanent
+ * dipoles); opts->field as for evr_propagate, opts->mu NULL. */
+int evr_coupled_propagate_field(evr_coupled *H, evr_coupled *mu, const evr_propa_opts *opts, int nwp,
+                                double *psi, double *autocorr, evr_propa_info *info);
 
 /* march_cheby on a multi-system operator (evr_multi_create), one complex
  * wavepacket per system (psi: sum_k nb_k x 2, Re and Im columns), every

@@ -87,3 +87,40 @@ def test_pyrazine_L2_vs_exact(evr, oracle_mod):
     err = np.abs(got - ref).max()
     print("pyrazine L=2 GPU vs exact %.3e" % err)
     assert err < 1e-9
+
+
+def test_pyrazine_field_driven(evr, oracle_mod):
+    """Coupled states driven by a field through a dipole operator of the
+    states (mu_11 = 0.1 Q_2, mu_12 = mu_21 = 0.3 + 0.05 Q_1, type-0 blocks), 12D
+    model, 30 steps with a piecewise-constant field per wavepacket, two
+    wavepackets: against exp(-i (H - E_n mu) dt) of the oracle's operators."""
+    P = P12
+    b, H = _coupled(evr, P)
+    q = b.grid()
+    m11, m12 = 0.1 * q[:, 1], 0.3 + 0.05 * q[:, 0]
+    p11 = evr.Plan(b, type=0, V=m11, max_nvec=4)
+    p12 = evr.Plan(b, type=0, V=m12, max_nvec=4)
+    Mu = evr.Coupled([[p11, p12], [p12, None]])
+    s, Hd = coupled_dense(oracle_mod, P)
+    n = s.nb_packed
+    I = np.eye(n)
+    M11 = s.oppsi(oracle_mod.Op(0, V=m11), I)
+    M12 = s.oppsi(oracle_mod.Op(0, V=m12), I)
+    Md = np.block([[M11, M12], [M12, np.zeros((n, n))]])
+    nsteps, dt = 30, P.DT
+    E = 0.02 * np.sin(0.3 * np.arange(nsteps)[:, None] + np.arange(2)[None, :])
+    rng = np.random.default_rng(8)
+    p = np.zeros((2 * n, 2), complex)
+    p[n, 0] = 1.0
+    p[:, 1] = rng.standard_normal(2 * n) + 1j * rng.standard_normal(2 * n)
+    p[:, 1] /= np.linalg.norm(p[:, 1])
+    got, _, info = H.propagate(p, dt, nsteps, npoly=P.NPOLY, poly_tol=1e-12, field=E, mu=Mu)
+    ref = p.copy()
+    for k in range(nsteps):
+        for w in range(2):
+            ref[:, w] = sl.expm(-1j * (Hd - E[k, w] * Md) * dt) @ ref[:, w]
+    err = np.abs(got - ref).max()
+    print("field-driven coupled states vs exact %.3e (max order %d)" % (err, info.max_order))
+    assert err < 1e-9
+    with pytest.raises(RuntimeError, match="evr_coupled_propagate_field"):
+        H.propagate(p, dt, 2, field=E[:2])          # a field needs the coupled dipole
