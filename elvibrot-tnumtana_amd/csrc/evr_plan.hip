@@ -326,6 +326,23 @@ hipError_t zero_fill(T *a, int64_t n, hipStream_t s)
   return hipGetLastError();
 }
 
+// Columns c < nc of X (nb x nc, column-major) = unit vectors e_{b0 + c}.
+__global__ void k_unit_cols(int64_t nb, int nc, int64_t b0, double *X)
+{
+  const int64_t n = nb * nc;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k % nb, c = k / nb;
+    X[k] = i == b0 + c ? 1.0 : 0.0;
+  }
+}
+
+// diag[b0 + c] = Y[b0 + c, c] for c < nc
+__global__ void k_take_diag(int64_t nb, int nc, int64_t b0, const double *Y, double *diag)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nc) diag[b0 + c] = Y[b0 + c + (int64_t)c * nb];
+}
+
 // What the assembled apply and the scatter read: one plan's tables, or the
 // concatenated tables of several plans (evr_multi).
 struct AsmOp {
@@ -1521,6 +1538,37 @@ int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG)
 int evr_plan_diagonal(evr_plan *p, double *diag, int flags)
 {
   if (!p || !diag) return fail(EVR_ERR_ARG, "NULL argument");
+  if (p->keo_grid) {
+    // G, J, s per grid point: the per-point factors do not separate over the
+    // modes, so the diagonal is read off H applied to blocks of unit vectors
+    // (nb_packed / max_nvec applications: exact, and costly for large bases)
+    const int64_t nb = p->nb_packed;
+    const int nv = std::max(1, p->max_nvec);
+    DevBuf<double> X, Y, out;
+    hipError_t e = X.alloc((size_t)nb * nv);
+    if (e == hipSuccess) e = Y.alloc((size_t)nb * nv);
+    if (e == hipSuccess && !(flags & EVR_DEVICE_PTRS)) e = out.alloc((size_t)nb);
+    int rc = EVR_OK;
+    if (e != hipSuccess) rc = fail(EVR_ERR_HIP, std::string("diagonal buffers: ") + hipGetErrorString(e));
+    double *dst = (flags & EVR_DEVICE_PTRS) ? diag : out.ptr;
+    for (int64_t b0 = 0; rc == EVR_OK && b0 < nb; b0 += nv) {
+      const int nc = (int)std::min<int64_t>(nv, nb - b0);
+      hipLaunchKernelGGL(k_unit_cols, dim3(blocks_for(nb * nc)), dim3(256), 0, p->stream, nb, nc, b0, X.ptr);
+      rc = evr_oppsi(p, nc, X.ptr, Y.ptr, EVR_DEVICE_PTRS);
+      if (rc == EVR_OK) {
+        hipLaunchKernelGGL(k_take_diag, dim3((nc + 255) / 256), dim3(256), 0, p->stream, nb, nc, b0, Y.ptr, dst);
+        e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(EVR_ERR_HIP, std::string("diagonal: ") + hipGetErrorString(e));
+      }
+    }
+    if (rc == EVR_OK && !(flags & EVR_DEVICE_PTRS)) {
+      e = hipMemcpyAsync(diag, dst, sizeof(double) * nb, hipMemcpyDeviceToHost, p->stream);
+      if (e != hipSuccess) rc = fail(EVR_ERR_HIP, std::string("diagonal: ") + hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(p->stream);
+    X.release(); Y.release(); out.release();
+    return rc;
+  }
   // V part: GridTOBasis of V with W o B^T as the grid->basis matrices
   const double *Gin = p->hasV ? p->d_V.ptr : nullptr;
   if (!Gin) {

@@ -282,8 +282,10 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
 /* Diagonal of H in the packed basis (nb_packed values): V part exact
  * (sum_t w_t W^(x) diag(V) B^(x) diagonal, one grid->basis pass of V with
  * W o B^T), kinetic part from the diagonals of the per-mode factors.  Used as
- * the Davidson preconditioner (NewVec_type 2, DAV:1392-1413).  flags:
- * EVR_DEVICE_PTRS if diag is a device pointer. */
+ * the Davidson preconditioner (NewVec_type 2, DAV:1392-1413).  A plan with
+ * G, J, s per grid point (evr_plan_set_keo_grid) reads its exact diagonal off
+ * H applied to blocks of max_nvec unit vectors (nb_packed / max_nvec
+ * applications).  flags: EVR_DEVICE_PTRS if diag is a device pointer. */
 int evr_plan_diagonal(evr_plan *p, double *diag, int flags);
 
 /* BasisTOGrid: sub_PsiBasisRep_TO_GridRep (PSIBG:60) -> RecRvecB_TO_RVecG

@@ -67,3 +67,22 @@ def test_keo_grid_refusals(evr):
     p1 = evr.Plan(b, type=1, V=np.zeros(len(q)), F2=-0.5 * np.eye(D), max_nvec=2)
     with pytest.raises(RuntimeError, match="type-10"):
         p1.set_keo_grid(Gq)
+
+
+def test_keo_grid_diagonal(evr, oracle_mod):
+    """The diagonal of the grid form (H applied to blocks of unit vectors, the
+    last block partial) against the oracle's dense operator."""
+    D, L = 4, 4
+    s = oracle_mod.SG4(D, L)
+    b = evr.Basis.sg4(D, L)
+    q = s.grid()
+    Gq, J, sq = tabulated_keo(q, 31)
+    V = 0.5 * (q ** 2).sum(1) + 0.05 * q[:, 0] ** 3
+    plan = evr.Plan(b, type=10, V=V, G0=np.eye(D), max_nvec=4)
+    plan.set_keo_grid(Gq, J, sq)
+    n = s.nb_packed
+    assert n % 4 != 0
+    Hd = s.oppsi(oracle_mod.Op(10, V=V, Jac=J, s=sq, Gq=Gq), np.eye(n))
+    err = relerr(plan.diagonal(), np.diag(Hd))
+    print("G per point: diagonal rel err %.3e" % err)
+    assert err < 1e-12
