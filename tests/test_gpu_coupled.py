@@ -89,11 +89,13 @@ def test_pyrazine_L2_vs_exact(evr, oracle_mod):
     assert err < 1e-9
 
 
-def test_pyrazine_field_driven(evr, oracle_mod):
+@pytest.mark.parametrize("graph", [False, True])
+def test_pyrazine_field_driven(evr, oracle_mod, graph):
     """Coupled states driven by a field through a dipole operator of the
     states (mu_11 = 0.1 Q_2, mu_12 = mu_21 = 0.3 + 0.05 Q_1, type-0 blocks), 12D
     model, 30 steps with a piecewise-constant field per wavepacket, two
-    wavepackets: against exp(-i (H - E_n mu) dt) of the oracle's operators."""
+    wavepackets, adaptive and with captured steps: against
+    exp(-i (H - E_n mu) dt) of the oracle's operators."""
     P = P12
     b, H = _coupled(evr, P)
     q = b.grid()
@@ -114,13 +116,14 @@ def test_pyrazine_field_driven(evr, oracle_mod):
     p[n, 0] = 1.0
     p[:, 1] = rng.standard_normal(2 * n) + 1j * rng.standard_normal(2 * n)
     p[:, 1] /= np.linalg.norm(p[:, 1])
-    got, _, info = H.propagate(p, dt, nsteps, npoly=P.NPOLY, poly_tol=1e-12, field=E, mu=Mu)
+    got, _, info = H.propagate(p, dt, nsteps, npoly=P.NPOLY, poly_tol=1e-12, field=E, mu=Mu,
+                               graph=graph)
     ref = p.copy()
     for k in range(nsteps):
         for w in range(2):
             ref[:, w] = sl.expm(-1j * (Hd - E[k, w] * Md) * dt) @ ref[:, w]
     err = np.abs(got - ref).max()
-    print("field-driven coupled states vs exact %.3e (max order %d)" % (err, info.max_order))
+    print("field-driven coupled states (graph=%s) vs exact %.3e (max order %d)" % (graph, err, info.max_order))
     assert err < 1e-9
     with pytest.raises(RuntimeError, match="evr_coupled_propagate_field"):
         H.propagate(p, dt, 2, field=E[:2])          # a field needs the coupled dipole
