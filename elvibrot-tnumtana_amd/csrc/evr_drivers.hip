@@ -23,6 +23,9 @@
 
 #include "../../include/evr_hip.h"
 #include "evr_internal.hpp"
+#include "evr_propa.hpp"
+
+using OpRef = evr::propa::OpRef;
 
 namespace {
 
@@ -216,7 +219,7 @@ void jacobi_sym(int n, std::vector<double> A, std::vector<double> &ev, std::vect
 }
 
 struct Dav {
-  evr_plan *plan = nullptr;
+  OpRef op;  // one plan, or coupled electronic states
   hipStream_t st = nullptr;
   rocblas_handle bh = nullptr;
   int64_t nb = 0;
@@ -228,7 +231,8 @@ struct Dav {
   {
     for (int c0 = 0; c0 < nc; c0 += max_nvec) {
       const int k = std::min(max_nvec, nc - c0);
-      RTRY(evr_oppsi(plan, k, X + nb * c0, HX + nb * c0, flags | EVR_DEVICE_PTRS));
+      RTRY(op.plan ? evr_oppsi(op.plan, k, X + nb * c0, HX + nb * c0, flags | EVR_DEVICE_PTRS)
+                   : evr_coupled_oppsi(op.coupled, k, X + nb * c0, HX + nb * c0, flags | EVR_DEVICE_PTRS));
       if (flags & EVR_FP32) hpsi32 += k; else hpsi64 += k;
     }
     return EVR_OK;
@@ -335,17 +339,32 @@ struct Dav {
 
 }  // namespace
 
+static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, double *evecs,
+                         evr_davidson_info *info);
+
 extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *evals, double *evecs,
                             evr_davidson_info *info)
 {
-  if (!plan || !o || !evals) return set_error(EVR_ERR_ARG, "NULL argument");
+  if (!plan) return set_error(EVR_ERR_ARG, "NULL argument");
+  return davidson_impl(OpRef(plan), o, evals, evecs, info);
+}
+
+extern "C" int evr_coupled_davidson(evr_coupled *C, const evr_davidson_opts *o, double *evals, double *evecs,
+                                    evr_davidson_info *info)
+{
+  if (!C) return set_error(EVR_ERR_ARG, "NULL argument");
+  return davidson_impl(OpRef(C), o, evals, evecs, info);
+}
+
+static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, double *evecs,
+                         evr_davidson_info *info)
+{
+  if (!o || !evals) return set_error(EVR_ERR_ARG, "NULL argument");
   const auto t_start = std::chrono::steady_clock::now();
-  int64_t sz[7];
-  RTRY(evr_plan_sizes(plan, sz));
   Dav d;
-  d.plan = plan;
-  d.nb = sz[0];
-  d.st = (hipStream_t)evr_plan_stream(plan);
+  d.op = op;
+  RTRY(op.rows(&d.nb));
+  d.st = op.stream();
   const int nev = o->nev;
   const int blk = o->block > 0 ? o->block : std::min(nev, 16);
   if (nev < 1 || nev > d.nb) return set_error(EVR_ERR_ARG, "nev out of range");
@@ -393,7 +412,8 @@ extern "C" int evr_davidson(evr_plan *plan, const evr_davidson_opts *o, double *
       HTRY(hipMemcpyAsync(diag.p, o->diag, sizeof(double) * nb, hipMemcpyHostToDevice, d.st));
       hdiag.assign(o->diag, o->diag + nb);
     } else {
-      RTRY(evr_plan_diagonal(plan, diag.p, EVR_DEVICE_PTRS));
+      RTRY(op.plan ? evr_plan_diagonal(op.plan, diag.p, EVR_DEVICE_PTRS)
+                   : evr_coupled_diagonal(op.coupled, diag.p, EVR_DEVICE_PTRS));
       hdiag.resize(nb);
       HTRY(hipMemcpyAsync(hdiag.data(), diag.p, sizeof(double) * nb, hipMemcpyDeviceToHost, d.st));
       HTRY(hipStreamSynchronize(d.st));

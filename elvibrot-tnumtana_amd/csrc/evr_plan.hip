@@ -1878,6 +1878,26 @@ int evr_coupled_sizes(const evr_coupled *c, int64_t *nb, int *nb0, int *max_nvec
   return EVR_OK;
 }
 
+int evr_coupled_diagonal(evr_coupled *c, double *diag, int flags)
+{
+  if (!c || !diag) return fail(EVR_ERR_ARG, "NULL argument");
+  const bool dev = (flags & EVR_DEVICE_PTRS) != 0;
+  for (int i = 0; i < c->nb0; ++i) {
+    evr_plan *b = c->blocks[(size_t)i * c->nb0 + i];
+    double *di = diag + (int64_t)i * c->nb;
+    if (b) {
+      const int rc = evr_plan_diagonal(b, di, dev ? EVR_DEVICE_PTRS : 0);
+      if (rc) return rc;
+    } else if (dev) {
+      HIP_TRY(zero_fill(di, c->nb, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    } else {
+      std::fill(di, di + c->nb, 0.0);
+    }
+  }
+  return EVR_OK;
+}
+
 void *evr_coupled_stream(evr_coupled *c) { return c ? (void *)c->stream : nullptr; }
 
 int evr_coupled_oppsi(evr_coupled *c, int nvec, const double *psi, double *hpsi, int flags)

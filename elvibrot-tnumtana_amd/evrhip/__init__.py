@@ -428,6 +428,29 @@ class Coupled:
             out[:, c0:c1] = y
         return out[:, 0] if one else out
 
+    def diagonal(self):
+        """Diagonal of the coupled operator (nb0 * nb_packed,), state-major."""
+        d = np.zeros(self.nb0 * self.nb_packed)
+        check(load().evr_coupled_diagonal(self.h, _d(d), 0))
+        return d
+
+    def davidson(self, nev, block=16, max_sub=0, max_it=200, conv_resi=1e-8, conv_ene=1e-9,
+                 precond=True, diag=None, guess=None, want_vectors=False, verbose=False):
+        """Lowest nev vibronic levels by the block Davidson of Plan.davidson."""
+        mv = ctypes.c_int()
+        check(load().evr_coupled_sizes(self.h, None, None, ctypes.byref(mv)))
+        n = self.nb0 * self.nb_packed
+        dg = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
+        gs = None if guess is None else np.asfortranarray(guess, dtype=np.float64)
+        o = DavidsonOpts(nev, block, max_sub, max_it, conv_resi, conv_ene, 1 if precond else 0,
+                         _d(dg), _d(gs), 0 if gs is None else gs.shape[1], FP64, 1e-3,
+                         min(block, mv.value), int(verbose))
+        ev = np.zeros(nev)
+        vec = np.zeros((n, nev), order="F") if want_vectors else None
+        info = DavidsonInfo()
+        check(load().evr_coupled_davidson(self.h, ctypes.byref(o), _d(ev), _d(vec), ctypes.byref(info)))
+        return ev, vec, info
+
     def propagate(self, psi, dt, nsteps, method=CHEBY, npoly=200, poly_tol=1e-12, Hmin=0.0, Hmax=0.0,
                   psi0=None, autocorr=False, graph=False, field=None, mu=None):
         """march_cheby / march_SIL on the coupled states (see Plan.propagate);

@@ -124,6 +124,9 @@ _SIGS = {
     "evr_coupled_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]),
     "evr_coupled_propagate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PropaOpts), ctypes.c_int,
                                              ctypes.c_void_p, _dp, ctypes.POINTER(PropaInfo)]),
+    "evr_coupled_diagonal": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
+    "evr_coupled_davidson": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DavidsonOpts), _dp, _dp,
+                                            ctypes.POINTER(DavidsonInfo)]),
     "evr_coupled_propagate_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(PropaOpts),
                                                    ctypes.c_int, ctypes.c_void_p, _dp,
                                                    ctypes.POINTER(PropaInfo)]),

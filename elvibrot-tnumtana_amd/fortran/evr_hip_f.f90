@@ -351,6 +351,19 @@ module evr_hip_f
       type(evr_propa_info), intent(out) :: info
       integer(c_int) :: evr_coupled_propagate
     end function
+    function evr_coupled_diagonal(c, diag, flags) bind(C, name="evr_coupled_diagonal")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, diag
+      integer(c_int), value :: flags
+      integer(c_int) :: evr_coupled_diagonal
+    end function
+    function evr_coupled_davidson(c, opts, evals, evecs, info) bind(C, name="evr_coupled_davidson")
+      import :: c_int, c_ptr, evr_davidson_opts, evr_davidson_info
+      type(c_ptr), value :: c, evals, evecs
+      type(evr_davidson_opts), intent(in) :: opts
+      type(evr_davidson_info), intent(out) :: info
+      integer(c_int) :: evr_coupled_davidson
+    end function
     function evr_coupled_propagate_field(c, mu, opts, nwp, psi, autocorr, info) &
         bind(C, name="evr_coupled_propagate_field")
       import :: c_int, c_ptr, evr_propa_opts, evr_propa_info

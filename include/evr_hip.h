@@ -410,6 +410,18 @@ int evr_coupled_propagate(evr_coupled *H, const evr_propa_opts *opts, int nwp, d
 int evr_coupled_propagate_field(evr_coupled *H, evr_coupled *mu, const evr_propa_opts *opts, int nwp,
                                 double *psi, double *autocorr, evr_propa_info *info);
 
+/* Diagonal of a coupled operator (nb0 x nb_packed values, state-major): the
+ * diagonals of the blocks H_ii (evr_plan_diagonal), zero for a missing block.
+ * flags: EVR_DEVICE_PTRS if diag is a device pointer. */
+int evr_coupled_diagonal(evr_coupled *C, double *diag, int flags);
+
+/* Lowest levels of coupled electronic states by the block Davidson of
+ * evr_davidson (sub_propagation_Davidson on an operator with nb_bie > 1,
+ * DAV:516-606); vectors of nb0 x nb_packed rows, the preconditioner diagonal
+ * from evr_coupled_diagonal unless opts->diag is given. */
+int evr_coupled_davidson(evr_coupled *C, const evr_davidson_opts *opts, double *evals, double *evecs,
+                         evr_davidson_info *info);
+
 /* march_cheby on a multi-system operator (evr_multi_create), one complex
  * wavepacket per system (psi: sum_k nb_k x 2, Re and Im columns), every
  * system with its own interval [Hmin[k], Hmax[k]] (its E0, Esc and Bessel

@@ -127,3 +127,20 @@ def test_pyrazine_field_driven(evr, oracle_mod, graph):
     assert err < 1e-9
     with pytest.raises(RuntimeError, match="evr_coupled_propagate_field"):
         H.propagate(p, dt, 2, field=E[:2])          # a field needs the coupled dipole
+
+
+def test_pyrazine_davidson_levels(evr, oracle_mod):
+    """Lowest 8 vibronic levels of the 12D pyrazine model on the L=2 basis (724
+    coupled functions) by the block Davidson with the coupled diagonal as
+    preconditioner, against the dense eigenvalues of the oracle's operator."""
+    b, H = _coupled(evr, P12, 2, max_nvec=8)
+    s, Hd = coupled_dense(oracle_mod, P12, 2)
+    n = s.nb_packed
+    assert relerr(H.diagonal(), np.diag(Hd)) < 1e-12
+    ref = np.linalg.eigvalsh(0.5 * (Hd + Hd.T))[:8]
+    ev, vec, info = H.davidson(8, block=8, conv_resi=1e-9, want_vectors=True)
+    err = np.abs(ev - ref).max() * 219474.6313705      # hartree -> cm-1
+    print("pyrazine 12D L=2 Davidson: max |dE| %.3e cm-1 in %d iterations" % (err, info.iterations))
+    assert err < 1e-6
+    r = Hd @ vec - vec * ev[None, :]
+    assert np.abs(r).max() < 1e-7 and vec.shape == (2 * n, 8)
