@@ -353,6 +353,8 @@ extern "C" int evr_coupled_davidson(evr_coupled *C, const evr_davidson_opts *o, 
                                     evr_davidson_info *info)
 {
   if (!C) return set_error(EVR_ERR_ARG, "NULL argument");
+  if (o && o->precision == EVR_FP32)
+    return set_error(EVR_ERR_ARG, "coupled-state Davidson is FP64 only (precision EVR_FP64)");
   return davidson_impl(OpRef(C), o, evals, evecs, info);
 }
 

@@ -418,7 +418,8 @@ int evr_coupled_diagonal(evr_coupled *C, double *diag, int flags);
 /* Lowest levels of coupled electronic states by the block Davidson of
  * evr_davidson (sub_propagation_Davidson on an operator with nb_bie > 1,
  * DAV:516-606); vectors of nb0 x nb_packed rows, the preconditioner diagonal
- * from evr_coupled_diagonal unless opts->diag is given. */
+ * from evr_coupled_diagonal unless opts->diag is given; FP64 only
+ * (opts->precision EVR_FP32 is refused). */
 int evr_coupled_davidson(evr_coupled *C, const evr_davidson_opts *opts, double *evals, double *evecs,
                          evr_davidson_info *info);
 
