@@ -155,6 +155,38 @@ extern "C" int evr_control_field(const evr_control_opts *o, int nf, const double
   return EVR_OK;
 }
 
+extern "C" int evr_control_field_sub(const evr_control_opts *o, int nf, const double *amp, int method, int nsub,
+                                     double *field_sub)
+{
+  if (!o || !amp || !field_sub || nf < 1 || o->nsteps < 1 || o->nk < 1 || !(o->T > 0.0) || nsub < 1)
+    return set_error(EVR_ERR_ARG, "bad argument");
+  if (method != EVR_PROPA_TAYLOR && method != EVR_PROPA_RK4)
+    return set_error(EVR_ERR_ARG, "method must be EVR_PROPA_TAYLOR or EVR_PROPA_RK4");
+  if (method == EVR_PROPA_RK4 && nsub != 3) return set_error(EVR_ERR_ARG, "RK4 takes nsub = 3");
+  // sin^2(x) sin(k x) = sin(k x)/2 - sin((k+2) x)/4 - sin((k-2) x)/4 with
+  // x = w t, w = pi/T, and d^j/dt^j sin(m w t) = (m w)^j sin(m w t + j pi/2)
+  const double w = M_PI / o->T, dt = o->T / o->nsteps;
+  for (int n = 0; n < o->nsteps; ++n)
+    for (int j = 0; j < nsub; ++j) {
+      const bool taylor = method == EVR_PROPA_TAYLOR;
+      const double t = n * dt + (taylor ? 0.0 : 0.5 * j * dt);
+      const int der = taylor ? j : 0;
+      for (int f = 0; f < nf; ++f) {
+        double e = 0.0;
+        for (int k = 1; k <= o->nk; ++k) {
+          double v = 0.0;
+          const int ms[3] = {k, k + 2, k - 2};
+          const double cs[3] = {0.5, -0.25, -0.25};
+          for (int q = 0; q < 3; ++q)
+            v += cs[q] * std::pow(ms[q] * w, der) * std::sin(ms[q] * w * t + der * 0.5 * M_PI);
+          e += v * amp[(size_t)f * o->nk + (k - 1)];
+        }
+        field_sub[((size_t)n * nsub + j) * nf + f] = e;
+      }
+    }
+  return EVR_OK;
+}
+
 extern "C" int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control_opts *o, int nf,
                                     const double *amp, const double *psi0, const double *phi1,
                                     double *J, double *grad_a, double *grad_E, double *grad_V,
@@ -167,8 +199,10 @@ extern "C" int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control
   const auto t0 = std::chrono::steady_clock::now();
   const int N = o->nsteps, ck = o->checkpoint > 0 ? o->checkpoint : 32;
   const double dt = o->T / N;
+  if (o->precision != EVR_FP64 && o->precision != EVR_FP32)
+    return set_error(EVR_ERR_ARG, "precision must be EVR_FP64 or EVR_FP32");
   Batch P;
-  RTRY(P.init(H, mu, nf, EVR_FP64));
+  RTRY(P.init(H, mu, nf, o->precision));
   int64_t sz[7];
   RTRY(evr_plan_sizes(H, sz));
   const int64_t nb = P.nb, nblk = nb * 2 * nf, nq = sz[3];
@@ -284,8 +318,8 @@ extern "C" int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control
         bool have_l2 = false;
         for (int k = K - 1; k >= 0; --k) {
           // H lam_{k+1}, mu lam_{k+1}
-          RTRY(P.apply(H, l1, cw.HW.p, EVR_FP64));
-          RTRY(P.apply(mu, l1, MW.p, EVR_FP64));
+          RTRY(P.apply(H, l1, cw.HW.p, P.flags));
+          RTRY(P.apply(mu, l1, MW.p, P.flags));
           // order k+1 used Hs w_k with factor s_{k+1} = -i (k = 0) or -2i
           const double sk = (k == 0) ? -1.0 : -2.0;  // s = i * sk
           // dE_n += Re((-s/esc) <mu lam|w_k>) = Re((-i sk/esc) <MW|w_k>)
@@ -339,6 +373,24 @@ extern "C" int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control
       for (int f = 0; f < nf; ++f)
         RTRY(evr::plan_grid_to_user(H, gv.data() + (size_t)nq * f, grad_V + (size_t)nq * f));
     }
+  }
+  if (o->precision == EVR_FP32) {
+    // refinement of the objective: one FP64 forward sweep from psi0
+    Batch P64;
+    RTRY(P64.init(H, mu, nf, EVR_FP64));
+    for (int f = 0; f < nf; ++f)
+      HTRY(hipMemcpyAsync(PSI.p + 2 * nb * f, psi0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, P64.st));
+    int r64 = 0;
+    for (int n = 0; n < N; ++n) {
+      RTRY(P64.set_field(field.data() + (size_t)n * nf));
+      ChebRecord rr;
+      RTRY(cheb_step(P64, cw, PSI.p, P64.dE.p, MW.p, dt, o->npoly, o->poly_tol, hmin, hmax, rr, r64, nullptr,
+                     false));
+    }
+    std::vector<cplx> ov64;
+    RTRY(P64.dots(PHI.p, 0, 1, PSI.p, ov64));
+    for (int f = 0; f < nf; ++f) J[f] = std::norm(ov64[f]);
+    P.hpsi += P64.hpsi;
   }
   if (info) {
     info->hpsi = P.hpsi;

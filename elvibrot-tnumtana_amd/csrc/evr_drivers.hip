@@ -1,6 +1,6 @@
 // Eigen-solver driver over the H|psi> engine: block Davidson with FP64
 // Rayleigh-Ritz, two-pass block orthogonalisation and optional FP32 H|psi>
-// refined in FP64 (see include/evr_hip.h, evr_davidson).
+// of the new vectors with FP64 defect corrections on the Ritz vectors.
 //
 // Follows the reference's sub_propagation_Davidson
 // (Source_ElVibRot/sub_propagation/sub_module_Davidson.f90:54-748): one new
@@ -133,14 +133,20 @@ __global__ void k_sum_parts(int m, int n, int nbat, const double *P, double *C, 
 
 // Rayleigh-Ritz matrix, columns m0..m-1 (and the matching rows) from
 // C1 = V^T HV[:, m0:m] (m x nn) and C2 = V[:, m0:m]^T HV (nn x m):
-// S(i, j) = S(j, i) = (C1(i, j - m0) + C2(j - m0, i)) / 2, the symmetric part
-__global__ void k_place_sym(int m, int m0, const double *C1, const double *C2, double *S, int lds)
+// S(i, j) = S(j, i) = (C1(i, j - m0) + C2(j - m0, i)) / 2, the symmetric part.
+// Columns below nex hold FP64 products H v (mixed precision: the others are
+// FP32 products): a coupling of such a column with another one is taken from
+// the FP64 product alone, v_j^T (H v_i) with i < nex <= j, so the FP32 error
+// of H v_j enters the Ritz vectors of the wanted states only through their
+// small components on v_j.
+__global__ void k_place_sym(int m, int m0, const double *C1, const double *C2, double *S, int lds, int nex)
 {
   const int nn = m - m0;
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < (int64_t)m * nn;
        g += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(g % m), jj = (int)(g / m), j = m0 + jj;
-    const double v = 0.5 * (C1[i + (int64_t)m * jj] + C2[jj + (int64_t)nn * i]);
+    const double a = C1[i + (int64_t)m * jj], b = C2[jj + (int64_t)nn * i];
+    const double v = (i < nex && j >= nex) ? b : (j < nex && i >= nex) ? a : 0.5 * (a + b);
     S[i + (int64_t)lds * j] = v;
     S[j + (int64_t)lds * i] = v;
   }
@@ -372,7 +378,10 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
   if (nev < 1 || nev > d.nb) return set_error(EVR_ERR_ARG, "nev out of range");
   int mmax = o->max_sub > 0 ? o->max_sub : std::max(3 * nev, nev + 4 * blk);
   mmax = (int)std::min<int64_t>(mmax, d.nb);
-  if (mmax < nev + 1 && mmax < d.nb) return set_error(EVR_ERR_ARG, "max_sub must exceed nev");
+  // a collapse keeps the Ritz vectors of every wanted state and makes room
+  // for one block of new vectors (DAV:516-606 keeps the wanted states)
+  if (mmax < nev + blk && mmax < d.nb)
+    return set_error(EVR_ERR_ARG, "max_sub must be at least nev + block (" + std::to_string(nev + blk) + ")");
   const int max_it = o->max_it > 0 ? o->max_it : 200;
   d.max_nvec = (int)o->max_nvec_hint;
   if (d.max_nvec <= 0) d.max_nvec = blk;
@@ -408,7 +417,7 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
   // preconditioner diagonal
   const bool use_diag = o->precond != 0;
   std::vector<double> hdiag;
-  if (use_diag || !o->guess) {
+  if (use_diag || !(o->guess || o->guess_file)) {
     HTRY(diag.alloc(nb));
     if (o->diag) {
       HTRY(hipMemcpyAsync(diag.p, o->diag, sizeof(double) * nb, hipMemcpyHostToDevice, d.st));
@@ -422,11 +431,29 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
     }
   }
 
+  // wavepacket files: the basis descriptor maps file rows to packed rows
+  int nb0 = 1;
+  if (o->guess_file || o->save_file) {
+    if (!o->basis) return set_error(EVR_ERR_ARG, "guess_file / save_file need opts->basis");
+    if (o->basis->nb_packed < 1 || nb % o->basis->nb_packed)
+      return set_error(EVR_ERR_ARG, "opts->basis does not match the operator's vector length");
+    nb0 = (int)(nb / o->basis->nb_packed);
+  }
+  std::vector<double> file_guess;
+  int nfile = 0;
+  if (o->guess_file) {
+    file_guess.resize((size_t)nb * mmax);
+    RTRY(evr_psi_read(o->basis, nb0, 0, mmax, file_guess.data(), &nfile, o->guess_file));
+    if (nfile < 1) return set_error(EVR_ERR_IO, std::string("no wavepacket in ") + o->guess_file);
+  }
+  const double *guess = o->guess_file ? file_guess.data() : o->guess;
+  const int nguess = o->guess_file ? nfile : o->nguess;
+
   // initial subspace: given vectors, else unit vectors on the lowest diagonal entries
   int m = 0;
-  int k0 = o->guess ? std::min<int>(o->nguess, mmax) : std::min(mmax, std::max(nev, blk));
-  if (o->guess) {
-    HTRY(hipMemcpyAsync(V.p, o->guess, sizeof(double) * nb * k0, hipMemcpyHostToDevice, d.st));
+  int k0 = guess ? std::min<int>(nguess, mmax) : std::min(mmax, std::max(nev, blk));
+  if (guess) {
+    HTRY(hipMemcpyAsync(V.p, guess, sizeof(double) * nb * k0, hipMemcpyHostToDevice, d.st));
   } else {
     std::vector<int64_t> idx(nb);
     for (int64_t i = 0; i < nb; ++i) idx[i] = i;
@@ -453,9 +480,23 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
     }
     m = done;
   }
-  int flags = (o->precision == EVR_FP32) ? EVR_FP32 : EVR_FP64;
-  bool refined = flags == EVR_FP64;
+  // Precision.  EVR_FP64: every H|psi> in FP64.  EVR_FP32 (mixed): every
+  // new vector's H|psi> in FP32; the subspace algebra (Rayleigh-Ritz,
+  // orthonormalisation, residuals) stays FP64, and FP64 enters as a defect
+  // correction: the subspace is collapsed on the Ritz vectors and H applied
+  // to them in FP64 when the residuals computed from the mixed H V have
+  // converged, stopped falling (the FP32 floor) or first dropped below
+  // refine_resi.  The leading nex columns of HV are then FP64 products; their
+  // couplings with later (FP32) columns are taken from the FP64 side
+  // (k_place_sym), so the Ritz vectors of the wanted states keep converging
+  // below the FP32 floor between corrections, and a collapse keeps them as
+  // nearly exact columns.  Convergence is declared only on an all-FP64 H V
+  // (right after a correction), so the reported residuals are FP64 residuals.
+  const bool mixed = o->precision == EVR_FP32;
+  const int flags = mixed ? EVR_FP32 : EVR_FP64;
   RTRY(d.apply(V.p, HV.p, m, flags));
+  int nex = mixed ? 0 : m;  // leading columns of HV that are FP64 products
+  int ncorr = 0;            // FP64 defect corrections made
 
   int m_s = 0;  // columns of V whose rows and columns are in Sk
   std::vector<double> th(mmax), th_old(nev, 1e300), rn(nev, 1e300);
@@ -463,8 +504,7 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
   int it = 0, nconv = 0;
   double maxres = 0.0;
   bool done = false;
-  // FP32 phase: also refine once the residual stops falling (the FP32 floor
-  // of H|psi>, which the Smolyak combination amplifies, is above refine_resi)
+  bool fresh = false;  // this Rayleigh-Ritz step follows a defect correction
   double best_res = 1e300;
   int stall = 0;
   for (it = 1; it <= max_it; ++it) {
@@ -475,13 +515,13 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
     if (m_s == 0) {
       RTRY(d.gemm_tn(m, m, V.p, HV.p, S.p, m));
       hipLaunchKernelGGL(k_place_sym, dim3(nblocks((int64_t)m * m)), dim3(256), 0, d.st, m, 0, S.p, S.p,
-                         Sk.p, mmax);
+                         Sk.p, mmax, nex);
     } else if (m > m_s) {
       const int nn = m - m_s;
       RTRY(d.gemm_tn(m, nn, V.p, HV.p + nb * m_s, tmpC.p, m));
       RTRY(d.gemm_tn(nn, m, V.p + nb * m_s, HV.p, C2.p, nn));
       hipLaunchKernelGGL(k_place_sym, dim3(nblocks((int64_t)m * nn)), dim3(256), 0, d.st, m, m_s, tmpC.p,
-                         C2.p, Sk.p, mmax);
+                         C2.p, Sk.p, mmax, nex);
     }
     HTRY(hipGetLastError());
     m_s = m;
@@ -517,36 +557,47 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
         maxres = std::max(maxres, rn[j0 + c]);
       }
     }
+    // convergence: |r| < conv_resi and |E(it) - E(it-1)| < conv_ene
+    // (DAV:407-446).  Right after a defect correction the energy change
+    // measures the FP32 error of the previous step, not the iteration, so
+    // the FP64 residual alone decides there (|dE| <= |r|^2 / gap).
     nconv = 0;
     for (int j = 0; j < ne; ++j) {
-      conv[j] = (rn[j] < o->conv_resi && std::fabs(th[j] - th_old[j]) < o->conv_ene) ? 1 : 0;
+      conv[j] = (rn[j] < o->conv_resi && (fresh || std::fabs(th[j] - th_old[j]) < o->conv_ene)) ? 1 : 0;
       nconv += conv[j];
       th_old[j] = th[j];
     }
+    fresh = false;
     if (o->verbose) {
-      fprintf(stderr, "davidson it %d m %d conv %d/%d max|r| %.3e E0 %.10f %s\n", it, m, nconv, nev,
-              maxres, th[0], refined ? "fp64" : "fp32");
+      fprintf(stderr, "davidson it %d m %d (fp64 cols %d) conv %d/%d max|r| %.3e E0 %.10f\n", it, m, nex,
+              nconv, nev, maxres, th[0]);
       if (o->verbose > 1)
         for (int j = 0; j < ne; ++j) fprintf(stderr, "   %3d %.10f %.3e\n", j, th[j], rn[j]);
     }
-    if (!refined) {
+    const bool all_conv = nconv == ne && ne == nev;
+    if (all_conv && nex == m) { done = true; break; }
+    if (mixed) {
       if (maxres < 0.9 * best_res) {
         best_res = maxres;
         stall = 0;
       } else {
         ++stall;
       }
+      if (all_conv || stall >= 6 || (ncorr == 0 && maxres < o->refine_resi)) {
+        // defect correction: collapse on the Ritz vectors and apply H in FP64
+        const int keep = kr;
+        HTRY(hipMemcpyAsync(V.p, XV.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
+        RTRY(d.apply(V.p, HV.p, keep, EVR_FP64));
+        m = keep;
+        nex = keep;
+        m_s = 0;
+        ++ncorr;
+        fresh = true;
+        best_res = 1e300;
+        stall = 0;
+        continue;
+      }
     }
-    if (!refined && (maxres < o->refine_resi || nconv == ne || stall >= 6)) {
-      // FP64 refinement: rebuild H V in FP64, continue with FP64 H|psi>
-      flags = EVR_FP64;
-      refined = true;
-      RTRY(d.apply(V.p, HV.p, m, flags));
-      m_s = 0;
-      for (int j = 0; j < nev; ++j) th_old[j] = 1e300;
-      continue;
-    }
-    if (nconv == ne && ne == nev && refined) { done = true; break; }
     // new vectors: one per unconverged wanted state (DAV:1290-1588); when there
     // are more than the block holds, the states with the largest residuals
     std::vector<int> sel;
@@ -557,14 +608,18 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
       sel.resize(blk);
       std::sort(sel.begin(), sel.end());
     }
-    if (sel.empty())  // converged in FP32 before the switch: force the refinement
-      for (int j = 0; j < ne && (int)sel.size() < blk; ++j) sel.push_back(j);
+    if (sel.empty()) break;  // all converged on a mixed H V: handled above
     int nc = (int)sel.size();
     if (m + nc > mmax) {
-      // collapse on the current Ritz vectors (XV, XH hold kr of them)
+      // collapse on the current Ritz vectors (XV, XH hold kr of them; at
+      // least nev, since mmax >= nev + block)
       const int keep = std::min(kr, mmax - nc);
       HTRY(hipMemcpyAsync(V.p, XV.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
       HTRY(hipMemcpyAsync(HV.p, XH.p, sizeof(double) * nb * keep, hipMemcpyDeviceToDevice, d.st));
+      // FP64: all kept columns exact.  Mixed: the wanted Ritz vectors stay
+      // nearly exact when the subspace had FP64 columns (their FP32 parts
+      // carry small coefficients); the other kept vectors are FP32 products.
+      nex = !mixed ? keep : (nex > 0 ? std::min(keep, nev) : 0);
       m = keep;
       m_s = 0;
     }
@@ -587,6 +642,7 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
     RTRY(d.apply(T.p, HT.p, kept, flags));
     HTRY(hipMemcpyAsync(V.p + nb * m, T.p, sizeof(double) * nb * kept, hipMemcpyDeviceToDevice, d.st));
     HTRY(hipMemcpyAsync(HV.p + nb * m, HT.p, sizeof(double) * nb * kept, hipMemcpyDeviceToDevice, d.st));
+    if (!mixed) nex = m + kept;
     m += kept;
   }
   // results: Ritz values and vectors of the last Rayleigh-Ritz step
@@ -594,6 +650,13 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
   if (evecs)
     HTRY(hipMemcpyAsync(evecs, XV.p, sizeof(double) * nb * std::min(nev, m), hipMemcpyDeviceToHost, d.st));
   HTRY(hipStreamSynchronize(d.st));
+  if (o->save_file && m > 0) {
+    const int ns = std::min(nev, m);
+    std::vector<double> vs((size_t)nb * ns);
+    HTRY(hipMemcpyAsync(vs.data(), XV.p, sizeof(double) * vs.size(), hipMemcpyDeviceToHost, d.st));
+    HTRY(hipStreamSynchronize(d.st));
+    RTRY(evr_psi_save(o->basis, nb0, ns, 0, vs.data(), 0, 1, o->save_file));
+  }
   if (info) {
     info->iterations = std::min(it, max_it);
     info->converged = done ? 1 : 0;

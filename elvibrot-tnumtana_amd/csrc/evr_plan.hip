@@ -88,6 +88,7 @@ struct evr_desc_owner {
 
 struct evr_plan {
   int D = 0, type = 0, max_nvec = 0;  // D: the term kernel's mode positions
+  int device = 0;                     // HIP device current at creation
   int D_full = 0;                     // modes of the basis (> D when compacted)
   int64_t keoc_stride = 0;            // doubles per KEO coefficient set
   bool dp_order = false;
@@ -99,17 +100,20 @@ struct evr_plan {
   std::vector<evr::TermDev> terms;
   std::vector<evr::SlotDev> slots;
   std::vector<int> dp_perm;  // DP: reference grid index of each term-grid point
+  DevBuf<int> d_dp_perm;     // the same on the device (multi-vector transforms)
   // device tables
   DevBuf<evr::TermDev> d_terms;
   DevBuf<evr::SlotDev> d_slots;
   std::vector<evr::StepDev> steps;
   DevBuf<evr::StepDev> d_steps;
-  DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag, d_D1;
+  DevBuf<double> d_bw64, d_keo64, d_x, d_bwdiag, d_D1, d_D2;
   DevBuf<float> d_bw32, d_keo32;
   // type 10 with G, J, s per grid point (evr_plan_set_keo_grid): the tables,
   // the (term, first point) items of the grid kernels, and work grids for
   // gvec vectors at a time (psi_G, the D derivative columns, the KEO output)
   bool hermitian = false, keo_grid = false;
+  int grid_type = 0;  // keo_grid: 10 (G, J, s per point) or 1 (F2, F1 per point)
+  DevBuf<double> d_F2g, d_F1g;  // type 1 operator grids (sum_nq x D x D, sum_nq x D)
   int gvec = 0;
   int64_t n_gitems = 0;
   DevBuf<double> d_GG, d_Jg, d_Sg, d_Gg, d_Pg, d_Fg;
@@ -127,6 +131,8 @@ struct evr_plan {
   DevBuf<double> d_coefC, d_coefA, d_coefF1, d_keoc;
   DevBuf<int> d_order[kNumClasses];
   int64_t nunits[kNumClasses] = {};
+  std::vector<int> tclass;  // kernel class of every term (static rule or autotuned)
+  bool keo_layout = false;  // the term layouts hold the KEO factors in LDS
   size_t lds_bytes64[kNumClasses] = {}, lds_bytes32[kNumClasses] = {};
   // work buffers
   DevBuf<double> d_X, d_R, d_psi, d_hpsi, d_G;
@@ -162,7 +168,9 @@ struct evr_plan {
     d_coefC.release(); d_coefA.release(); d_coefF1.release(); d_keoc.release();
     for (int c = 0; c < kNumClasses; ++c) d_order[c].release();
     d_X.release(); d_R.release(); d_psi.release(); d_hpsi.release(); d_G.release();
-    d_A64.release(); d_A32.release(); d_asm_items.release();
+    d_A64.release(); d_A32.release(); d_asm_items.release(); d_D1.release(); d_D2.release();
+    d_GG.release(); d_Jg.release(); d_Sg.release(); d_Gg.release(); d_Pg.release(); d_Fg.release();
+    d_F2g.release(); d_F1g.release(); d_gitems.release(); d_dp_perm.release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -275,6 +283,117 @@ double term_flops(const evr_plan *p, const evr::TermDev &td)
   return f;
 }
 
+// Slab split of term td for kernel class c (the largest first-m-modes slab
+// of each mode list that fits the class's LDS budget) and its 2m contraction
+// steps, B->G then G->B, appended to steps; words: LDS doubles needed.
+bool layout_term(const evr_plan *p, evr::TermDev &td, int c, std::vector<evr::StepDev> &steps, size_t &words)
+{
+  const int D = p->D;
+  int nbd[evr::kMaxDev], nqd[evr::kMaxDev], fullm[evr::kMaxDev];
+  for (int d = 0; d < D; ++d) {
+    nbd[d] = p->slots[td.slot[d]].nb;
+    nqd[d] = p->slots[td.slot[d]].nq;
+    fullm[d] = d;
+  }
+  const int64_t nbt = td.nbt;
+  const int nvw = kClasses[c].nvw;
+  const size_t keo_bufs = kClasses[c].ept <= 2 ? 3 : 1;  // LDS buffers of the KEO path
+  const size_t base = td.lds_mats + (size_t)nbt * nvw;
+  const size_t keo_words = p->keo_layout ? keo_bufs * nbt * nvw : 0;
+  // half the LDS when the term fits in it, so two workgroups share a CU
+  const size_t half = (kLdsLimit / 2 - 1024) / 8;
+  const size_t budget =
+      base + std::max<size_t>(keo_words, 2 * (size_t)nvw + 2 * nbt) <= half ? half : kLdsLimit / 8;
+  words = base + keo_words;
+  auto split = [&](const int *ml, int nm, int &m_, int &nbA_, int &QA_, int &nbB_, int &QB_,
+                   int &step0) -> bool {
+    int best = -1;
+    int64_t nbA = 1, QA = 1;
+    for (int m = 0; m <= nm; ++m) {
+      if (m > 0) {
+        nbA *= nbd[ml[m - 1]];
+        QA *= nqd[ml[m - 1]];
+      }
+      const size_t work = std::max<size_t>(keo_words, 2 * (size_t)QA * nvw + 2 * (nbt / nbA));
+      if (base + work <= budget) best = m;
+    }
+    if (best < 0) return false;
+    m_ = best;
+    nbA_ = 1;
+    QA_ = 1;
+    int64_t qall = 1;
+    for (int k = 0; k < nm; ++k) qall *= nqd[ml[k]];
+    for (int k = 0; k < best; ++k) { nbA_ *= nbd[ml[k]]; QA_ *= nqd[ml[k]]; }
+    nbB_ = (int)(nbt / nbA_);
+    QB_ = (int)(qall / QA_);
+    step0 = (int)steps.size();
+    for (int k = 0; k < best; ++k) {
+      evr::StepDev sd{};
+      int n1 = 1, n3 = 1;
+      for (int e2 = 0; e2 < k; ++e2) n1 *= nqd[ml[e2]];
+      for (int e2 = k + 1; e2 < best; ++e2) n3 *= nbd[ml[e2]];
+      sd.n1 = n1; sd.n3 = n3; sd.nin = nbd[ml[k]]; sd.nout = nqd[ml[k]];
+      sd.mat = p->slots[td.slot[ml[k]]].offB;
+      steps.push_back(sd);
+    }
+    for (int k = 0; k < best; ++k) {
+      evr::StepDev sd{};
+      int n1 = 1, n3 = 1;
+      for (int e2 = 0; e2 < k; ++e2) n1 *= nbd[ml[e2]];
+      for (int e2 = k + 1; e2 < best; ++e2) n3 *= nqd[ml[e2]];
+      sd.n1 = n1; sd.n3 = n3; sd.nin = nqd[ml[k]]; sd.nout = nbd[ml[k]];
+      sd.mat = p->slots[td.slot[ml[k]]].offW;
+      steps.push_back(sd);
+    }
+    words = std::max(words, base + std::max<size_t>(keo_words, 2 * (size_t)QA_ * nvw + 2 * nbB_));
+    return true;
+  };
+  return split(fullm, D, td.m, td.nbA, td.QA, td.nbB, td.QB, td.step0) &&
+         split(td.vmode, td.Dv, td.vm, td.vnbA, td.vQA, td.vnbB, td.vQB, td.vstep0);
+}
+
+// Lays out every term for its class p->tclass[t]: slab splits and steps, LDS
+// sizes and flops per class, the per-class work lists (most expensive terms
+// first); uploads the term table, the steps and the lists.
+int apply_classes(evr_plan *p)
+{
+  std::vector<evr::StepDev> steps;
+  for (int c = 0; c < kNumClasses; ++c) {
+    p->lds_bytes64[c] = p->lds_bytes32[c] = 0;
+    p->flops_class[c] = 0.0;
+    p->nunits[c] = 0;
+    p->d_order[c].release();
+  }
+  std::vector<double> cost(p->nterms);
+  for (int64_t t = 0; t < p->nterms; ++t) {
+    evr::TermDev &td = p->terms[t];
+    const int c = p->tclass[t];
+    if (kClasses[c].max_nbt() < td.nbt) return fail(EVR_ERR_ARG, "term class too small for its basis");
+    size_t words = 0;
+    if (!layout_term(p, td, c, steps, words)) return fail(EVR_ERR_SIZE, "term does not fit in LDS");
+    p->lds_bytes64[c] = std::max(p->lds_bytes64[c], words * 8);
+    p->lds_bytes32[c] = std::max(p->lds_bytes32[c], words * 4);
+    p->flops_class[c] += term_flops(p, td);
+    double qv = 1.0;
+    for (int j = 0; j < td.Dv; ++j) qv *= p->slots[td.slot[td.vmode[j]]].nq;
+    cost[t] = qv * 50.0 + (double)td.nbt * (td.na + 1) * (td.na + 1) * 6.0;
+  }
+  p->steps.swap(steps);
+  for (int c = 0; c < kNumClasses; ++c) {
+    std::vector<int> ids;
+    for (int64_t t = 0; t < p->nterms; ++t)
+      if (p->tclass[t] == c) ids.push_back((int)t);
+    if (ids.empty()) continue;
+    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    while (ids.size() % 8) ids.push_back(-1);
+    p->nunits[c] = (int64_t)ids.size();
+    HIP_TRY(p->d_order[c].upload(ids.data(), ids.size(), p->stream));
+  }
+  HIP_TRY(p->d_terms.upload(p->terms.data(), p->terms.size(), p->stream));
+  HIP_TRY(p->d_steps.upload(p->steps.data(), p->steps.size(), p->stream));
+  return EVR_OK;
+}
+
 // Apply every term class for `nvec` vectors held in X ([nb_packed][nvec]).
 template <typename T>
 int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *Gin, const T *bw,
@@ -379,9 +498,11 @@ hipError_t reduce_op(const AsmOp &o, const T *R, int nvec, double *out)
 }
 
 template <typename T>
-hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out)
+hipError_t launch_reduce(evr_plan *p, const T *R, int nvec, double *out, double *P = nullptr)
 {
-  return reduce_op<T>(plan_op(p), R, nvec, out);
+  AsmOp o = plan_op(p);
+  if (P) o.P = P;
+  return reduce_op<T>(o, R, nvec, out);
 }
 
 // R = H_t X_t for every term from the assembled pool A (32 vectors per pass).
@@ -510,17 +631,25 @@ int keo_grid_hpsi(evr_plan *p, int nvec, const double *dpsi, double *dh)
     const int k = std::min(p->gvec, nvec - c0);
     int rc = evr::plan_b2g_device(p, k, dpsi + nb * c0, p->d_Gg.ptr);
     if (rc) return rc;
+    const bool t1 = p->grid_type == 1;
     hipLaunchKernelGGL(evr::k_gk_deriv, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
-                       p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Sg.ptr,
-                       p->d_Gg.ptr, nq, k, p->d_Pg.ptr);
+                       p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr,
+                       t1 ? nullptr : p->d_Sg.ptr, p->d_Gg.ptr, nq, k, p->d_Pg.ptr);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(evr::k_gk_combine, dim3(blocks_for(nq)), dim3(256), 0, p->stream, D, nq, k,
-                       p->d_GG.ptr, p->d_Jg.ptr, p->d_Pg.ptr);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(evr::k_gk_out, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
-                       p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
-                       p->d_Sg.ptr, p->d_V.ptr, p->d_Gg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
-    HIP_TRY(hipGetLastError());
+    if (t1) {
+      hipLaunchKernelGGL(evr::k_t1_out, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                         p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_D2.ptr, p->d_gitems.ptr,
+                         p->d_F2g.ptr, p->d_F1g.ptr, p->d_V.ptr, p->d_Gg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
+      HIP_TRY(hipGetLastError());
+    } else {
+      hipLaunchKernelGGL(evr::k_gk_combine, dim3(blocks_for(nq)), dim3(256), 0, p->stream, D, nq, k,
+                         p->d_GG.ptr, p->d_Jg.ptr, p->d_Pg.ptr);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(evr::k_gk_out, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                         p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
+                         p->d_Sg.ptr, p->d_V.ptr, p->d_Gg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
+      HIP_TRY(hipGetLastError());
+    }
     rc = run_terms<double>(p, evr::kModeG2B, k, nullptr, p->d_R.ptr, nullptr, p->d_Fg.ptr, p->d_bw64.ptr,
                            p->d_keo64.ptr, p->d_V.ptr);
     if (rc) return rc;
@@ -537,6 +666,37 @@ int fold_v(evr_plan *p)
   hipLaunchKernelGGL(evr::k_fold_v, dim3((unsigned)p->nterms), dim3(256), 0, p->stream, p->D,
                      p->d_terms.ptr, p->d_slots.ptr, p->d_bw64.ptr, p->d_V.ptr, p->d_Veff.ptr);
   HIP_TRY(hipGetLastError());
+  return EVR_OK;
+}
+
+// Work items (term, first point) of the grid-form kernels and the work grids
+// of gvec vectors at a time (psi_G, its D derivatives, the operator output):
+// made once per plan.
+int grid_items(evr_plan *p)
+{
+  if (p->d_gitems.ptr) return EVR_OK;
+  std::vector<int2> items;
+  for (int64_t t = 0; t < p->nterms; ++t)
+    for (int q0 = 0; q0 < p->terms[t].nqt; q0 += 256) items.push_back(make_int2((int)t, q0));
+  if ((int64_t)items.size() > (int64_t)0xFFFFFFFFu / 256)
+    return fail(EVR_ERR_SIZE, "too many grid work items for one launch");
+  p->n_gitems = (int64_t)items.size();
+  HIP_TRY(p->d_gitems.upload(items.data(), items.size(), p->stream));
+  return EVR_OK;
+}
+
+int grid_form_buffers(evr_plan *p)
+{
+  if (p->d_Gg.ptr) return EVR_OK;
+  const int D = p->D;
+  const int64_t nq = p->sum_nq;
+  int rc = grid_items(p);
+  if (rc) return rc;
+  // (D + 2) x sum_nq doubles per vector
+  p->gvec = (int)std::max<int64_t>(1, std::min<int64_t>(p->max_nvec, (int64_t)(8e9 / (8.0 * (D + 2) * nq))));
+  HIP_TRY(p->d_Gg.alloc((size_t)nq * p->gvec));
+  HIP_TRY(p->d_Pg.alloc((size_t)nq * D * p->gvec));
+  HIP_TRY(p->d_Fg.alloc((size_t)nq * p->gvec));
   return EVR_OK;
 }
 
@@ -716,13 +876,14 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->dp_order = bd->grid_last_fastest != 0;
   p->nterms = bd->nterms;
   p->nb_packed = bd->nb_packed;
+  HIP_TRY(hipGetDevice(&p->device));
   HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
 
   // ---- slots: 1D matrices and basis-space KEO factors
   const int ns = D * nlev;
   std::vector<int64_t> ox, oB, oD;
   slot_offsets(bd, ox, oB, oD);
-  std::vector<double> bw, keo, xs, bwdiag, d1;
+  std::vector<double> bw, keo, xs, bwdiag, d1, d2;
   std::vector<char> slot_fold(ns, 0);  // one function with vanishing P, Q, R
   p->slots.resize(ns);
   for (int s = 0; s < ns; ++s) {
@@ -745,6 +906,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     xs.insert(xs.end(), bd->x + ox[s], bd->x + ox[s + 1]);
     sd.offD1 = (int64_t)d1.size();
     d1.insert(d1.end(), bd->D1 + oD[s], bd->D1 + oD[s + 1]);
+    d2.insert(d2.end(), bd->D2 + oD[s], bd->D2 + oD[s + 1]);
     evr::Slot1D s1;
     s1.nb = nb;
     s1.nq = nq;
@@ -849,6 +1011,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
       xs.push_back(0.0);
       sd.offD1 = (int64_t)d1.size();
       d1.push_back(0.0);
+      d2.push_back(0.0);
       sd.offK = (int64_t)keo.size();
       keo.insert(keo.end(), evr::kNumFactors, 0.0);
       keo[sd.offK + Td] = fTd;
@@ -912,7 +1075,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->terms.resize(p->nterms);
   int64_t onb = 0, onq = 0, onvq = 0;
   std::vector<int> cls(p->nterms);
-  std::vector<double> cost(p->nterms);
+  p->keo_layout = p->hasKeo;
   for (int64_t t = 0; t < p->nterms; ++t) {
     evr::TermDev &td = p->terms[t];
     int nbd[evr::kMaxDev], nqd[evr::kMaxDev];
@@ -940,17 +1103,13 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     while (c < kNumClasses && kClasses[c].max_nbt() < nbt) ++c;
     if (c == kNumClasses) return fail(EVR_ERR_SIZE, "term basis larger than 8192 functions");
     cls[t] = c;
-    const int nvw = kClasses[c].nvw;
-    const size_t keo_bufs = kClasses[c].ept <= 2 ? 3 : 1;  // LDS buffers of the KEO path
     // mode lists: all modes (basis <-> grid transforms), the modes with more
     // than one function (the H|psi> V part, one-function modes folded into
     // V_eff), and the modes whose P, Q, R do not vanish (the kinetic part)
-    int fullm[evr::kMaxDev];
     td.Dv = 0;
     td.na = 0;
     int64_t qv = 1;
     for (int d = 0; d < D; ++d) {
-      fullm[d] = d;
       if (nbd[d] > 1) {
         td.vmode[td.Dv++] = d;
         qv *= nqd[d];
@@ -963,65 +1122,6 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     td.off_asm = p->asm_elems;
     p->asm_elems += (int64_t)((nbt + 15) / 16) * evr::asm_ksteps(nbt) * 64;
     p->asm_flops_per_vec += 2.0 * (double)nbt * (double)nbt;
-    // largest slab (first m modes of a list) that fits the LDS budget (double
-    // words); then the 2m slab contraction steps, B->G then G->B
-    const size_t base = td.lds_mats + (size_t)nbt * nvw;
-    const size_t keo_words = p->hasKeo ? keo_bufs * nbt * nvw : 0;
-    // half the LDS when the term fits in it, so two workgroups share a CU
-    const size_t half = (kLdsLimit / 2 - 1024) / 8;
-    const size_t budget =
-        base + std::max<size_t>(keo_words, 2 * (size_t)nvw + 2 * nbt) <= half ? half : kLdsLimit / 8;
-    size_t words = base + keo_words;
-    auto split = [&](const int *ml, int nm, int &m_, int &nbA_, int &QA_, int &nbB_, int &QB_,
-                     int &step0) -> bool {
-      int best = -1;
-      int64_t nbA = 1, QA = 1;
-      for (int m = 0; m <= nm; ++m) {
-        if (m > 0) {
-          nbA *= nbd[ml[m - 1]];
-          QA *= nqd[ml[m - 1]];
-        }
-        const size_t work = std::max<size_t>(keo_words, 2 * (size_t)QA * nvw + 2 * (nbt / nbA));
-        if (base + work <= budget) best = m;
-      }
-      if (best < 0) return false;
-      m_ = best;
-      nbA_ = 1;
-      QA_ = 1;
-      int64_t qall = 1;
-      for (int k = 0; k < nm; ++k) qall *= nqd[ml[k]];
-      for (int k = 0; k < best; ++k) { nbA_ *= nbd[ml[k]]; QA_ *= nqd[ml[k]]; }
-      nbB_ = (int)(nbt / nbA_);
-      QB_ = (int)(qall / QA_);
-      step0 = (int)p->steps.size();
-      for (int k = 0; k < best; ++k) {
-        evr::StepDev sd{};
-        int n1 = 1, n3 = 1;
-        for (int e2 = 0; e2 < k; ++e2) n1 *= nqd[ml[e2]];
-        for (int e2 = k + 1; e2 < best; ++e2) n3 *= nbd[ml[e2]];
-        sd.n1 = n1; sd.n3 = n3; sd.nin = nbd[ml[k]]; sd.nout = nqd[ml[k]];
-        sd.mat = p->slots[td.slot[ml[k]]].offB;
-        p->steps.push_back(sd);
-      }
-      for (int k = 0; k < best; ++k) {
-        evr::StepDev sd{};
-        int n1 = 1, n3 = 1;
-        for (int e2 = 0; e2 < k; ++e2) n1 *= nbd[ml[e2]];
-        for (int e2 = k + 1; e2 < best; ++e2) n3 *= nqd[ml[e2]];
-        sd.n1 = n1; sd.n3 = n3; sd.nin = nqd[ml[k]]; sd.nout = nbd[ml[k]];
-        sd.mat = p->slots[td.slot[ml[k]]].offW;
-        p->steps.push_back(sd);
-      }
-      words = std::max(words, base + std::max<size_t>(keo_words, 2 * (size_t)QA_ * nvw + 2 * nbB_));
-      return true;
-    };
-    if (!split(fullm, D, td.m, td.nbA, td.QA, td.nbB, td.QB, td.step0) ||
-        !split(td.vmode, td.Dv, td.vm, td.vnbA, td.vQA, td.vnbB, td.vQB, td.vstep0))
-      return fail(EVR_ERR_SIZE, "term does not fit in LDS");
-    p->lds_bytes64[c] = std::max(p->lds_bytes64[c], words * 8);
-    p->lds_bytes32[c] = std::max(p->lds_bytes32[c], words * 4);
-    cost[t] = (double)qv * 50.0 + (double)nbt * (td.na + 1) * (td.na + 1) * 6.0;
-    p->flops_class[c] += term_flops(p.get(), td);
     onb += nbt;
     onq += nqt;
     onvq += qv;
@@ -1033,16 +1133,12 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->sum_vq = onvq;
   if (onb > INT32_MAX) return fail(EVR_ERR_SIZE, "sum of term basis sizes exceeds int32");
 
-  // ---- per-class work lists, most expensive terms first
-  for (int c = 0; c < kNumClasses; ++c) {
-    std::vector<int> ids;
-    for (int64_t t = 0; t < p->nterms; ++t)
-      if (cls[t] == c) ids.push_back((int)t);
-    if (ids.empty()) continue;
-    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    while (ids.size() % 8) ids.push_back(-1);
-    p->nunits[c] = (int64_t)ids.size();
-    HIP_TRY(p->d_order[c].upload(ids.data(), ids.size(), p->stream));
+  // ---- slab layouts for the classes (the static rule: the smallest class
+  // that holds the term) and the per-class work lists
+  p->tclass = cls;
+  {
+    const int rc = apply_classes(p.get());
+    if (rc) return rc;
   }
 
   // ---- deterministic scatter: the term rows of packed function pk, in
@@ -1080,9 +1176,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
 
   // ---- uploads
   std::vector<float> bw32(bw.begin(), bw.end()), keo32(keo.begin(), keo.end());
-  HIP_TRY(p->d_terms.upload(p->terms.data(), p->terms.size(), p->stream));
   HIP_TRY(p->d_slots.upload(p->slots.data(), p->slots.size(), p->stream));
-  HIP_TRY(p->d_steps.upload(p->steps.data(), p->steps.size(), p->stream));
   HIP_TRY(p->d_bw64.upload(bw.data(), bw.size(), p->stream));
   HIP_TRY(p->d_bwdiag.upload(bwdiag.data(), bwdiag.size(), p->stream));
   HIP_TRY(p->d_keo64.upload(keo.data(), keo.size(), p->stream));
@@ -1090,7 +1184,9 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   HIP_TRY(p->d_keo32.upload(keo32.data(), keo32.size(), p->stream));
   HIP_TRY(p->d_x.upload(xs.data(), xs.size(), p->stream));
   HIP_TRY(p->d_D1.upload(d1.data(), d1.size(), p->stream));
+  HIP_TRY(p->d_D2.upload(d2.data(), d2.size(), p->stream));
   HIP_TRY(p->d_map.upload(bd->map, onb, p->stream));
+  if (p->dp_order) HIP_TRY(p->d_dp_perm.upload(p->dp_perm.data(), p->dp_perm.size(), p->stream));
   {
     std::vector<int> seg, fseg(p->nb_packed + 1);
     seg.reserve(p->nb_packed + p->nb_packed / 8 + 1);
@@ -1253,24 +1349,44 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
   if (!Jac || !sqRhoOVERJac) one.assign(nq, 1.0);
   HIP_TRY(p->d_Jg.upload(Jac ? Jac : one.data(), nq, p->stream));
   HIP_TRY(p->d_Sg.upload(sqRhoOVERJac ? sqRhoOVERJac : one.data(), nq, p->stream));
-  if (!p->keo_grid) {
-    std::vector<int2> items;
-    for (int64_t t = 0; t < p->nterms; ++t)
-      for (int q0 = 0; q0 < p->terms[t].nqt; q0 += 256) items.push_back(make_int2((int)t, q0));
-    p->n_gitems = (int64_t)items.size();
-    HIP_TRY(p->d_gitems.upload(items.data(), items.size(), p->stream));
-    // work grids of gvec vectors: (D + 2) x sum_nq doubles each
-    p->gvec = (int)std::max<int64_t>(1, std::min<int64_t>(p->max_nvec, (int64_t)(8e9 / (8.0 * (D + 2) * nq))));
-    HIP_TRY(p->d_Gg.alloc((size_t)nq * p->gvec));
-    HIP_TRY(p->d_Pg.alloc((size_t)nq * D * p->gvec));
-    HIP_TRY(p->d_Fg.alloc((size_t)nq * p->gvec));
-  }
+  int rc = grid_form_buffers(p);
+  if (rc) return rc;
   // the term kernels keep the B->G and G->B transforms; the operator itself
   // is the grid form
   p->hasKeo = false;
   p->hasA = false;
   p->hasF1 = false;
   p->keo_grid = true;
+  p->grid_type = 10;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return EVR_OK;
+}
+
+int evr_plan_set_type1_grid(evr_plan *p, const double *F2, const double *F1)
+{
+  if (!p || !F2) return fail(EVR_ERR_ARG, "NULL argument");
+  if (p->type != 1) return fail(EVR_ERR_ARG, "operator grids F2, F1 per point belong to a type-1 plan");
+  if (p->D != p->D_full) return fail(EVR_ERR_ARG, "operator grids per point need D <= 12");
+  if (p->dp_order) return fail(EVR_ERR_ARG, "operator grids per point are for SG4 bases");
+  if (p->hermitian) return fail(EVR_ERR_ARG, "operator grids per point have no symmetric form (hermitian = 0)");
+  const int D = p->D;
+  const int64_t nq = p->sum_nq;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->d_A64.release();
+  p->d_A32.release();
+  HIP_TRY(p->d_F2g.upload(F2, (size_t)D * D * nq, p->stream));
+  if (F1) {
+    HIP_TRY(p->d_F1g.upload(F1, (size_t)D * nq, p->stream));
+  } else {
+    p->d_F1g.release();
+  }
+  int rc = grid_form_buffers(p);
+  if (rc) return rc;
+  p->hasKeo = false;
+  p->hasA = false;
+  p->hasF1 = false;
+  p->keo_grid = true;
+  p->grid_type = 1;
   HIP_TRY(hipStreamSynchronize(p->stream));
   return EVR_OK;
 }
@@ -1293,7 +1409,7 @@ int evr_plan_assemble(evr_plan *p, int which)
   if (which < 1 || which > (EVR_ASM_FP64 | EVR_ASM_FP32))
     return fail(EVR_ERR_ARG, "which must be EVR_ASM_FP64 and/or EVR_ASM_FP32");
   if (p->keo_grid)
-    return fail(EVR_ERR_ARG, "a plan with G, J, s per grid point runs the grid form (not assembled)");
+    return fail(EVR_ERR_ARG, "a plan with operator grids per point runs the grid form (not assembled)");
   const size_t n = (size_t)p->asm_elems;
   if ((which & EVR_ASM_FP64) && !p->d_A64.ptr) {
     if (p->d_A64.alloc(n) != hipSuccess) {
@@ -1316,9 +1432,7 @@ int evr_plan_assemble(evr_plan *p, int which)
     std::vector<int2> items;
     for (int64_t t = 0; t < p->nterms; ++t) {
       const int nbt = p->terms[t].nbt;
-      int cc = 0;
-      while (kClasses[cc].max_nbt() < nbt) ++cc;
-      if (cc != c) continue;
+      if (p->tclass[t] != c) continue;
       for (int c0 = 0; c0 < nbt; c0 += kClasses[c].nvw) items.push_back(int2{(int)t, c0});
     }
     if (items.empty()) continue;
@@ -1371,11 +1485,7 @@ int evr_plan_release_assembled(evr_plan *p)
 static void recount_flops(evr_plan *p)
 {
   for (int c = 0; c < kNumClasses; ++c) p->flops_class[c] = 0;
-  for (const evr::TermDev &td : p->terms) {
-    int c = 0;
-    while (kClasses[c].max_nbt() < td.nbt) ++c;
-    p->flops_class[c] += term_flops(p, td);
-  }
+  for (int64_t t = 0; t < p->nterms; ++t) p->flops_class[p->tclass[t]] += term_flops(p, p->terms[t]);
 }
 
 int evr_plan_set_profiling(evr_plan *p, int on)
@@ -1444,12 +1554,75 @@ static bool matrix_free_cheaper(const evr_plan *p, int nvec, bool fp32)
   return t_mf < t_asm;
 }
 
+}  // extern "C"
+
+// H|psi> of nvec device columns on the plan stream with the work buffers
+// given: X the packed block (nb_packed x nvec, FP64 words; FP32 uses the
+// first half), R the term rows in scatter order (nrows x nvec words), P the
+// segment sums (nseg x nvec doubles).  Every word of X, R and P that a call
+// reads is written earlier in the same call.
+static int oppsi_core(evr_plan *p, int nvec, const double *dpsi, double *dh, int flags, double *Xbuf,
+                      double *Rbuf, double *Pbuf)
+{
+  const bool fp32 = (flags & EVR_FP32) != 0;
+  const int64_t n = p->nb_packed * nvec;
+  int rc;
+  if (p->keo_grid && fp32) return fail(EVR_ERR_ARG, "a plan with operator grids per point (G, J, s or F2, F1) is FP64 only");
+  const bool use_asm = !(flags & EVR_MATRIX_FREE) &&
+                       (fp32 ? p->d_A32.ptr != nullptr : p->d_A64.ptr != nullptr) &&
+                       !((flags & EVR_AUTO_PATH) && matrix_free_cheaper(p, nvec, fp32));
+  if (use_asm) {
+    if (fp32) {
+      float *X = reinterpret_cast<float *>(Xbuf), *R = reinterpret_cast<float *>(Rbuf);
+      hipLaunchKernelGGL(evr::k_pack_in<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                         p->nb_packed, nvec, X);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(apply_assembled<float>(p, p->d_A32.ptr, X, nvec, R));
+      HIP_TRY(launch_reduce<float>(p, R, nvec, dh, Pbuf));
+    } else {
+      double *X = Xbuf, *R = Rbuf;
+      hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                         p->nb_packed, nvec, X);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(apply_assembled<double>(p, p->d_A64.ptr, X, nvec, R));
+      HIP_TRY(launch_reduce<double>(p, R, nvec, dh, Pbuf));
+    }
+  } else if (!fp32) {
+    if (p->keo_grid) {
+      rc = keo_grid_hpsi(p, nvec, dpsi, dh);
+      if (rc) return rc;
+    } else {
+      double *X = Xbuf, *R = Rbuf;
+      hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                         p->nb_packed, nvec, X);
+      HIP_TRY(hipGetLastError());
+      rc = run_terms<double>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw64.ptr,
+                             p->d_keo64.ptr, p->d_Veff.ptr);
+      if (rc) return rc;
+      HIP_TRY(launch_reduce<double>(p, R, nvec, dh, Pbuf));
+    }
+  } else {
+    rc = ensure_v32(p);
+    if (rc) return rc;
+    float *X = reinterpret_cast<float *>(Xbuf), *R = reinterpret_cast<float *>(Rbuf);
+    hipLaunchKernelGGL(evr::k_pack_in<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
+                       p->nb_packed, nvec, X);
+    HIP_TRY(hipGetLastError());
+    rc = run_terms<float>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw32.ptr,
+                          p->d_keo32.ptr, p->d_Veff32.ptr);
+    if (rc) return rc;
+    HIP_TRY(launch_reduce<float>(p, R, nvec, dh, Pbuf));
+  }
+  return EVR_OK;
+}
+
+extern "C" {
+
 int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
 {
   if (!p || !psi || !hpsi) return fail(EVR_ERR_ARG, "NULL argument");
   if (nvec < 1 || nvec > p->max_nvec) return fail(EVR_ERR_ARG, "nvec must be in 1..max_nvec");
   const bool dev = (flags & EVR_DEVICE_PTRS) != 0;
-  const bool fp32 = (flags & EVR_FP32) != 0;
   const int64_t n = p->nb_packed * nvec;
   const double *dpsi = psi;
   double *dh = hpsi;
@@ -1458,53 +1631,8 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
     dpsi = p->d_psi.ptr;
     dh = p->d_hpsi.ptr;
   }
-  int rc;
-  if (p->keo_grid && fp32) return fail(EVR_ERR_ARG, "a plan with G, J, s per grid point is FP64 only");
-  const bool use_asm = !(flags & EVR_MATRIX_FREE) &&
-                       (fp32 ? p->d_A32.ptr != nullptr : p->d_A64.ptr != nullptr) &&
-                       !((flags & EVR_AUTO_PATH) && matrix_free_cheaper(p, nvec, fp32));
-  if (use_asm) {
-    if (fp32) {
-      float *X = reinterpret_cast<float *>(p->d_X.ptr), *R = reinterpret_cast<float *>(p->d_R.ptr);
-      hipLaunchKernelGGL(evr::k_pack_in<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
-                         p->nb_packed, nvec, X);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(apply_assembled<float>(p, p->d_A32.ptr, X, nvec, R));
-      HIP_TRY(launch_reduce<float>(p, R, nvec, dh));
-    } else {
-      double *X = p->d_X.ptr, *R = p->d_R.ptr;
-      hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
-                         p->nb_packed, nvec, X);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(apply_assembled<double>(p, p->d_A64.ptr, X, nvec, R));
-      HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
-    }
-  } else if (!fp32) {
-    if (p->keo_grid) {
-      rc = keo_grid_hpsi(p, nvec, dpsi, dh);
-      if (rc) return rc;
-    } else {
-      double *X = p->d_X.ptr, *R = p->d_R.ptr;
-      hipLaunchKernelGGL(evr::k_pack_in<double>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
-                         p->nb_packed, nvec, X);
-      HIP_TRY(hipGetLastError());
-      rc = run_terms<double>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw64.ptr,
-                             p->d_keo64.ptr, p->d_Veff.ptr);
-      if (rc) return rc;
-      HIP_TRY(launch_reduce<double>(p, R, nvec, dh));
-    }
-  } else {
-    rc = ensure_v32(p);
-    if (rc) return rc;
-    float *X = reinterpret_cast<float *>(p->d_X.ptr), *R = reinterpret_cast<float *>(p->d_R.ptr);
-    hipLaunchKernelGGL(evr::k_pack_in<float>, dim3(blocks_for(n)), dim3(256), 0, p->stream, dpsi,
-                       p->nb_packed, nvec, X);
-    HIP_TRY(hipGetLastError());
-    rc = run_terms<float>(p, evr::kModeHpsi, nvec, X, R, nullptr, nullptr, p->d_bw32.ptr,
-                          p->d_keo32.ptr, p->d_Veff32.ptr);
-    if (rc) return rc;
-    HIP_TRY(launch_reduce<float>(p, R, nvec, dh));
-  }
+  const int rc = oppsi_core(p, nvec, dpsi, dh, flags, p->d_X.ptr, p->d_R.ptr, p->d_P.ptr);
+  if (rc) return rc;
   if (!dev) {
     HIP_TRY(hipMemcpyAsync(hpsi, dh, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1586,6 +1714,109 @@ int evr_plan_diagonal(evr_plan *p, double *diag, int flags)
                            p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   return EVR_OK;
+}
+
+// ncol columns through the B->G mode, the weighted G->B mode or a grid
+// derivative, device pointers on the plan stream (DP plans: the reference's
+// grid order through k_perm_cols and a work grid).
+static int transform_cols(evr_plan *p, int what, int ncol, const double *in, double *out, int di, int dj)
+{
+  const int64_t nq = p->sum_nq, nb = p->nb_packed;
+  DevBuf<double> tmp;
+  if (p->dp_order && what != 0) HIP_TRY(tmp.alloc((size_t)nq * std::min(ncol, p->max_nvec)));
+  for (int c0 = 0; c0 < ncol; c0 += p->max_nvec) {
+    const int k = std::min(p->max_nvec, ncol - c0);
+    const unsigned gb = blocks_for(nq * k);
+    if (what == 0) {  // basis -> grid
+      double *G = out + nq * c0;
+      int rc = evr::plan_b2g_device(p, k, in + nb * c0, p->dp_order ? p->d_G.ptr : G);
+      if (rc) return rc;
+      if (p->dp_order) {
+        hipLaunchKernelGGL(evr::k_perm_cols, dim3(gb), dim3(256), 0, p->stream, nq, k, p->d_dp_perm.ptr,
+                           p->d_G.ptr, G, 1);
+        HIP_TRY(hipGetLastError());
+      }
+      continue;
+    }
+    const double *Gin = in + nq * c0;
+    if (p->dp_order) {
+      hipLaunchKernelGGL(evr::k_perm_cols, dim3(gb), dim3(256), 0, p->stream, nq, k, p->d_dp_perm.ptr, Gin,
+                         tmp.ptr, 0);
+      HIP_TRY(hipGetLastError());
+      Gin = tmp.ptr;
+    }
+    if (what == 1) {  // grid -> basis, weighted Smolyak sum
+      int rc = run_terms<double>(p, evr::kModeG2B, k, nullptr, p->d_R.ptr, nullptr, Gin, p->d_bw64.ptr,
+                                 p->d_keo64.ptr, p->d_V.ptr);
+      if (rc) return rc;
+      HIP_TRY(launch_reduce<double>(p, p->d_R.ptr, k, out + nb * c0));
+    } else {  // derivative on the grids
+      double *dst = p->dp_order ? p->d_G.ptr : out + nq * c0;
+      hipLaunchKernelGGL(evr::k_grid_deriv, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream,
+                         p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_D2.ptr, p->d_gitems.ptr, di, dj,
+                         Gin, nq, k, dst);
+      HIP_TRY(hipGetLastError());
+      if (p->dp_order) {
+        hipLaunchKernelGGL(evr::k_perm_cols, dim3(gb), dim3(256), 0, p->stream, nq, k, p->d_dp_perm.ptr,
+                           p->d_G.ptr, out + nq * c0, 1);
+        HIP_TRY(hipGetLastError());
+      }
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(p->stream));  // tmp is released on return
+  return EVR_OK;
+}
+
+// Host or device columns for transform_cols: host arrays go through device
+// copies made for the call.
+static int transform_entry(evr_plan *p, int what, int ncol, const double *in, double *out, int flags, int di,
+                           int dj)
+{
+  if (!p || !in || !out) return fail(EVR_ERR_ARG, "NULL argument");
+  if (ncol < 1) return fail(EVR_ERR_ARG, "ncol must be >= 1");
+  if (p->D != p->D_full && what == 2) return fail(EVR_ERR_ARG, "grid derivatives need D <= 12");
+  const int64_t nq = p->sum_nq, nb = p->nb_packed;
+  if (p->dp_order) {
+    const int64_t perm_cols = std::min(ncol, p->max_nvec);
+    if (!p->d_G.ptr || p->d_G.n < (size_t)(nq * perm_cols)) HIP_TRY(p->d_G.alloc((size_t)(nq * perm_cols)));
+  }
+  if (what == 2) {
+    int rc = grid_items(p);
+    if (rc) return rc;
+  }
+  const int64_t nin = (what == 0 ? nb : nq) * ncol, nout = (what == 1 ? nb : nq) * ncol;
+  if (flags & EVR_DEVICE_PTRS) return transform_cols(p, what, ncol, in, out, di, dj);
+  DevBuf<double> din, dout;
+  HIP_TRY(din.alloc((size_t)nin));
+  HIP_TRY(dout.alloc((size_t)nout));
+  HIP_TRY(hipMemcpyAsync(din.ptr, in, sizeof(double) * nin, hipMemcpyHostToDevice, p->stream));
+  int rc = transform_cols(p, what, ncol, din.ptr, dout.ptr, di, dj);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, dout.ptr, sizeof(double) * nout, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return EVR_OK;
+}
+
+int evr_basis_to_grid_n(evr_plan *p, int ncol, const double *psiB, double *psiG, int flags)
+{
+  return transform_entry(p, 0, ncol, psiB, psiG, flags, -1, -1);
+}
+
+int evr_grid_to_basis_n(evr_plan *p, int ncol, const double *psiG, double *psiB, int flags)
+{
+  return transform_entry(p, 1, ncol, psiG, psiB, flags, -1, -1);
+}
+
+int evr_grid_deriv(evr_plan *p, int ncol, const int *tab_der, const double *psiG, double *out, int flags)
+{
+  if (!tab_der) return fail(EVR_ERR_ARG, "NULL argument");
+  const int D = p ? p->D_full : 0;
+  int d[2];
+  for (int k = 0; k < 2; ++k) {
+    if (tab_der[k] > D) return fail(EVR_ERR_ARG, "tab_der entries must be <= D");
+    d[k] = tab_der[k] > 0 ? tab_der[k] - 1 : -1;  // tab_der < 0 counts as 0 (BG:1250)
+  }
+  return transform_entry(p, 2, ncol, psiG, out, flags, d[0], d[1]);
 }
 
 int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB)

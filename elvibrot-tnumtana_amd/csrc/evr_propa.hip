@@ -836,12 +836,177 @@ extern "C" int evr_coupled_propagate_field(evr_coupled *C, evr_coupled *mu, cons
   return propagate_impl(OpRef(C), OpRef(mu), o, nwp, psi, autocorr, info);
 }
 
+// Time-dependent field integrators of the reference: march_noD_field (Taylor
+// series of the time-dependent equation with the field derivatives) and
+// march_RK4_field (classical Runge-Kutta on fcn_field), on the batch.
+static int propagate_field_impl(OpRef H, OpRef mu, const evr_propa_opts *o, int nwp, double *psi,
+                                double *autocorr, evr_propa_info *info)
+{
+  const bool taylor = o->method == EVR_PROPA_TAYLOR;
+  if (o->nsteps < 0 || !(o->dt > 0.0)) return set_error(EVR_ERR_ARG, "bad dt or nsteps");
+  if (o->field) return set_error(EVR_ERR_ARG, "TAYLOR and RK4 take the field in field_sub (field must be NULL)");
+  if ((o->field_sub != nullptr) != (bool)mu)
+    return set_error(EVR_ERR_ARG, "field_sub and the coupling plan go together");
+  if (o->field_sub && (taylor ? o->nsub < 1 : o->nsub != 3))
+    return set_error(EVR_ERR_ARG, taylor ? "Taylor needs nsub >= 1 field derivatives"
+                                         : "RK4 needs nsub = 3 field values per step");
+  if (taylor && (o->npoly < 2 || o->npoly > 64)) return set_error(EVR_ERR_ARG, "Taylor needs 2 <= npoly <= 64");
+  const auto t_start = std::chrono::steady_clock::now();
+  Batch P;
+  RTRY(P.init(H, mu, nwp, o->precision == EVR_FP32 ? EVR_FP32 : EVR_FP64));
+  const int64_t nb = P.nb, nblk = nb * 2 * nwp;
+  const double dt = o->dt;
+  const hipMemcpyKind h2d = o->device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  // E0: the centre of the spectral interval (the reference's para_H%E0 shift)
+  double hmin = o->Hmin, hmax = o->Hmax;
+  if (!(hmin < hmax)) RTRY(lanczos_bounds(H, 60, &hmin, &hmax));
+  const double e0 = 0.5 * (hmin + hmax);
+  const int nw = taylor ? o->npoly : 5;   // work vectors
+  DBuf PSI, PSI0, W, HM;
+  HTRY(PSI.alloc(nblk));
+  HTRY(W.alloc(nblk * nw));
+  HTRY(HM.alloc(2 * nblk));               // H x and mu x, contiguous
+  double *HW = HM.p, *MW = HM.p + nblk;
+  HTRY(hipMemcpyAsync(PSI.p, psi, sizeof(double) * nblk, h2d, P.st));
+  if (autocorr) {
+    HTRY(PSI0.alloc(nblk));
+    HTRY(hipMemcpyAsync(PSI0.p, o->psi0 ? o->psi0 : psi, sizeof(double) * nblk, h2d, P.st));
+  }
+  HTRY(hipStreamSynchronize(P.st));
+  const std::vector<cplx> one(nwp, 1.0), zero(nwp, 0.0), me0(nwp, -e0);
+  const std::vector<cplx> phase(nwp, std::exp(cplx(0.0, -e0 * dt)));
+  std::vector<double> E(nwp, 0.0);
+  int max_order = 0;
+  // y = -i [(H - E0) x - E mu x]  (fcn_field), the field values E already set
+  auto fcn = [&](const double *x, double *y, bool field) -> int {
+    RTRY(P.apply(H, x, HW, P.flags));
+    if (field) {
+      RTRY(P.apply(mu, x, MW, P.flags));
+      RTRY(launch_field(P, P.dE.p, MW, HW));
+    }
+    RTRY(P.comb(x, 0, 1, me0, &one, HW));
+    return P.comb(HW, 0, 1, std::vector<cplx>(nwp, cplx(0.0, -1.0)), nullptr, y);
+  };
+  for (int step = 0; step < o->nsteps; ++step) {
+    const double *fs = o->field_sub ? o->field_sub + (int64_t)step * o->nsub * nwp : nullptr;
+    if (taylor) {
+      // ---- march_noD_field (MARCH:1635-1836): w_0 = psi, psi accumulates w_i
+      int max_der = 0;
+      std::vector<double> fk;   // fk[k*nwp + w] = -E^(k)(t_n)
+      if (fs) {
+        const int nb_der = std::min(o->nsub - 1, o->npoly - 1);
+        fk.assign((size_t)(nb_der + 1) * nwp, 0.0);
+        double fac = dt;
+        for (int k = 0; k <= nb_der; ++k) {
+          if (k > 0) fac *= dt / k;
+          double mx = 0.0;
+          for (int w = 0; w < nwp; ++w) {
+            fk[(size_t)k * nwp + w] = -fs[(size_t)k * nwp + w];
+            mx = std::max(mx, std::fabs(fs[(size_t)k * nwp + w]));
+          }
+          if (k > 0 && mx * fac > o->poly_tol) max_der = k;
+        }
+      }
+      // the field term is skipped when every derivative is below poly_tol
+      // (MARCH:1740-1742)
+      bool field_on = false;
+      for (double v : fk) field_on |= std::fabs(v) >= o->poly_tol;
+      HTRY(hipMemcpyAsync(W.p, PSI.p, sizeof(double) * nblk, hipMemcpyDeviceToDevice, P.st));
+      int order = 0;
+      for (int i = 0; i + 1 < o->npoly; ++i) {
+        const double *wi = W.p + nblk * i;
+        double *wn = W.p + nblk * (i + 1);
+        RTRY(P.apply(H, wi, HW, P.flags));
+        RTRY(P.comb(wi, 0, 1, me0, &one, HW));                 // (H - E0) w_i
+        if (field_on) {
+          // s = sum_{k <= min(i, max_der)} (dt^k/k!) (-E^(k)) w_{i-k}; HW += mu s
+          const int K = std::min(i, max_der);
+          std::vector<cplx> c((size_t)(K + 1) * nwp);
+          double f = 1.0;
+          for (int k = 0; k <= K; ++k) {
+            if (k > 0) f *= dt / k;
+            for (int w = 0; w < nwp; ++w) c[(size_t)(K - k) * nwp + w] = f * fk[(size_t)k * nwp + w];
+          }
+          RTRY(P.comb(W.p + nblk * (i - K), nblk, K + 1, c, nullptr, wn));
+          RTRY(P.apply(mu, wn, MW, P.flags));
+          RTRY(P.comb(MW, 0, 1, one, &one, HW));
+        }
+        // w_{i+1} = (-i dt/(i+1)) HW;  psi += w_{i+1}
+        RTRY(P.comb(HW, 0, 1, std::vector<cplx>(nwp, cplx(0.0, -dt / (i + 1))), nullptr, wn));
+        RTRY(P.comb(wn, 0, 1, one, &one, PSI.p));
+        order = i + 1;
+        std::vector<cplx> nn;
+        RTRY(P.dots(wn, 0, 1, wn, nn));
+        bool done = true;
+        for (int w = 0; w < nwp; ++w) {
+          if (!(nn[w].real() < 1e15)) return set_error(EVR_ERR_STATE, "Taylor series diverges (dt too large)");
+          if (nn[w].real() >= o->poly_tol) done = false;
+        }
+        if (done) break;
+      }
+      max_order = std::max(max_order, order);
+      RTRY(P.scale(phase, PSI.p));
+    } else {
+      // ---- march_RK4_field (MARCH:392-543)
+      double *w1 = W.p, *w2 = W.p + nblk, *w3 = W.p + 2 * nblk, *w4 = W.p + 3 * nblk, *w5 = W.p + 4 * nblk;
+      auto set_E = [&](int j) -> int {
+        if (!fs) return EVR_OK;
+        for (int w = 0; w < nwp; ++w) E[w] = fs[(size_t)j * nwp + w];
+        return P.set_field(E.data());
+      };
+      const std::vector<cplx> h2(nwp, 0.5 * dt), h1(nwp, dt), two(nwp, 2.0), h6(nwp, dt / 6.0);
+      RTRY(set_E(0));
+      RTRY(fcn(PSI.p, w1, fs != nullptr));                                   // w1 = f(t, psi)
+      RTRY(P.comb(w1, 0, 1, h2, nullptr, w2));
+      RTRY(P.comb(PSI.p, 0, 1, one, &one, w2));                               // w2 = psi + w1 dt/2
+      RTRY(set_E(1));
+      RTRY(fcn(w2, w3, fs != nullptr));                                       // w3 = f(t + dt/2, w2)
+      RTRY(P.comb(w3, 0, 1, h2, nullptr, w2));
+      RTRY(P.comb(PSI.p, 0, 1, one, &one, w2));                               // w2 = psi + w3 dt/2
+      RTRY(fcn(w2, w4, fs != nullptr));                                       // w4 = f(t + dt/2, w2)
+      RTRY(P.comb(w4, 0, 1, h1, nullptr, w2));
+      RTRY(P.comb(PSI.p, 0, 1, one, &one, w2));                               // w2 = psi + w4 dt
+      RTRY(P.comb(w3, 0, 1, one, &one, w4));                                  // w4 = w3 + w4
+      RTRY(set_E(2));
+      RTRY(fcn(w2, w3, fs != nullptr));                                       // w3 = f(t + dt, w2)
+      RTRY(P.comb(w1, 0, 1, one, nullptr, w5));
+      RTRY(P.comb(w3, 0, 1, one, &one, w5));
+      RTRY(P.comb(w4, 0, 1, two, &one, w5));                                  // w5 = w1 + w3 + 2 w4
+      RTRY(P.comb(w5, 0, 1, h6, &one, PSI.p));                                // psi += w5 dt/6
+      RTRY(P.scale(phase, PSI.p));
+      max_order = 4;
+    }
+    if (autocorr) {
+      std::vector<cplx> ac;
+      RTRY(P.dots(PSI0.p, 0, 1, PSI.p, ac));
+      for (int w = 0; w < nwp; ++w) {
+        autocorr[2 * ((int64_t)step * nwp + w)] = ac[w].real();
+        autocorr[2 * ((int64_t)step * nwp + w) + 1] = ac[w].imag();
+      }
+    }
+  }
+  HTRY(hipMemcpyAsync(psi, PSI.p, sizeof(double) * nblk,
+                      o->device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, P.st));
+  HTRY(hipStreamSynchronize(P.st));
+  if (info) {
+    info->hpsi = P.hpsi;
+    info->max_order = max_order;
+    info->retries = 0;
+    info->Hmin = hmin;
+    info->Hmax = hmax;
+    info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return EVR_OK;
+}
+
 static int propagate_impl(OpRef H, OpRef mu, const evr_propa_opts *o, int nwp, double *psi, double *autocorr,
                           evr_propa_info *info)
 {
   if (!H || !o || !psi || nwp < 1) return set_error(EVR_ERR_ARG, "bad argument");
+  if (o->method == EVR_PROPA_TAYLOR || o->method == EVR_PROPA_RK4)
+    return propagate_field_impl(H, mu, o, nwp, psi, autocorr, info);
   if (o->method != EVR_PROPA_CHEBY && o->method != EVR_PROPA_SIL)
-    return set_error(EVR_ERR_ARG, "method must be EVR_PROPA_CHEBY or EVR_PROPA_SIL");
+    return set_error(EVR_ERR_ARG, "method must be EVR_PROPA_CHEBY, _SIL, _TAYLOR or _RK4");
   if (o->method == EVR_PROPA_CHEBY && o->npoly < 3)
     return set_error(EVR_ERR_ARG, "Chebyshev needs npoly >= 3");
   if (o->method == EVR_PROPA_SIL && (o->npoly < 2 || o->npoly > 63))

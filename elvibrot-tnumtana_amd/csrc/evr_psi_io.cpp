@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <iterator>
 #include <map>
@@ -164,10 +165,10 @@ std::string lower(std::string s)
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int evr_psi_save(const evr_basis_desc *bd, int nb0, int nwp, int cplx, const double *psi, int version,
-                 int formatted, const char *path)
+int psi_save(const evr_basis_desc *bd, int nb0, int nwp, int cplx, const double *psi, int version,
+             int formatted, const char *path)
 {
   if (!bd || !psi || !path || nb0 < 1 || nwp < 1) return fail_io(EVR_ERR_ARG, "bad argument");
   if (version != 0 && version != 1) return fail_io(EVR_ERR_ARG, "version must be 0 or 1");
@@ -242,8 +243,8 @@ int evr_psi_save(const evr_basis_desc *bd, int nb0, int nwp, int cplx, const dou
   return EVR_OK;
 }
 
-int evr_psi_read(const evr_basis_desc *bd, int nb0, int cplx, int max_wp, double *psi, int *nwp,
-                 const char *path)
+int psi_read(const evr_basis_desc *bd, int nb0, int cplx, int max_wp, double *psi, int *nwp,
+             const char *path)
 {
   if (!bd || !psi || !path || !nwp || nb0 < 1 || max_wp < 1) return fail_io(EVR_ERR_ARG, "bad argument");
   *nwp = 0;
@@ -289,6 +290,14 @@ int evr_psi_read(const evr_basis_desc *bd, int nb0, int cplx, int max_wp, double
       if (h[2] != 1) return fail_io(EVR_ERR_IO, "unformatted wavepacket files are version 1");
       const int npsi = h[0];
       const int64_t ftot = h[1];
+      // counts from the file are checked before anything is sized from them:
+      // every index record takes 4 (D + 2) + 8 bytes of the file
+      const std::streampos here = f.tellg();
+      f.seekg(0, std::ios::end);
+      const int64_t fsize = (int64_t)f.tellg();
+      f.seekg(here);
+      if (npsi < 0 || ftot < 0 || ftot > fsize / (4 * (D + 2) + 8))
+        return fail_io(EVR_ERR_IO, "header counts (nb_psi, nb_tot) do not fit the file");
       std::vector<int64_t> rows(ftot);
       std::vector<int> key(D + 2);
       for (int64_t i = 0; i < ftot; ++i) {
@@ -333,6 +342,9 @@ int evr_psi_read(const evr_basis_desc *bd, int nb0, int cplx, int max_wp, double
       else if (k == "nb_tot") ftot = v;
     }
     if (version != 1) return fail_io(EVR_ERR_IO, "namelist header with Version_File /= 1");
+    // every index line holds D + 2 integers (at least 2 characters each)
+    if (npsi < 0 || ftot < 0 || ftot > (long)(text.size() / (2 * (D + 2)) + 1))
+      return fail_io(EVR_ERR_IO, "header counts (nb_psi, nb_tot) do not fit the file");
     std::vector<int64_t> rows(ftot);
     std::vector<int> key(D + 2);
     for (long i = 0; i < ftot; ++i) {
@@ -376,6 +388,89 @@ int evr_psi_read(const evr_basis_desc *bd, int nb0, int cplx, int max_wp, double
   }
   *nwp = (int)std::min<long>(w, max_wp);
   return EVR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// The entry points return a status for any failure, including an allocation
+// that a malformed file or an oversized basis makes fail.
+int evr_psi_save(const evr_basis_desc *bd, int nb0, int nwp, int cplx, const double *psi, int version,
+                 int formatted, const char *path)
+{
+  try {
+    return psi_save(bd, nb0, nwp, cplx, psi, version, formatted, path);
+  } catch (const std::exception &e) {
+    return fail_io(EVR_ERR_IO, std::string("evr_psi_save: ") + e.what());
+  }
+}
+
+int evr_propa_save_restart(double T, int nwp, int64_t size_wp, const double *psi, const char *path)
+{
+  if (!psi || !path || nwp < 1 || size_wp < 1) return fail_io(EVR_ERR_ARG, "bad argument");
+  try {
+    std::ofstream f(path);
+    if (!f) return fail_io(EVR_ERR_IO, std::string("cannot open ") + path);
+    // write(no_restart,*) T, size(WP), size(WP(1)%CvecB), then each CvecB
+    f << " " << fmt_real(T) << " " << nwp << " " << size_wp << "\n";
+    for (int w = 0; w < nwp; ++w) {
+      const double *re = psi + (size_t)(2 * w) * size_wp, *im = psi + (size_t)(2 * w + 1) * size_wp;
+      for (int64_t i = 0; i < size_wp; ++i) f << " (" << fmt_real(re[i]) << "," << fmt_real(im[i]) << ")\n";
+    }
+    if (!f) return fail_io(EVR_ERR_IO, std::string("write failed: ") + path);
+    return EVR_OK;
+  } catch (const std::exception &e) {
+    return fail_io(EVR_ERR_IO, std::string("evr_propa_save_restart: ") + e.what());
+  }
+}
+
+int evr_propa_read_restart(const char *path, int nwp, int64_t size_wp, double *psi, double *T, int *found)
+{
+  if (!path || !psi || !T || !found || nwp < 1 || size_wp < 1) return fail_io(EVR_ERR_ARG, "bad argument");
+  try {
+    *T = 0.0;
+    *found = 0;
+    std::ifstream f(path);
+    if (!f) return fail_io(EVR_ERR_IO, std::string("cannot open ") + path);
+    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<double> v;
+    // read(no_restart,*,IOSTAT=err_read) T, nb_WP_file, size_WP_file: a file
+    // whose first record does not hold them means no restart (T0 = 0)
+    const size_t eol = text.find('\n');
+    if (!lexed_numbers(text.substr(0, eol), v) || v.size() < 3) return EVR_OK;
+    const double t0 = v[0];
+    const double nwf = v[1], szf = v[2];
+    if (nwf != (double)nwp || szf != (double)size_wp)
+      return fail_io(EVR_ERR_IO, "restart file holds " + std::to_string((long long)nwf) + " wavepackets of " +
+                                     std::to_string((long long)szf) + " values, not " + std::to_string(nwp) +
+                                     " of " + std::to_string((long long)size_wp));
+    v.clear();
+    if (eol == std::string::npos || !lexed_numbers(text.substr(eol + 1), v))
+      return fail_io(EVR_ERR_IO, "bad wavepacket values in the restart file");
+    if (v.size() < (size_t)(2 * nwp) * size_wp) return fail_io(EVR_ERR_IO, "restart file too short");
+    for (int w = 0; w < nwp; ++w)
+      for (int64_t i = 0; i < size_wp; ++i) {
+        psi[(size_t)(2 * w) * size_wp + i] = v[2 * ((size_t)w * size_wp + i)];
+        psi[(size_t)(2 * w + 1) * size_wp + i] = v[2 * ((size_t)w * size_wp + i) + 1];
+      }
+    *T = t0;
+    *found = 1;
+    return EVR_OK;
+  } catch (const std::exception &e) {
+    return fail_io(EVR_ERR_IO, std::string("evr_propa_read_restart: ") + e.what());
+  }
+}
+
+int evr_psi_read(const evr_basis_desc *bd, int nb0, int cplx, int max_wp, double *psi, int *nwp,
+                 const char *path)
+{
+  try {
+    return psi_read(bd, nb0, cplx, max_wp, psi, nwp, path);
+  } catch (const std::exception &e) {
+    if (nwp) *nwp = 0;
+    return fail_io(EVR_ERR_IO, std::string("evr_psi_read: ") + e.what());
+  }
 }
 
 }  // extern "C"

@@ -802,12 +802,12 @@ __device__ __forceinline__ void asm_tiles(const T *__restrict__ At, const int *_
   for (int m = 0; m < MA; ++m)
 #pragma unroll
     for (int n = 0; n < NTILE; ++n) acc[m][n] = acc_t{T(0), T(0), T(0), T(0)};
-  // X rows [kc0, kc0 + kAsmKc) into LDS.  Rows past nb_t and pruned rows
-  // meet zero columns of the assembled matrix (padding, and pruned functions
-  // are assembled as zero columns), and vectors past nvec are not stored, so
-  // every read goes to a clamped valid address with no test: all RPT reads of
-  // a thread are in flight together.  (A non-finite value in row 0 of the
-  // block would reach the terms with pruned rows through 0 x inf.)
+  // X rows [kc0, kc0 + kAsmKc) into LDS.  Every read goes to a clamped valid
+  // address with no branch, so all RPT reads of a thread are in flight
+  // together; rows past nb_t and pruned rows are then staged as zeros (they
+  // meet zero columns of the assembled matrix, and a zero keeps a non-finite
+  // value elsewhere in the block out of this term), and vectors past nvec
+  // are not stored.
   const int sv = threadIdx.x % NV, sk = threadIdx.x / NV;
   const int vcol = vbase + sv < nvec ? vbase + sv : nvec - 1;
   auto stage = [&](int kc0) {
@@ -816,11 +816,15 @@ __device__ __forceinline__ void asm_tiles(const T *__restrict__ At, const int *_
 #pragma unroll
     for (int j = 0; j < RPT; ++j) {
       const int kg = kc0 + sk + KST * j;
-      pk[j] = mp[kg < nbt ? kg : nbt - 1];
+      const int mk = mp[kg < nbt ? kg : nbt - 1];
+      pk[j] = kg < nbt ? mk : -1;
     }
     T val[RPT];
 #pragma unroll
-    for (int j = 0; j < RPT; ++j) val[j] = X[(int64_t)(pk[j] >= 0 ? pk[j] : 0) * nvec + vcol];
+    for (int j = 0; j < RPT; ++j) {
+      const T x = X[(int64_t)(pk[j] >= 0 ? pk[j] : 0) * nvec + vcol];
+      val[j] = pk[j] >= 0 ? x : T(0);
+    }
 #pragma unroll
     for (int j = 0; j < RPT; ++j) xs[(sk + KST * j) * LD + sv] = val[j];
     __syncthreads();
@@ -1222,6 +1226,125 @@ __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict
       st *= nq;
     }
     F[v * sum_nq + g] = scale * o + V[g] * G[v * sum_nq + g];
+  }
+}
+
+// ------------------------------------------- type 1 with F2, F1 per point --
+// The reference's type-1 grid combine of one term (OPSG4:1490-1546) with its
+// operator grids per point (get_OpGrid_type1_OF_ONEDP_FOR_SG4,
+// OPSG4:2820-3084):  out = V psi_G + sum_i F1(:, i) d_i psi_G
+//   + sum_i F2(:, i, i) d2_i psi_G + sum_{i != j} F2(:, i, j) d_i (d_j psi_G),
+// d = dnRGG%d1, d2 = dnRGG%d2 along one mode of the term grid; P holds
+// d_j psi_G (k_gk_deriv without a scale).  F2 is (sum_nq, D, D) Fortran,
+// F1 (sum_nq, D) or NULL.  One thread per grid point.
+__global__ __launch_bounds__(256) void k_t1_out(int D, const TermDev *__restrict__ terms,
+                                                const SlotDev *__restrict__ slots, const double *__restrict__ D1,
+                                                const double *__restrict__ D2, const int2 *__restrict__ items,
+                                                const double *__restrict__ F2, const double *__restrict__ F1,
+                                                const double *__restrict__ V, const double *__restrict__ G,
+                                                const double *__restrict__ P, int64_t sum_nq, int k,
+                                                double *__restrict__ F)
+{
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int p = it.y + threadIdx.x;
+  if (p >= td.nqt) return;
+  const int64_t g = td.off_nq + p;
+  int qd[kMaxDev];
+  int64_t st[kMaxDev];
+  int64_t s = 1;
+  for (int d = 0; d < D; ++d) {
+    const int nq = slots[td.slot[d]].nq;
+    qd[d] = (int)((p / s) % nq);
+    st[d] = s;
+    s *= nq;
+  }
+  for (int v = 0; v < k; ++v) {
+    const double *Gv = G + (int64_t)v * sum_nq;
+    double o = V[g] * Gv[g];
+    for (int i = 0; i < D; ++i) {
+      const SlotDev &sl = slots[td.slot[i]];
+      const int64_t p0 = td.off_nq + p - qd[i] * st[i];
+      if (F1) o += F1[g + sum_nq * i] * P[((int64_t)v * D + i) * sum_nq + g];
+      const double fii = F2[g + sum_nq * (i + (int64_t)D * i)];
+      if (fii != 0.0) o += fii * grid_d1(D2 + sl.offD1, sl.nq, qd[i], st[i], Gv, nullptr, p0);
+      for (int j = 0; j < D; ++j) {
+        if (j == i) continue;
+        const double fij = F2[g + sum_nq * (i + (int64_t)D * j)];
+        if (fij != 0.0)
+          o += fij * grid_d1(D1 + sl.offD1, sl.nq, qd[i], st[i], P + ((int64_t)v * D + j) * sum_nq, nullptr, p0);
+      }
+    }
+    F[(int64_t)v * sum_nq + g] = o;
+  }
+}
+
+// ----------------------------------------------------- grid derivative --
+// DerivOp_TO_RDP_OF_SmolaykRep (BGSG4:2690-2795) on every term grid, ncol
+// columns of the concatenated term grids: i == j >= 0: dnRGG%d2 along mode
+// i; one of them < 0: dnRGG%d1 along the other; i != j both >= 0: d1 along
+// mode j, then along mode i; both < 0: a copy.  One thread per grid point.
+__global__ __launch_bounds__(256) void k_grid_deriv(const TermDev *__restrict__ terms,
+                                                    const SlotDev *__restrict__ slots,
+                                                    const double *__restrict__ D1, const double *__restrict__ D2,
+                                                    const int2 *__restrict__ items, int di, int dj,
+                                                    const double *__restrict__ in, int64_t sum_nq, int ncol,
+                                                    double *__restrict__ out)
+{
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int p = it.y + threadIdx.x;
+  if (p >= td.nqt) return;
+  const int64_t g = td.off_nq + p;
+  // stride and coordinate of modes di, dj at point p (first mode fastest)
+  int64_t st = 1, sti = 1, stj = 1;
+  int qi = 0, qj = 0, nqi = 1, nqj = 1;
+  int64_t oi = 0, oj = 0;
+  for (int d = 0; d <= (di > dj ? di : dj); ++d) {
+    const SlotDev &sl = slots[td.slot[d]];
+    const int q = (int)((p / st) % sl.nq);
+    if (d == di) { sti = st; qi = q; nqi = sl.nq; oi = sl.offD1; }
+    if (d == dj) { stj = st; qj = q; nqj = sl.nq; oj = sl.offD1; }
+    st *= sl.nq;
+  }
+  for (int v = 0; v < ncol; ++v) {
+    const double *x = in + (int64_t)v * sum_nq;
+    double r;
+    if (di < 0 && dj < 0) {
+      r = x[g];
+    } else if (di == dj || di < 0 || dj < 0) {
+      const bool two = di == dj;
+      const int m = di >= 0 ? di : dj;
+      const int64_t s = m == di ? sti : stj;
+      const int q = m == di ? qi : qj, n = m == di ? nqi : nqj;
+      const double *M = (two ? D2 : D1) + (m == di ? oi : oj);
+      r = grid_d1(M, n, q, s, x, nullptr, g - q * s);
+    } else {
+      r = 0.0;
+      const int64_t p00 = g - qi * sti - qj * stj;
+      for (int a = 0; a < nqi; ++a) {
+        double inner = 0.0;
+        for (int b = 0; b < nqj; ++b) inner += D1[oj + qj + nqj * b] * x[p00 + a * sti + b * stj];
+        r += D1[oi + qi + nqi * a] * inner;
+      }
+    }
+    out[(int64_t)v * sum_nq + g] = r;
+  }
+}
+
+// Reference direct-product grid order (last mode fastest) <-> term-grid order
+// (first mode fastest) for ncol columns: to_user: out[perm[i]] = in[i], else
+// out[i] = in[perm[i]].
+__global__ __launch_bounds__(256) void k_perm_cols(int64_t n, int ncol, const int *__restrict__ perm,
+                                                   const double *__restrict__ in, double *__restrict__ out,
+                                                   int to_user)
+{
+  const int64_t tot = n * ncol;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < tot;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k % n, c = k / n;
+    if (to_user) out[perm[i] + c * n] = in[k];
+    else out[k] = in[perm[i] + c * n];
   }
 }
 

@@ -28,6 +28,8 @@ ASM_FP64 = 1
 ASM_FP32 = 2
 CHEBY = 1
 SIL = 2
+TAYLOR = 3
+RK4 = 4
 
 
 def _to_pairs(psi):
@@ -53,6 +55,39 @@ def control_field(amp, T, nsteps):
     out = np.zeros((nsteps, nf))
     check(load().evr_control_field(ctypes.byref(o), nf, _d(amp), _d(out)))
     return out
+
+
+def control_field_sub(amp, T, nsteps, method, nsub):
+    """The control fields sampled for the TAYLOR (derivatives at t_n) or RK4
+    (t_n, t_n + dt/2, t_n + dt) integrators: (nsteps, nsub, nf)."""
+    amp = np.asfortranarray(amp, dtype=np.float64)
+    if amp.ndim == 1:
+        amp = amp.reshape(-1, 1, order="F")
+    nk, nf = amp.shape
+    o = ControlOpts(T, nsteps, nk, 0, 3, 1e-12, 0.0, 0.0)
+    out = np.zeros((nsteps, nsub, nf))
+    check(load().evr_control_field_sub(ctypes.byref(o), nf, _d(amp), method, nsub, _d(out)))
+    return out
+
+
+def save_restart(path, T, psi):
+    """SaveWP_restart: time T and complex wavepackets psi (size_WP,) or
+    (size_WP, nwp)."""
+    psi = np.asarray(psi)
+    if psi.ndim == 1:
+        psi = psi[:, None]
+    r = _to_pairs(psi)
+    check(load().evr_propa_save_restart(T, psi.shape[1], psi.shape[0], _d(r), os.fsencode(path)))
+
+
+def read_restart(path, nwp, size_wp):
+    """ReadWP_restart: (T, psi (size_WP, nwp) complex, found)."""
+    r = np.zeros((size_wp, 2 * nwp), order="F")
+    T = ctypes.c_double()
+    found = ctypes.c_int()
+    check(load().evr_propa_read_restart(os.fsencode(path), nwp, size_wp, _d(r), ctypes.byref(T),
+                                        ctypes.byref(found)))
+    return T.value, _from_pairs(r), bool(found.value)
 
 
 def bessel_j(n, r):
@@ -180,6 +215,13 @@ class Plan:
         S = None if s is None else np.ascontiguousarray(s, dtype=np.float64)
         check(load().evr_plan_set_keo_grid(self.h, _d(G), _d(J), _d(S)))
 
+    def set_type1_grid(self, F2q, F1q=None):
+        """Type 1 with the operator grids per point: F2q (sum_nq, D, D) with
+        F2q[g, i, j] multiplying d_i d_j, F1q (sum_nq, D) multiplying d_i."""
+        F2 = np.asarray(F2q, dtype=np.float64).ravel(order="F").copy()
+        F1 = None if F1q is None else np.asarray(F1q, dtype=np.float64).ravel(order="F").copy()
+        check(load().evr_plan_set_type1_grid(self.h, _d(F2), _d(F1)))
+
     def gen_potential_poly(self, ham):
         D = ham.D
         c3 = np.ascontiguousarray(ham.c3.reshape(D * D * D))   # (i*D + j)*D + k
@@ -245,15 +287,20 @@ class Plan:
 
     def davidson(self, nev, block=16, max_sub=0, max_it=200, conv_resi=1e-8, conv_ene=1e-9,
                  precond=True, diag=None, guess=None, precision=FP64, refine_resi=1e-3,
-                 want_vectors=False, verbose=False):
-        """Lowest nev levels by block Davidson (sub_propagation_Davidson)."""
+                 want_vectors=False, verbose=False, guess_file=None, save_file=None):
+        """Lowest nev levels by block Davidson (sub_propagation_Davidson).
+        guess_file / save_file: wavepacket files (sub_save_psi formats) read as
+        start vectors / written with the nev Ritz vectors."""
         keep = []
         dg = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
         gs = None if guess is None else np.asfortranarray(guess, dtype=np.float64)
         keep += [dg, gs]
         o = DavidsonOpts(nev, block, max_sub, max_it, conv_resi, conv_ene, 1 if precond else 0,
                          _d(dg), _d(gs), 0 if gs is None else gs.shape[1], precision, refine_resi,
-                         min(block, self.max_nvec), int(verbose))
+                         min(block, self.max_nvec), int(verbose),
+                         ctypes.cast(self.basis.ptr, ctypes.c_void_p) if (guess_file or save_file) else None,
+                         None if guess_file is None else os.fsencode(guess_file),
+                         None if save_file is None else os.fsencode(save_file))
         ev = np.zeros(nev)
         vec = np.zeros((self.nb_packed, nev), order="F") if want_vectors else None
         info = DavidsonInfo()
@@ -267,7 +314,8 @@ class Plan:
         return lo.value, hi.value
 
     def propagate(self, psi, dt, nsteps, method=CHEBY, npoly=200, poly_tol=1e-12, Hmin=0.0, Hmax=0.0,
-                  field=None, mu=None, psi0=None, precision=FP64, autocorr=False, graph=False):
+                  field=None, mu=None, psi0=None, precision=FP64, autocorr=False, graph=False,
+                  field_sub=None):
         """psi(t + nsteps dt) for complex psi (nb_packed,) or (nb_packed, nwp): march_cheby / march_SIL.
 
         field: (nsteps, nwp) values of the field coupling -E mu (mu a Plan on the
@@ -280,10 +328,14 @@ class Plan:
         if field is not None:
             fl = np.ascontiguousarray(np.asarray(field, dtype=np.float64).reshape(nsteps, nwp))
         p0 = None if psi0 is None else _to_pairs(psi0)
+        fs, nsub = None, 0
+        if field_sub is not None:       # (nsteps, nsub, nwp): TAYLOR / RK4
+            fs = np.ascontiguousarray(field_sub, dtype=np.float64)
+            nsub = fs.shape[1]
         o = PropaOpts(method, dt, nsteps, npoly, poly_tol, Hmin, Hmax, _d(fl),
                       None if mu is None else mu.h,
                       None if p0 is None else p0.ctypes.data_as(ctypes.c_void_p), precision, 0,
-                      1 if graph else 0)
+                      1 if graph else 0, _d(fs), nsub)
         ac = np.zeros((nsteps, nwp, 2)) if autocorr else None
         info = PropaInfo()
         check(load().evr_propagate(self.h, ctypes.byref(o), nwp, r.ctypes.data_as(ctypes.c_void_p), _d(ac),
@@ -294,14 +346,15 @@ class Plan:
         return out, (None if ac is None else ac[..., 0] + 1j * ac[..., 1]), info
 
     def control_gradient(self, mu, amp, psi0, phi1, T, nsteps, checkpoint=32, npoly=200, poly_tol=1e-14,
-                         Hmin=0.0, Hmax=0.0, want=("a",)):
+                         Hmin=0.0, Hmax=0.0, want=("a",), precision=FP64):
         """J_f = |<phi1|psi_f(T)>|^2 for the fields of amp (nk x nf) and its gradients.
 
         want: any of "a" (nk x nf), "E" (nsteps x nf), "V" (sum_nq x nf).
+        precision FP32: sweeps with FP32 H|psi>, J refined in FP64.
         Returns (J, {name: gradient}, info)."""
         amp = np.asfortranarray(np.asarray(amp, dtype=np.float64).reshape(np.shape(amp)[0], -1))
         nk, nf = amp.shape
-        o = ControlOpts(T, nsteps, nk, checkpoint, npoly, poly_tol, Hmin, Hmax)
+        o = ControlOpts(T, nsteps, nk, checkpoint, npoly, poly_tol, Hmin, Hmax, precision)
         p0 = _to_pairs(np.asarray(psi0).reshape(-1, 1))
         p1 = _to_pairs(np.asarray(phi1).reshape(-1, 1))
         J = np.zeros(nf)
@@ -331,6 +384,46 @@ class Plan:
         b = np.zeros(self.nb_packed)
         check(load().evr_grid_to_basis(self.h, _d(psiG), _d(b)))
         return b
+
+    def BasisRep_TO_GridRep_n(self, psiB):
+        """BasisTOGrid of the columns of psiB (nb_packed x ncol) -> (sum_nq x ncol)."""
+        psiB = np.asfortranarray(psiB, dtype=np.float64)
+        if psiB.ndim == 1:
+            psiB = psiB.reshape(-1, 1, order="F")
+        out = np.zeros((self.basis.sum_nq, psiB.shape[1]), order="F")
+        check(load().evr_basis_to_grid_n(self.h, psiB.shape[1], psiB.ctypes.data, out.ctypes.data, 0))
+        return out
+
+    def GridRep_TO_BasisRep_n(self, psiG):
+        """GridTOBasis of the columns of psiG (sum_nq x ncol) -> (nb_packed x ncol)."""
+        psiG = np.asfortranarray(psiG, dtype=np.float64)
+        if psiG.ndim == 1:
+            psiG = psiG.reshape(-1, 1, order="F")
+        out = np.zeros((self.basis.nb_packed, psiG.shape[1]), order="F")
+        check(load().evr_grid_to_basis_n(self.h, psiG.shape[1], psiG.ctypes.data, out.ctypes.data, 0))
+        return out
+
+    def DerivOp_TO_RVecG(self, psiG, tab_der):
+        """Grid derivative of the columns of psiG (DerivOp_TO_RVecG): tab_der =
+        (i, j), 1-based modes, 0 for none."""
+        psiG = np.asfortranarray(psiG, dtype=np.float64)
+        if psiG.ndim == 1:
+            psiG = psiG.reshape(-1, 1, order="F")
+        out = np.zeros_like(psiG, order="F")
+        td = (ctypes.c_int * 2)(*tab_der)
+        check(load().evr_grid_deriv(self.h, psiG.shape[1], td, psiG.ctypes.data, out.ctypes.data, 0))
+        return out
+
+    def transform_device(self, what, ncol, in_ptr, out_ptr, tab_der=(0, 0)):
+        """Device-pointer forms: what = 'b2g', 'g2b' or 'deriv'."""
+        L = load()
+        if what == "b2g":
+            check(L.evr_basis_to_grid_n(self.h, ncol, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr), DEVICE_PTRS))
+        elif what == "g2b":
+            check(L.evr_grid_to_basis_n(self.h, ncol, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr), DEVICE_PTRS))
+        else:
+            td = (ctypes.c_int * 2)(*tab_der)
+            check(L.evr_grid_deriv(self.h, ncol, td, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr), DEVICE_PTRS))
 
     def close(self):
         if getattr(self, "h", None):
@@ -435,15 +528,16 @@ class Coupled:
         return d
 
     def davidson(self, nev, block=16, max_sub=0, max_it=200, conv_resi=1e-8, conv_ene=1e-9,
-                 precond=True, diag=None, guess=None, want_vectors=False, verbose=False):
-        """Lowest nev vibronic levels by the block Davidson of Plan.davidson."""
+                 precond=True, diag=None, guess=None, want_vectors=False, verbose=False, precision=FP64):
+        """Lowest nev vibronic levels by the block Davidson of Plan.davidson
+        (FP64 only: the library refuses precision=FP32)."""
         mv = ctypes.c_int()
         check(load().evr_coupled_sizes(self.h, None, None, ctypes.byref(mv)))
         n = self.nb0 * self.nb_packed
         dg = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
         gs = None if guess is None else np.asfortranarray(guess, dtype=np.float64)
         o = DavidsonOpts(nev, block, max_sub, max_it, conv_resi, conv_ene, 1 if precond else 0,
-                         _d(dg), _d(gs), 0 if gs is None else gs.shape[1], FP64, 1e-3,
+                         _d(dg), _d(gs), 0 if gs is None else gs.shape[1], precision, 1e-3,
                          min(block, mv.value), int(verbose))
         ev = np.zeros(nev)
         vec = np.zeros((n, nev), order="F") if want_vectors else None

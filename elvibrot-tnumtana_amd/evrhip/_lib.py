@@ -29,7 +29,8 @@ class DavidsonOpts(ctypes.Structure):
                 ("max_it", ctypes.c_int), ("conv_resi", ctypes.c_double), ("conv_ene", ctypes.c_double),
                 ("precond", ctypes.c_int), ("diag", _dp), ("guess", _dp), ("nguess", ctypes.c_int),
                 ("precision", ctypes.c_int), ("refine_resi", ctypes.c_double),
-                ("max_nvec_hint", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("max_nvec_hint", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("basis", ctypes.c_void_p), ("guess_file", ctypes.c_char_p), ("save_file", ctypes.c_char_p)]
 
 
 class DavidsonInfo(ctypes.Structure):
@@ -42,7 +43,8 @@ class PropaOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("dt", ctypes.c_double), ("nsteps", ctypes.c_int),
                 ("npoly", ctypes.c_int), ("poly_tol", ctypes.c_double), ("Hmin", ctypes.c_double),
                 ("Hmax", ctypes.c_double), ("field", _dp), ("mu", ctypes.c_void_p), ("psi0", ctypes.c_void_p),
-                ("precision", ctypes.c_int), ("device_ptrs", ctypes.c_int), ("graph", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("device_ptrs", ctypes.c_int), ("graph", ctypes.c_int),
+                ("field_sub", _dp), ("nsub", ctypes.c_int)]
 
 
 class PropaInfo(ctypes.Structure):
@@ -53,7 +55,7 @@ class PropaInfo(ctypes.Structure):
 class ControlOpts(ctypes.Structure):
     _fields_ = [("T", ctypes.c_double), ("nsteps", ctypes.c_int), ("nk", ctypes.c_int),
                 ("checkpoint", ctypes.c_int), ("npoly", ctypes.c_int), ("poly_tol", ctypes.c_double),
-                ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double)]
+                ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double), ("precision", ctypes.c_int)]
 
 
 class ControlInfo(ctypes.Structure):
@@ -72,6 +74,10 @@ _SIGS = {
     "evr_basis_desc_free": (None, [ctypes.POINTER(BasisDesc)]),
     "evr_basis_desc_sizes": (ctypes.c_int, [ctypes.POINTER(BasisDesc), _i64p]),
     "evr_basis_desc_grid": (ctypes.c_int, [ctypes.POINTER(BasisDesc), _dp]),
+    "evr_propa_save_restart": (ctypes.c_int, [ctypes.c_double, ctypes.c_int, ctypes.c_int64, _dp,
+                                              ctypes.c_char_p]),
+    "evr_propa_read_restart": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, _dp, _dp,
+                                              ctypes.POINTER(ctypes.c_int)]),
     "evr_psi_save": (ctypes.c_int, [ctypes.POINTER(BasisDesc), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
     "evr_psi_read": (ctypes.c_int, [ctypes.POINTER(BasisDesc), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
@@ -81,6 +87,7 @@ _SIGS = {
     "evr_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "evr_plan_set_potential": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "evr_plan_set_keo_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "evr_plan_set_type1_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "evr_plan_gen_potential_poly": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "evr_plan_sizes": (ctypes.c_int, [ctypes.c_void_p, _i64p]),
     "evr_plan_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
@@ -88,6 +95,12 @@ _SIGS = {
     "evr_plan_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, _dp, _dp, _i64p,
                                              ctypes.POINTER(ctypes.c_int)]),
     "evr_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "evr_basis_to_grid_n": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int]),
+    "evr_grid_to_basis_n": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int]),
+    "evr_grid_deriv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "evr_basis_to_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "evr_plan_diagonal": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     "evr_davidson": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DavidsonOpts), _dp, _dp,
@@ -101,6 +114,8 @@ _SIGS = {
     "evr_plan_release_assembled": (ctypes.c_int, [ctypes.c_void_p]),
     "evr_plan_bounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
     "evr_control_field": (ctypes.c_int, [ctypes.POINTER(ControlOpts), ctypes.c_int, _dp, _dp]),
+    "evr_control_field_sub": (ctypes.c_int, [ctypes.POINTER(ControlOpts), ctypes.c_int, _dp, ctypes.c_int,
+                                             ctypes.c_int, _dp]),
     "evr_control_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ControlOpts),
                                             ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                             ctypes.POINTER(ControlInfo)]),

@@ -20,7 +20,7 @@ module evr_hip_f
   private :: c_strlen
 
   integer(c_int), parameter :: EVR_OK = 0, EVR_FP64 = 0, EVR_FP32 = 1, EVR_DEVICE_PTRS = 2
-  integer(c_int), parameter :: EVR_PROPA_CHEBY = 1, EVR_PROPA_SIL = 2
+  integer(c_int), parameter :: EVR_PROPA_CHEBY = 1, EVR_PROPA_SIL = 2, EVR_PROPA_TAYLOR = 3, EVR_PROPA_RK4 = 4
   integer(c_int), parameter :: EVR_MATRIX_FREE = 4, EVR_AUTO_PATH = 8, EVR_ASM_FP64 = 1, EVR_ASM_FP32 = 2
 
   type, bind(C) :: evr_basis_desc
@@ -47,6 +47,7 @@ module evr_hip_f
     integer(c_int) :: nguess = 0, precision = EVR_FP64
     real(c_double) :: refine_resi = 1.0d-3
     integer(c_int) :: max_nvec_hint = 0, verbose = 0
+    type(c_ptr)    :: basis = c_null_ptr, guess_file = c_null_ptr, save_file = c_null_ptr
   end type evr_davidson_opts
 
   type, bind(C) :: evr_davidson_info
@@ -62,6 +63,8 @@ module evr_hip_f
     real(c_double) :: poly_tol = 1.0d-12, Hmin = 0.0d0, Hmax = 0.0d0
     type(c_ptr)    :: field = c_null_ptr, mu = c_null_ptr, psi0 = c_null_ptr
     integer(c_int) :: precision = EVR_FP64, device_ptrs = 0, graph = 0
+    type(c_ptr)    :: field_sub = c_null_ptr
+    integer(c_int) :: nsub = 0
   end type evr_propa_opts
 
   type, bind(C) :: evr_propa_info
@@ -74,6 +77,7 @@ module evr_hip_f
     real(c_double) :: T = 0.0d0
     integer(c_int) :: nsteps = 1, nk = 1, checkpoint = 0, npoly = 200
     real(c_double) :: poly_tol = 1.0d-14, Hmin = 0.0d0, Hmax = 0.0d0
+    integer(c_int) :: precision = EVR_FP64
   end type evr_control_opts
 
   type, bind(C) :: evr_control_info

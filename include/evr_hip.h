@@ -197,6 +197,19 @@ int evr_plan_set_potential(evr_plan *p, const double *V);
  * memory: (D^2 + 2) sum_nq tables and (D + 2) sum_nq work doubles per vector
  * processed at once. */
 int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const double *sqRhoOVERJac);
+/* Type 1 with the operator grids per grid point, as the reference's
+ * get_OpGrid_type1_OF_ONEDP_FOR_SG4 (OPSG4:2820-3084) tabulates them from
+ * Tnum (TnumKEO_TO_tab_d0H, Source_PrimOperator/sub_PrimOp.f90:1540-1700) and
+ * combines them (OPSG4:1490-1546): H psi_G = V psi_G + sum_ij F2(:, i, j)
+ * d_i d_j psi_G + sum_i F1(:, i) d_i psi_G on every term grid, d_i d_i through
+ * dnRGG%d2 and d_i d_j (i /= j) through dnRGG%d1 on both modes.  F2 is
+ * Fortran (sum_nq, D, D) over the term-concatenated grid, F2(g, i, j) the
+ * coefficient of d_i d_j (the reference's one term per pair i <= j: put its
+ * value in one triangle and zero in the other); F1 is (sum_nq, D) or NULL.
+ * Replaces the plan's constant F2/F1.  SG4 bases with D <= 12, a type-1 plan
+ * with hermitian = 0, FP64 and matrix-free only (EVR_ERR_ARG for an assembly
+ * or FP32); the plan diagonal then comes from H on blocks of unit vectors. */
+int evr_plan_set_type1_grid(evr_plan *p, const double *F2, const double *F1);
 /* Generate V on the device: V(q) = sum_i v2_i q_i^2 + sum_{i<=j<=k} c_ijk
  * q_i q_j q_k + sum_{i<j} d_ij q_i^2 q_j^2 + sum_i e_i q_i^4, with c given as a
  * dense D^3 array (only i<=j<=k read), d as D x D (only i<j read). */
@@ -296,16 +309,35 @@ int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG);
  * (BG:59), SG4 branch with the weighted Smolyak sum (BGSG4:951-1028). */
 int evr_grid_to_basis(evr_plan *p, const double *psiG, double *psiB);
 
+/* Multi-vector forms of BasisTOGrid / GridTOBasis (sub_PsiBasisRep_TO_GridRep
+ * PSIBG:60, sub_PsiGridRep_TO_BasisRep PSIBG:178, for a table of psi):
+ * ncol columns, psiB nb_packed x ncol and psiG sum_nq x ncol column-major
+ * (the grid in the caller's order: concatenated term grids for SG4, the
+ * reference's DP order for a DP basis); flags EVR_DEVICE_PTRS for device
+ * pointers on the plan stream (then no host copies are made). */
+int evr_basis_to_grid_n(evr_plan *p, int ncol, const double *psiB, double *psiG, int flags);
+int evr_grid_to_basis_n(evr_plan *p, int ncol, const double *psiG, double *psiB, int flags);
+/* Grid derivative DerivOp_TO_RVecG(RVecG, nq, basis, tab_der) (BG:1210-1434;
+ * SG4: DerivOp_TO_RDP_OF_SmolaykRep on every term grid, BGSG4:2690-2795) of
+ * ncol grid columns: tab_der = (i, j), 1-based modes, 0 (or < 0) for none:
+ * (i, i) dnRGG%d2 along i; (i, 0) or (0, i) dnRGG%d1 along i; (i, j) d1 along
+ * both; (0, 0) a copy.  Layout and flags as evr_grid_to_basis_n; D <= 12. */
+int evr_grid_deriv(evr_plan *p, int ncol, const int *tab_der, const double *psiG, double *out, int flags);
+
 /* ---- eigen-solver --------------------------------------------------------- */
 /* Block Davidson for the nev lowest levels: sub_propagation_Davidson
  * (DAV:54-748) with sub_MakeHPsi (DAV:880-1024), Rayleigh-Ritz in FP64
  * (DAV:1028-1143, here a device symmetric eigensolver), residuals
  * (DAV:1214-1288), new vectors (DAV:1290-1588: NewVec_type 2 when precond=1,
- * type 4 otherwise), collapse (DAV:516-606).  Symmetric H expected (type 1, or
- * a plan created with hermitian=1).  precision EVR_FP32: FP32 H|psi> until the
- * largest residual drops below refine_resi (or stops falling: 6 iterations
- * without a 10% decrease, the FP32 floor), then H V is rebuilt in FP64 and the
- * iterations finish in FP64 (the FP64 refinement). */
+ * type 4 otherwise), collapse (DAV:516-606; max_sub >= nev + block).
+ * Symmetric H expected (type 1, or a plan created with hermitian=1).
+ * precision EVR_FP32 (mixed): the H|psi> of every new vector in FP32, the
+ * subspace algebra in FP64, and FP64 defect corrections: the subspace is
+ * collapsed on the Ritz vectors and H applied to them in FP64 when the
+ * residuals first drop below refine_resi, stop falling (6 iterations without
+ * a 10% decrease: the FP32 floor) or all meet the criteria; couplings of the
+ * FP64 columns with FP32 ones come from the FP64 side; convergence is
+ * declared on an all-FP64 H V only (FP64 residuals). */
 typedef struct evr_davidson_opts {
   int nev;                 /* levels wanted                                     */
   int block;               /* new vectors per iteration (<= plan max_nvec)      */
@@ -318,10 +350,20 @@ typedef struct evr_davidson_opts {
   const double *guess;     /* host start vectors nb_packed x nguess, or NULL:
                               unit vectors on the lowest diagonal entries       */
   int nguess;
-  int precision;           /* EVR_FP64 or EVR_FP32                              */
-  double refine_resi;      /* FP32 -> FP64 switch threshold                     */
+  int precision;           /* EVR_FP64 or EVR_FP32 (mixed, see above)           */
+  double refine_resi;      /* residual of the first FP64 defect correction      */
   int max_nvec_hint;       /* vectors per evr_oppsi call (0: block)             */
   int verbose;
+  /* Wavepacket files (read_WP of the reference, DAV:215-216 with
+   * ReadWP0_Davidson DAV:752 through sub_read_psi0; the levels' vectors saved
+   * with sub_save_psi as DAV:218): basis is the descriptor the plan was made
+   * from (file rows are matched by index tuple, evr_psi_read).  guess_file:
+   * real start vectors read from it (up to max_sub, in place of guess);
+   * save_file: the nev Ritz vectors written at the end (version 0,
+   * formatted).  NULL: not used. */
+  const evr_basis_desc *basis;
+  const char *guess_file;
+  const char *save_file;
 } evr_davidson_opts;
 
 typedef struct evr_davidson_info {
@@ -362,9 +404,21 @@ int evr_davidson(evr_plan *p, const evr_davidson_opts *opts, double *evals, doub
  * for wavepacket w, mu being a plan on the same basis (a dipole surface as a
  * type-0 operator); the field is constant over a step (for Chebyshev the
  * spectral interval is widened by max|field| * |mu|_max).
+ * Taylor (EVR_PROPA_TAYLOR, march_noD_field): psi(t_n + dt) = e^{-i E0 dt}
+ * sum_i w_i, w_0 = psi, w_{i+1} = (-i dt/(i+1)) [(H - E0) w_i - mu sum_{k<=i}
+ * (dt^k/k!) E^(k)(t_n) w_{i-k}] (the Taylor series of the time-dependent
+ * equation, field derivatives up to the largest k with max|E^(k)| dt^(k+1)/k!
+ * > poly_tol), orders up to npoly, stopping when every |w_{i+1}|^2 < poly_tol;
+ * E0 = (Hmin + Hmax)/2 (Hmin >= Hmax: from the Lanczos interval).
+ * RK4 (EVR_PROPA_RK4, march_RK4_field): classical Runge-Kutta on
+ * dpsi/dt = -i [(H - E0) - E(t) mu] psi (fcn_field) with the field at t_n,
+ * t_n + dt/2, t_n + dt, then psi *= e^{-i E0 dt}.
  * The operator must be symmetric (type 1, or a plan created hermitian=1). */
 #define EVR_PROPA_CHEBY 1
 #define EVR_PROPA_SIL 2
+#define EVR_PROPA_TAYLOR 3   /* march_noD_field (MARCH:1635-1836)               */
+#define EVR_PROPA_RK4 4      /* march_RK4_field with fcn_field (MARCH:392-543,
+                                1434-1515)                                      */
 typedef struct evr_propa_opts {
   int method;              /* EVR_PROPA_CHEBY or EVR_PROPA_SIL                  */
   double dt;               /* WPdeltaT                                          */
@@ -384,6 +438,13 @@ typedef struct evr_propa_opts {
                               checked after every replay and the step redone
                               adaptively when it fails; 0 = adaptive order per
                               step with a host test per order (march_cheby)     */
+  const double *field_sub; /* TAYLOR / RK4 (host): the time-dependent field of
+                              step n at sub-points, field_sub[(n*nsub + j)*nwp
+                              + w]: TAYLOR the derivatives E^(j)(t_n), j < nsub
+                              (sub_dnE, sub_module_field.f90:541); RK4 E(t_n),
+                              E(t_n + dt/2), E(t_n + dt) (nsub = 3); mu the
+                              coupling plan; NULL: no field                   */
+  int nsub;
 } evr_propa_opts;
 
 typedef struct evr_propa_info {
@@ -457,6 +518,11 @@ typedef struct evr_control_opts {
   double poly_tol;
   double Hmin, Hmax;       /* interval of H (Hmin >= Hmax: estimated); it is
                               widened by max|E| * |mu|_max                      */
+  int precision;           /* EVR_FP64; or EVR_FP32: the forward and backward
+                              sweeps apply H and mu in FP32 (wavepackets,
+                              recurrences and accumulations stay FP64), and J
+                              is refined by one FP64 forward sweep; the
+                              gradients are those of the FP32 sweeps           */
 } evr_control_opts;
 
 typedef struct evr_control_info {
@@ -467,6 +533,13 @@ typedef struct evr_control_info {
   double seconds_forward, seconds_backward;
 } evr_control_info;
 
+/* The same fields sampled for the time-dependent integrators of evr_propagate
+ * (EVR_PROPA_TAYLOR: the derivatives E^(j)(t_n), j < nsub, as sub_dnE gives
+ * them, Source_ElVibRot/sub_propagation/sub_module_field.f90:541; EVR_PROPA_RK4:
+ * E(t_n), E(t_n + dt/2), E(t_n + dt), nsub = 3), t_n = n T/nsteps, in the
+ * layout of evr_propa_opts.field_sub: field_sub[(n*nsub + j)*nf + f]. */
+int evr_control_field_sub(const evr_control_opts *opts, int nf, const double *amp, int method, int nsub,
+                          double *field_sub);
 /* Field values at the mid-steps: field[n*nf + f] (amp: nk x nf, column per field). */
 int evr_control_field(const evr_control_opts *opts, int nf, const double *amp, double *field);
 /* H: symmetric plan (hermitian=1 or type 1); mu: plan of the coupling on the
@@ -477,6 +550,17 @@ int evr_control_field(const evr_control_opts *opts, int nf, const double *amp, d
 int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control_opts *opts, int nf,
                          const double *amp, const double *psi0, const double *phi1, double *J,
                          double *grad_a, double *grad_E, double *grad_V, evr_control_info *info);
+
+/* Propagation restart files, SaveWP_restart / ReadWP_restart
+ * (Source_ElVibRot/sub_propagation/sub_module_propagation.f90:376-488): a
+ * formatted list-directed file "T nb_WP size_WP" followed by each
+ * wavepacket's complex coefficients (size_WP = nb0 x nb_packed values, here
+ * one "(re,im)" per line).  psi: size_WP x (2 nwp) columns (Re, Im) as
+ * evr_propagate.  Reading: *found = 0 and *T = 0 when the header cannot be
+ * read (the reference's "No restart, T0=0"), EVR_ERR_IO when the sizes in the
+ * file differ from (nwp, size_WP) (the reference STOPs). */
+int evr_propa_save_restart(double T, int nwp, int64_t size_wp, const double *psi, const char *path);
+int evr_propa_read_restart(const char *path, int nwp, int64_t size_wp, double *psi, double *T, int *found);
 
 /* Extreme eigenvalues of a symmetric plan from `steps` Lanczos steps with full
  * re-orthogonalisation (min(steps, nb_packed) used), the estimate behind the

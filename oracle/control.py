@@ -32,6 +32,24 @@ def field_values(amp, T, nsteps):
     return field_basis(T, nsteps, amp.shape[0]) @ amp
 
 
+def field_derivs(amp, T, t, nder):
+    """E^(j)(t), j = 0..nder, of E(t) = sin^2(pi t/T) sum_k a_k sin(k pi t/T)
+    (amp: nk or nk x nf) from sin^2(x) sin(kx) = sin(kx)/2 - sin((k+2)x)/4
+    - sin((k-2)x)/4 and d^j/dt^j sin(m w t) = (m w)^j sin(m w t + j pi/2),
+    w = pi/T.  Returns (nder + 1, nf)."""
+    amp = np.asarray(amp, dtype=np.float64).reshape(np.shape(amp)[0], -1)
+    nk = amp.shape[0]
+    w = np.pi / T
+    out = np.zeros((nder + 1, amp.shape[1]))
+    for j in range(nder + 1):
+        for k in range(1, nk + 1):
+            v = 0.0
+            for m, c in ((k, 0.5), (k + 2, -0.25), (k - 2, -0.25)):
+                v += c * (m * w) ** j * np.sin(m * w * t + j * np.pi / 2)
+            out[j] += v * amp[k - 1]
+    return out
+
+
 def grid_weights(s):
     """gw (sum_nq) of an oracle SG4 basis: term weight x tensor quadrature weight
     (W[0, q] / B[q, 0] per 1D rule, first mode fastest inside a term)."""

@@ -541,14 +541,18 @@ static void term_b2g(int D, const orc_1d *const *b, const double *in, double *ou
     nnb /= nb2;
     double *dst = bufs[cur];
     const double *M = b[d]->B; /* nq2 x nb2 */
+    /* the same sums (k ascending for every element), the pencil index iq
+     * innermost so the loads are contiguous */
     for (long long ib = 0; ib < nnb; ++ib)
-      for (long long iq = 0; iq < nnq; ++iq)
-        for (int q = 0; q < nq2; ++q) {
-          double s = 0.0;
-          for (int k = 0; k < nb2; ++k)
-            s += M[q + (size_t)nq2 * k] * src[iq + nnq * (k + (long long)nb2 * ib)];
-          dst[iq + nnq * (q + (long long)nq2 * ib)] = s;
+      for (int q = 0; q < nq2; ++q) {
+        double *o = dst + nnq * (q + (long long)nq2 * ib);
+        for (long long iq = 0; iq < nnq; ++iq) o[iq] = 0.0;
+        for (int k = 0; k < nb2; ++k) {
+          const double m = M[q + (size_t)nq2 * k];
+          const double *r = src + nnq * (k + (long long)nb2 * ib);
+          for (long long iq = 0; iq < nnq; ++iq) o[iq] += m * r[iq];
         }
+      }
     nnq *= nq2;
     src = dst;
     cur ^= 1;
@@ -573,13 +577,15 @@ static void term_g2b(int D, const orc_1d *const *b, const double *in, double *ou
     double *dst = bufs[cur];
     const double *M = b[d]->W; /* nb2 x nq2 */
     for (long long iq = 0; iq < nnq; ++iq)
-      for (long long ib = 0; ib < nnb; ++ib)
-        for (int k = 0; k < nb2; ++k) {
-          double s = 0.0;
-          for (int q = 0; q < nq2; ++q)
-            s += M[k + (size_t)nb2 * q] * src[ib + nnb * (q + (long long)nq2 * iq)];
-          dst[ib + nnb * (k + (long long)nb2 * iq)] = s;
+      for (int k = 0; k < nb2; ++k) {
+        double *o = dst + nnb * (k + (long long)nb2 * iq);
+        for (long long ib = 0; ib < nnb; ++ib) o[ib] = 0.0;
+        for (int q = 0; q < nq2; ++q) {
+          const double m = M[k + (size_t)nb2 * q];
+          const double *r = src + nnb * (q + (long long)nq2 * iq);
+          for (long long ib = 0; ib < nnb; ++ib) o[ib] += m * r[ib];
         }
+      }
     nnb *= nb2;
     src = dst;
     cur ^= 1;
@@ -597,13 +603,15 @@ static void term_deriv(int D, const orc_1d *const *b, int dim, const double *M,
   for (int d = dim + 1; d < D; ++d) nnq3 *= b[d]->nq;
   int nq2 = b[dim]->nq;
   for (long long i3 = 0; i3 < nnq3; ++i3)
-    for (long long i1 = 0; i1 < nnq1; ++i1)
-      for (int q = 0; q < nq2; ++q) {
-        double s = 0.0;
-        for (int k = 0; k < nq2; ++k)
-          s += M[q + (size_t)nq2 * k] * R[i1 + nnq1 * (k + (long long)nq2 * i3)];
-        tmp[i1 + nnq1 * (q + (long long)nq2 * i3)] = s;
+    for (int q = 0; q < nq2; ++q) {
+      double *o = tmp + nnq1 * (q + (long long)nq2 * i3);
+      for (long long i1 = 0; i1 < nnq1; ++i1) o[i1] = 0.0;
+      for (int k = 0; k < nq2; ++k) {
+        const double m = M[q + (size_t)nq2 * k];
+        const double *r = R + nnq1 * (k + (long long)nq2 * i3);
+        for (long long i1 = 0; i1 < nnq1; ++i1) o[i1] += m * r[i1];
       }
+    }
   long long n = nnq1 * nq2 * nnq3;
   memcpy(R, tmp, sizeof(double) * n);
 }
@@ -612,8 +620,9 @@ static void term_deriv(int D, const orc_1d *const *b, int dim, const double *M,
  * psiG is overwritten by (H psi)_G.  qpt: grid coordinates (nq x D). */
 static void term_grid_op(int D, const orc_1d *const *b, const orc_op *op, long long nq,
                          const double *V, const double *J, const double *s, const double *Gq,
-                         long long gstride, const double *qpt, double *psiG, double *w1,
-                         double *w2, double *PsiRj, double *PsiRi, double *OpPsiR)
+                         long long gstride, const double *qpt, const double *qmm, double *psiG,
+                         double *w1, double *w2, double *PsiRj, double *PsiRi, double *OpPsiR,
+                         double *PsiRall, const double *F2q, const double *F1q)
 {
   if (op->type == 0) {
     /* OPSG4:1448-1488 */
@@ -625,24 +634,30 @@ static void term_grid_op(int D, const orc_1d *const *b, const orc_op *op, long l
      * GridOp(term) * DerivOp(term) psi.  Terms: (0,0) -> V; (i,i) -> F2_ii
      * with dnRGG%d2; (i,j) i/=j -> F2_ij with d1 on dim i and d1 on dim j;
      * (i,0) -> F1_i with d1. */
+    /* F2q/F1q: GridOp(:, iterm) per point of the term grid (F2q, F1q point
+     * at the term's first grid point, column stride gstride) */
     memset(OpPsiR, 0, sizeof(double) * nq);
     if (V) for (long long p = 0; p < nq; ++p) OpPsiR[p] += V[p] * psiG[p];
     for (int i = 0; i < D; ++i)
       for (int j = 0; j < D; ++j) {
-        double f = op->F2[i + D * j];
+        const double *fq = op->F2q ? F2q + gstride * (i + (long long)D * j) : NULL;
+        double f = fq ? 1.0 : op->F2[i + D * j];
         if (f == 0.0) continue;
         memcpy(w1, psiG, sizeof(double) * nq);
         if (i == j) term_deriv(D, b, i, b[i]->D2, w1, w2);
         else { term_deriv(D, b, i, b[i]->D1, w1, w2); term_deriv(D, b, j, b[j]->D1, w1, w2); }
-        for (long long p = 0; p < nq; ++p) OpPsiR[p] += f * w1[p];
+        if (fq) for (long long p = 0; p < nq; ++p) OpPsiR[p] += fq[p] * w1[p];
+        else for (long long p = 0; p < nq; ++p) OpPsiR[p] += f * w1[p];
       }
-    if (op->F1)
+    if (op->F1 || F1q)
       for (int i = 0; i < D; ++i) {
-        double f = op->F1[i];
+        const double *fq = F1q ? F1q + gstride * i : NULL;
+        double f = fq ? 1.0 : op->F1[i];
         if (f == 0.0) continue;
         memcpy(w1, psiG, sizeof(double) * nq);
         term_deriv(D, b, i, b[i]->D1, w1, w2);
-        for (long long p = 0; p < nq; ++p) OpPsiR[p] += f * w1[p];
+        if (fq) for (long long p = 0; p < nq; ++p) OpPsiR[p] += fq[p] * w1[p];
+        else for (long long p = 0; p < nq; ++p) OpPsiR[p] += f * w1[p];
       }
     memcpy(psiG, OpPsiR, sizeof(double) * nq);
     return;
@@ -656,22 +671,68 @@ static void term_grid_op(int D, const orc_1d *const *b, const orc_op *op, long l
     term_deriv(D, b, j, b[j]->D1, PsiRj + (size_t)nq * j, w2);
   }
   memset(OpPsiR, 0, sizeof(double) * nq);
+  if (!Gq && op->Ga) {
+    /* G_ji(q) = G0_ji + sum_k Ga_jik q_k, evaluated per point as the
+     * reference's get_d0GG does (OPSG4:2724), for a block of points at a time
+     * (a small table GGiq(block, j, i)), then PsiRi = J sum_j GGiq(:,j,i)
+     * PsiRj(:,j) for every i */
+    enum { BLK = 64 };
+    double *Gb = (double *)malloc(sizeof(double) * (size_t)D * D * BLK);
+    for (long long p0 = 0; p0 < nq; p0 += BLK) {
+      const int np = (int)(nq - p0 < BLK ? nq - p0 : BLK);
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+          double *g = Gb + (size_t)BLK * (j + D * i);
+          const double g0 = op->G0[j + D * i];
+          for (int pp = 0; pp < np; ++pp) g[pp] = g0;
+          for (int k = 0; k < D; ++k) {
+            const double a = op->Ga[j + D * (i + D * k)];
+            const double *qk = qmm + (size_t)nq * k + p0;
+            for (int pp = 0; pp < np; ++pp) g[pp] += a * qk[pp];
+          }
+        }
+      for (int i = 0; i < D; ++i) {
+        double *o = PsiRall + (size_t)nq * i + p0;
+        for (int pp = 0; pp < np; ++pp) o[pp] = 0.0;
+        for (int j = 0; j < D; ++j) {
+          const double *g = Gb + (size_t)BLK * (j + D * i);
+          const double *pj = PsiRj + (size_t)nq * j + p0;
+          for (int pp = 0; pp < np; ++pp) o[pp] += g[pp] * pj[pp];
+        }
+        if (J) for (int pp = 0; pp < np; ++pp) o[pp] *= J[p0 + pp];
+      }
+    }
+    free(Gb);
+    for (int i = 0; i < D; ++i) {
+      term_deriv(D, b, i, b[i]->D1, PsiRall + (size_t)nq * i, w2);
+      for (long long p = 0; p < nq; ++p) OpPsiR[p] += PsiRall[(size_t)nq * i + p];
+    }
+    for (long long p = 0; p < nq; ++p)
+      psiG[p] = -0.5 * OpPsiR[p] / ((J ? J[p] : 1.0) * (s ? s[p] : 1.0)) + VPsi[p];
+    return;
+  }
   for (int i = 0; i < D; ++i) {
-    /* PsiRi = sum_j GGiq(:,j,i) * PsiRj(:,j); GGiq(:,j,i) = G_ji(q) */
-    for (long long p = 0; p < nq; ++p) {
-      double acc = 0.0;
+    /* PsiRi = sum_j GGiq(:,j,i) * PsiRj(:,j); GGiq(:,j,i) = G_ji(q), as
+     * whole-grid vector operations (OPSG4:1590-1610) */
+    if (Gq) {
+      for (long long p = 0; p < nq; ++p) PsiRi[p] = 0.0;
       for (int j = 0; j < D; ++j) {
-        double g;
-        if (Gq) {
-          g = Gq[p + gstride * (j + (long long)D * i)];   /* GGiq(p, j, i) */
-        } else {
-          g = op->G0[j + D * i];
+        const double *g = Gq + gstride * (j + (long long)D * i);   /* GGiq(:, j, i) */
+        const double *pj = PsiRj + (size_t)nq * j;
+        for (long long p = 0; p < nq; ++p) PsiRi[p] += g[p] * pj[p];
+      }
+      if (J) for (long long p = 0; p < nq; ++p) PsiRi[p] *= J[p];
+    } else {
+      for (long long p = 0; p < nq; ++p) {
+        double acc = 0.0;
+        for (int j = 0; j < D; ++j) {
+          double g = op->G0[j + D * i];
           if (op->Ga)
             for (int k = 0; k < D; ++k) g += op->Ga[j + D * (i + D * k)] * qpt[p * D + k];
+          acc += g * PsiRj[(size_t)nq * j + p];
         }
-        acc += g * PsiRj[(size_t)nq * j + p];
+        PsiRi[p] = acc * (J ? J[p] : 1.0);
       }
-      PsiRi[p] = acc * (J ? J[p] : 1.0);
     }
     term_deriv(D, b, i, b[i]->D1, PsiRi, w2);
     for (long long p = 0; p < nq; ++p) OpPsiR[p] += PsiRi[p];
@@ -682,6 +743,29 @@ static void term_grid_op(int D, const orc_1d *const *b, const orc_op *op, long l
 
 static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const double *psi,
                           double *hpsi, int nthreads, const long long *list, long long nlist);
+
+/* The potential at every point of one term grid, as the reference fills its
+ * grid cache point by point from the potential routine (OPSG4:2754-2767):
+ * coefficients v2[D], c[(i*D+j)*D+k] (i<=j<=k), d[i*D+j] (i<j), e[D]. */
+static void poly_potential(int D, const double *cf, long long nq, const double *qpt, double *V)
+{
+  const double *v2 = cf, *c3 = cf + D, *d22 = c3 + (size_t)D * D * D, *e4 = d22 + (size_t)D * D;
+  for (long long p = 0; p < nq; ++p) {
+    const double *q = qpt + p * D;
+    double val = 0.0;
+    for (int i = 0; i < D; ++i) {
+      double q2 = q[i] * q[i];
+      val += (v2[i] + e4[i] * q2) * q2;
+      for (int j = i + 1; j < D; ++j) val += d22[i * D + j] * q2 * q[j] * q[j];
+      for (int j = i; j < D; ++j) {
+        double qij = q[i] * q[j], acc = 0.0;
+        for (int k = j; k < D; ++k) acc += c3[((size_t)i * D + j) * D + k] * q[k];
+        val += qij * acc;
+      }
+    }
+    V[p] = val;
+  }
+}
 
 int orc_sg4_oppsi_range(const orc_sg4 *s, const orc_op *op, int nvec, const double *psi,
                         double *hpsi, int nthreads, long long t0, long long t1)
@@ -728,6 +812,11 @@ static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const do
     double *PsiRi = (double *)malloc(sizeof(double) * mq);
     double *OpPsiR = (double *)malloc(sizeof(double) * mq);
     double *qpt = (double *)malloc(sizeof(double) * mq * D);
+    double *qmm = (double *)malloc(sizeof(double) * mq * D);   /* mode-major copy */
+    double *Vt = (double *)malloc(sizeof(double) * mq);
+    /* PsiRi of every i at once (D x nq) for the G0 + Ga q path */
+    double *PsiRall = (op->type == 10 && op->Ga && !op->Gq)
+                          ? (double *)malloc(sizeof(double) * mq * D) : NULL;
     double *x = (double *)malloc(sizeof(double) * mb);
     double *y = (double *)malloc(sizeof(double) * mb);
     const orc_1d *b[64];
@@ -740,13 +829,21 @@ static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const do
       long long nbt = s->off_nb[t + 1] - s->off_nb[t];
       long long nqt = s->off_nq[t + 1] - s->off_nq[t];
       const int *map = s->map + s->off_nb[t];
-      if (op->type == 10 && op->Ga && !op->Gq) {
+      if ((op->type == 10 && op->Ga && !op->Gq) || (!op->V && op->Vpoly)) {
         for (long long p = 0; p < nqt; ++p) {
           long long r = p;
-          for (int d = 0; d < D; ++d) { qpt[p * D + d] = b[d]->x[r % b[d]->nq]; r /= b[d]->nq; }
+          for (int d = 0; d < D; ++d) {
+            qpt[p * D + d] = qmm[d * nqt + p] = b[d]->x[r % b[d]->nq];
+            r /= b[d]->nq;
+          }
         }
       }
+      const double *Gterm = op->Gq ? op->Gq + s->off_nq[t] : NULL;
       const double *V = op->V ? op->V + s->off_nq[t] : NULL;
+      if (!V && op->Vpoly) {
+        poly_potential(D, op->Vpoly, nqt, qpt, Vt);
+        V = Vt;
+      }
       const double *J = op->Jac ? op->Jac + s->off_nq[t] : NULL;
       const double *sr = op->sqRhoOVERJac ? op->sqRhoOVERJac + s->off_nq[t] : NULL;
       for (int v = 0; v < nvec; ++v) {
@@ -754,8 +851,9 @@ static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const do
         /* gather (BGSG4:1220-1238) */
         for (long long e = 0; e < nbt; ++e) x[e] = map[e] >= 0 ? pv[map[e]] : 0.0;
         term_b2g(D, b, x, G, T);
-        term_grid_op(D, b, op, nqt, V, J, sr, op->Gq ? op->Gq + s->off_nq[t] : NULL, s->sum_nq, qpt, G,
-                     w1, w2, PsiRj, PsiRi, OpPsiR);
+        term_grid_op(D, b, op, nqt, V, J, sr, Gterm, s->sum_nq, qpt, qmm, G, w1, w2, PsiRj, PsiRi,
+                     OpPsiR, PsiRall, op->F2q ? op->F2q + s->off_nq[t] : NULL,
+                     op->F1q ? op->F1q + s->off_nq[t] : NULL);
         term_g2b(D, b, G, y, T, w2);
         /* weighted scatter (BGSG4:1273-1287) with OMP ATOMIC */
         double wt = s->weights[t];
@@ -768,7 +866,7 @@ static int sg4_oppsi_list(const orc_sg4 *s, const orc_op *op, int nvec, const do
       }
     }
     free(G); free(T); free(w1); free(w2); free(PsiRj); free(PsiRi); free(OpPsiR);
-    free(qpt); free(x); free(y);
+    free(qpt); free(qmm); free(Vt); free(PsiRall); free(x); free(y);
   }
   return 0;
 }
@@ -820,6 +918,31 @@ int orc_sg4_g2b(const orc_sg4 *s, const double *psiG, double *psiB)
       if (map[e] >= 0) psiB[map[e]] += s->weights[t] * y[e];
   }
   free(y); free(T); free(T2);
+  return 0;
+}
+
+/* DerivOp_TO_RDP_OF_SmolaykRep (BGSG4:2690-2795) on every term grid of the
+ * concatenated SG4 grid: modes i, j 0-based, < 0 for none; i == j: dnRGG%d2,
+ * i /= j: d1 along i then along j (BGSG4:2762-2768 per derivative). */
+int orc_sg4_deriv(const orc_sg4 *s, int i, int j, const double *psiG, double *out)
+{
+  const int D = s->D;
+  if (i >= D || j >= D) return -1;
+  memcpy(out, psiG, sizeof(double) * s->sum_nq);
+  double *T = (double *)malloc(sizeof(double) * s->max_nq);
+  const orc_1d *b[64];
+  for (long long t = 0; t < s->nterms; ++t) {
+    const int *l = s->levels + t * D;
+    for (int d = 0; d < D; ++d) b[d] = &s->bas[l[d] + (s->LG + 1) * d];
+    double *R = out + s->off_nq[t];
+    if (i >= 0 && i == j) {
+      term_deriv(D, b, i, b[i]->D2, R, T);
+    } else {
+      if (i >= 0) term_deriv(D, b, i, b[i]->D1, R, T);
+      if (j >= 0) term_deriv(D, b, j, b[j]->D1, R, T);
+    }
+  }
+  free(T);
   return 0;
 }
 
@@ -935,18 +1058,36 @@ static void dp_deriv(const orc_dp *p, int dim, const double *M, double *R, doubl
   for (int d = 0; d < dim; ++d) nnq3 *= p->bas[d].nq;
   int nq2 = p->bas[dim].nq;
   for (long long i3 = 0; i3 < nnq3; ++i3)
-    for (long long i1 = 0; i1 < nnq1; ++i1)
-      for (int q = 0; q < nq2; ++q) {
-        double s = 0.0;
-        for (int k = 0; k < nq2; ++k)
-          s += M[q + (size_t)nq2 * k] * R[i1 + nnq1 * (k + (long long)nq2 * i3)];
-        tmp[i1 + nnq1 * (q + (long long)nq2 * i3)] = s;
+    for (int q = 0; q < nq2; ++q) {
+      double *o = tmp + nnq1 * (q + (long long)nq2 * i3);
+      for (long long i1 = 0; i1 < nnq1; ++i1) o[i1] = 0.0;
+      for (int k = 0; k < nq2; ++k) {
+        const double m = M[q + (size_t)nq2 * k];
+        const double *r = R + nnq1 * (k + (long long)nq2 * i3);
+        for (long long i1 = 0; i1 < nnq1; ++i1) o[i1] += m * r[i1];
       }
+    }
   memcpy(R, tmp, sizeof(double) * p->nq_tot);
 }
 
 int orc_dp_b2g(const orc_dp *p, const double *psiB, double *psiG) { dp_b2g(p, psiB, psiG); return 0; }
 int orc_dp_g2b(const orc_dp *p, const double *psiG, double *psiB) { dp_g2b(p, psiG, psiB); return 0; }
+
+/* DerivOp_TO_RVecG on the DP grid (BG:1291-1352), modes as orc_sg4_deriv. */
+int orc_dp_deriv(const orc_dp *p, int i, int j, const double *psiG, double *out)
+{
+  if (i >= p->D || j >= p->D) return -1;
+  memcpy(out, psiG, sizeof(double) * p->nq_tot);
+  double *tmp = (double *)malloc(sizeof(double) * p->nq_tot);
+  if (i >= 0 && i == j) {
+    dp_deriv(p, i, p->bas[i].D2, out, tmp);
+  } else {
+    if (i >= 0) dp_deriv(p, i, p->bas[i].D1, out, tmp);
+    if (j >= 0) dp_deriv(p, j, p->bas[j].D1, out, tmp);
+  }
+  free(tmp);
+  return 0;
+}
 
 int orc_dp_oppsi(const orc_dp *p, const orc_op *op, int nvec, const double *psi, double *hpsi)
 {

@@ -97,6 +97,22 @@ typedef struct {
                           Fortran (sum_nq, D, D): Gq(g, j, i) = GGiq(:,j,i) of
                           get_OpGrid_type10_OF_ONEDP_FOR_SG4 (OPSG4:2538-2816),
                           the term-concatenated grid; NULL: G0 + Ga q        */
+  const double *Vpoly; /* SG4 with V == NULL: the potential is evaluated at
+                          every term-grid point as the reference's grid cache
+                          is filled, by calling the potential routine per
+                          point (OPSG4:2754-2767, calcN_op), here the
+                          polynomial sum_i v2_i q_i^2 + sum_{i<=j<=k} c_ijk
+                          q_i q_j q_k + sum_{i<j} d_ij q_i^2 q_j^2 + sum_i e_i
+                          q_i^4 with coefficients v2[D], c[(i*D+j)*D+k],
+                          d[i*D+j], e[D] in one array; NULL: no potential.
+                          Pinned against an explicit V of the same formula. */
+  const double *F2q;   /* SG4 type 1 with the operator grids per point, as
+                          get_OpGrid_type1_OF_ONEDP_FOR_SG4 tabulates them
+                          (OPSG4:2820-3084): Fortran (sum_nq, D, D), F2q(g,i,j)
+                          the coefficient of d_i d_j at g (d_i d_i through
+                          dnRGG%d2, i /= j through d1 on i and on j); NULL:
+                          the constant F2                                   */
+  const double *F1q;   /* (sum_nq, D): coefficient of d_i per point, or NULL */
 } orc_op;
 
 /* H|psi> on the SG4 basis: sub_TabOpPsi_FOR_SGtype4 (OPSG4:678-979) with
@@ -125,6 +141,11 @@ int orc_sg4_oppsi_terms(const orc_sg4 *s, const orc_op *op, int nvec, const doub
  * (BGSG4:2101, BGSG4:951-1028; terms with |w|<1e-6 skipped, BGSG4:1004). */
 int orc_sg4_b2g(const orc_sg4 *s, const double *psiB, double *psiG);
 int orc_sg4_g2b(const orc_sg4 *s, const double *psiG, double *psiB);
+/* DerivOp_TO_RDP_OF_SmolaykRep (BGSG4:2690-2795) on every term grid: modes
+ * i, j 0-based (< 0: none); i == j dnRGG%d2, else d1 along i and along j.
+ * Pinned by the derivative of the grid image of a basis vector against the
+ * closed-form derivatives of the Hermite functions (D1 B = B', D2 B = B''). */
+int orc_sg4_deriv(const orc_sg4 *s, int i, int j, const double *psiG, double *out);
 
 /* ------------------------------------------------------------------ */
 /* Direct-product basis (SparseGrid_type=0).                           */
@@ -144,6 +165,8 @@ int orc_dp_oppsi(const orc_dp *p, const orc_op *op, int nvec,
                  const double *psi, double *hpsi);
 int orc_dp_b2g(const orc_dp *p, const double *psiB, double *psiG);
 int orc_dp_g2b(const orc_dp *p, const double *psiG, double *psiB);
+/* DerivOp_TO_RVecG on the DP grid (BG:1291-1352), modes as orc_sg4_deriv. */
+int orc_dp_deriv(const orc_dp *p, int i, int j, const double *psiG, double *out);
 
 #ifdef __cplusplus
 }

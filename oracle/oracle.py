@@ -20,7 +20,9 @@ _llp = ctypes.POINTER(ctypes.c_longlong)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle_evr.so")
+        # ORACLE_LIB: the same source built for one host (the CPU baseline's
+        # -march=native copy)
+        path = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle_evr.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle library not built: run `make -C oracle`")
         L = ctypes.CDLL(path)
@@ -45,6 +47,8 @@ def lib():
                                           _dp, ctypes.c_int, _llp, ctypes.c_longlong]
         L.orc_sg4_b2g.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_sg4_g2b.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_sg4_deriv.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_dp_deriv.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_herm_quadrature.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_herm_basis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_herm_dngg.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -70,20 +74,34 @@ def _i(a):
 
 class _Op(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int), ("V", _dp), ("G0", _dp), ("Ga", _dp), ("F2", _dp),
-                ("F1", _dp), ("Jac", _dp), ("sqRhoOVERJac", _dp), ("Gq", _dp)]
+                ("F1", _dp), ("Jac", _dp), ("sqRhoOVERJac", _dp), ("Gq", _dp), ("Vpoly", _dp),
+                ("F2q", _dp), ("F1q", _dp)]
+
+
+def poly_coefs(v2, c3, d22, e4):
+    """Coefficient array of the oracle's per-point polynomial potential:
+    v2[D], c3 as (i*D + j)*D + k, d22 as i*D + j, e4[D]."""
+    D = len(v2)
+    return np.concatenate([np.asarray(v2, float), np.asarray(c3, float).reshape(D ** 3),
+                           np.asarray(d22, float).reshape(D * D), np.asarray(e4, float)])
 
 
 class Op:
     """Operator description (type 0 / 1 / 10), arrays kept alive here."""
 
-    def __init__(self, type, V=None, G0=None, Ga=None, F2=None, F1=None, Jac=None, s=None, Gq=None):
-        """Gq: SG4 type 10 G per grid point, (sum_nq, D, D) with Gq[g, j, i] = GGiq(g, j, i)."""
+    def __init__(self, type, V=None, G0=None, Ga=None, F2=None, F1=None, Jac=None, s=None, Gq=None,
+                 Vpoly=None, F2q=None, F1q=None):
+        """Gq: SG4 type 10 G per grid point, (sum_nq, D, D) with Gq[g, j, i] = GGiq(g, j, i).
+        Vpoly: poly_coefs(...) of a polynomial potential evaluated per grid
+        point (SG4, in place of V).  F2q (sum_nq, D, D), F1q (sum_nq, D): SG4
+        type 1 operator grids per point, F2q[g, i, j] multiplying d_i d_j."""
         def f(a):
             return None if a is None else np.asarray(a, dtype=np.float64).ravel(order="F").copy()
-        self.arrays = dict(V=f(V), G0=f(G0), Ga=f(Ga), F2=f(F2), F1=f(F1), Jac=f(Jac), s=f(s), Gq=f(Gq))
+        self.arrays = dict(V=f(V), G0=f(G0), Ga=f(Ga), F2=f(F2), F1=f(F1), Jac=f(Jac), s=f(s), Gq=f(Gq),
+                           Vpoly=f(Vpoly), F2q=f(F2q), F1q=f(F1q))
         a = self.arrays
         self.st = _Op(type, _d(a["V"]), _d(a["G0"]), _d(a["Ga"]), _d(a["F2"]), _d(a["F1"]),
-                      _d(a["Jac"]), _d(a["s"]), _d(a["Gq"]))
+                      _d(a["Jac"]), _d(a["s"]), _d(a["Gq"]), _d(a["Vpoly"]), _d(a["F2q"]), _d(a["F1q"]))
 
 
 def herm_quadrature(nq):
@@ -215,6 +233,14 @@ class SG4:
         lib().orc_sg4_g2b(self.h, _d(psiG), _d(b))
         return b
 
+    def deriv(self, psiG, i, j):
+        """Grid derivative on every term grid; i, j 0-based modes, -1 for none."""
+        psiG = np.ascontiguousarray(psiG, dtype=np.float64)
+        out = np.zeros(self.sum_nq)
+        if lib().orc_sg4_deriv(self.h, i, j, _d(psiG), _d(out)) != 0:
+            raise ValueError("bad mode")
+        return out
+
 
 class DP:
     """Direct-product Hermite basis of the oracle (last dimension fastest)."""
@@ -256,3 +282,10 @@ class DP:
         b = np.zeros(self.nb_tot)
         lib().orc_dp_g2b(self.h, _d(psiG), _d(b))
         return b
+
+    def deriv(self, psiG, i, j):
+        psiG = np.ascontiguousarray(psiG, dtype=np.float64)
+        out = np.zeros(self.nq_tot)
+        if lib().orc_dp_deriv(self.h, i, j, _d(psiG), _d(out)) != 0:
+            raise ValueError("bad mode")
+        return out

@@ -20,6 +20,7 @@ exp(-i x r) = J_0(r) + 2 sum_k (-i)^k J_k(r) T_k(x) on a grid of x; unitarity.
 The Bessel values come from scipy.special.jv (not a restatement).
 """
 import numpy as np
+from scipy.linalg import expm
 from scipy.special import jv
 
 
@@ -118,3 +119,81 @@ def exact_step(H, psi, dt, E=None, M=None):
         ev, V = np.linalg.eigh(H - E[w] * M)
         out[:, w] = V @ (np.exp(-1j * ev * dt) * (V.T @ psi[:, w]))
     return out
+
+
+def taylor_field_step(H, psi, dt, E0, npoly, poly_tol, dnE=None, M=None):
+    """One march_noD_field step (MARCH:1635-1836) for psi (nb, nwp): the
+    Taylor series of i dpsi/dt = (H - E0 - E(t) M) psi about t_n with the
+    field derivatives dnE[k, w] = E^(k)(t_n):
+      w_0 = psi, w_{i+1} = (-i dt/(i+1)) [(H - E0) w_i
+                 + M sum_{k <= min(i, max_der)} (dt^k/k!) (-E^(k)) w_{i-k}],
+    max_der the largest k with max|E^(k)| dt^(k+1)/k! > poly_tol (MARCH:1698-1711),
+    stopping once |w_{i+1}|^2 < poly_tol for every column (this restatement
+    stops when all columns have; the reference loops per wavepacket);
+    psi(t_n + dt) = e^{-i E0 dt} sum_i w_i.  Returns (psi, order)."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    fk = None if dnE is None else -np.asarray(dnE, dtype=np.float64)
+    max_der = 0
+    if fk is not None:
+        fac = dt
+        for k in range(1, min(len(fk), npoly)):
+            fac *= dt / k
+            if np.abs(fk[k]).max() * fac > poly_tol:
+                max_der = k
+    work = [psi.copy()]
+    out = psi.copy()
+    order = 0
+    for i in range(npoly - 1):
+        hw = H @ work[i] - E0 * work[i]
+        if fk is not None and np.abs(fk).max() >= poly_tol:
+            s = np.zeros_like(psi)
+            f = 1.0
+            for k in range(0, min(i, max_der) + 1):
+                if k > 0:
+                    f *= dt / k
+                s = s + f * fk[k][None, :] * work[i - k]
+            hw = hw + M @ s
+        wn = (-1j * dt / (i + 1)) * hw
+        work.append(wn)
+        out = out + wn
+        order = i + 1
+        if np.all((np.abs(wn) ** 2).sum(0) < poly_tol):
+            break
+    return out * np.exp(-1j * E0 * dt), order
+
+
+def rk4_field_step(H, psi, dt, E0, Evals=None, M=None):
+    """One march_RK4_field step (MARCH:392-543) with fcn_field
+    (MARCH:1434-1515): f(t, x) = -i [(H - E0) x - E(t) M x], classical RK4
+    with the field at t_n, t_n + dt/2, t_n + dt (Evals[j, w]), then the phase
+    e^{-i E0 dt}."""
+    psi = np.asarray(psi, dtype=np.complex128)
+
+    def f(x, j):
+        y = H @ x - E0 * x
+        if Evals is not None:
+            y = y - (M @ x) * np.asarray(Evals[j])[None, :]
+        return -1j * y
+    w1 = f(psi, 0)
+    w3 = f(psi + w1 * dt / 2, 1)
+    w4 = f(psi + w3 * dt / 2, 1)
+    w2 = psi + w4 * dt
+    w4 = w3 + w4
+    w3 = f(w2, 2)
+    return (psi + (w1 + w3 + 2 * w4) * dt / 6) * np.exp(-1j * E0 * dt)
+
+
+def magnus4_propagate(H, psi, t0, t1, nsub, field, M):
+    """psi(t1) of i dpsi/dt = (H - E(t) M) psi by nsub fourth-order Magnus
+    steps (two Gauss points): the reference solution of the time-dependent
+    equation that the field integrators converge to."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    h = (t1 - t0) / nsub
+    c = np.sqrt(3.0) / 6.0
+    for n in range(nsub):
+        ta, tb = t0 + (n + 0.5 - c) * h, t0 + (n + 0.5 + c) * h
+        Ha = H - field(ta) * M
+        Hb = H - field(tb) * M
+        Om = -1j * h / 2 * (Ha + Hb) - (np.sqrt(3.0) / 12.0) * h * h * (Hb @ Ha - Ha @ Hb)
+        psi = expm(Om) @ psi
+    return psi

@@ -120,3 +120,18 @@ def test_autograd_function_wrapper(evr, case):
 
     a = torch.tensor(c["amp"][:, :1], dtype=torch.float64, requires_grad=True)
     assert torch.autograd.gradcheck(ControlJ.apply, (a,), eps=1e-6, atol=1e-8, rtol=1e-5)
+
+
+def test_fp32_refined_gradient(evr, case):
+    """precision FP32: both sweeps apply H and mu in FP32, J is refined by an
+    FP64 forward sweep (equal to the FP64 run's J to 1e-12); the gradients
+    carry the FP32 error of H|psi> (1e-4 relative here)."""
+    c = case
+    J64, g64, _ = run(c, c["amp"], want=("a", "E"))
+    J32, g32, info = c["plan"].control_gradient(c["mplan"], c["amp"], c["psi0"], c["phi1"], T, N, checkpoint=8,
+                                                npoly=100, poly_tol=1e-15, want=("a", "E"),
+                                                precision=evr.FP32)
+    ea = np.abs(g32["a"] - g64["a"]).max() / np.abs(g64["a"]).max()
+    print("FP32-refined control: |dJ| %.2e, grad_a rel err %.2e" % (np.abs(J32 - J64).max(), ea))
+    assert np.abs(J32 - J64).max() < 1e-12
+    assert ea < 1e-4

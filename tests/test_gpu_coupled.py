@@ -144,3 +144,23 @@ def test_pyrazine_davidson_levels(evr, oracle_mod):
     assert err < 1e-6
     r = Hd @ vec - vec * ev[None, :]
     assert np.abs(r).max() < 1e-7 and vec.shape == (2 * n, 8)
+
+
+def test_pyrazine_davidson_guess_and_collapse(evr, oracle_mod):
+    """Coupled-state Davidson started from user guess vectors (the dense
+    eigenvectors, perturbed) with a user diagonal and a subspace of nev +
+    block (a collapse every iteration), against the dense levels; FP32 is
+    refused with its message."""
+    b, H = _coupled(evr, P12, 2, max_nvec=4)
+    s, Hd = coupled_dense(oracle_mod, P12, 2)
+    Hs = 0.5 * (Hd + Hd.T)
+    w, U = np.linalg.eigh(Hs)
+    rng = np.random.default_rng(3)
+    guess = U[:, :6] + 1e-3 * rng.standard_normal((Hd.shape[0], 6))
+    ev, vec, info = H.davidson(6, block=4, max_sub=10, conv_resi=1e-9, guess=guess, diag=np.diag(Hd),
+                               want_vectors=True)
+    err = np.abs(ev - w[:6]).max() * 219474.6313705
+    print("pyrazine guess + collapse: max |dE| %.3e cm-1 in %d iterations" % (err, info.iterations))
+    assert info.converged == 1 and err < 1e-6
+    with pytest.raises(RuntimeError, match="FP64 only"):
+        H.davidson(6, block=4, precision=evr.FP32)

@@ -86,3 +86,54 @@ def test_davidson_c1(evr, precision):
         precision, info.iterations, info.hpsi_fp64, info.hpsi_fp32, err, info.seconds))
     assert info.converged == 1
     assert err < 1e-6
+    if precision == "fp32":
+        # the bulk of the H|psi> in FP32, FP64 only for the defect corrections
+        assert info.hpsi_fp32 >= 0.6 * (info.hpsi_fp32 + info.hpsi_fp64)
+
+
+def test_davidson_max_sub(evr, oracle_mod):
+    """max_sub below nev + block is refused (a collapse must keep every wanted
+    Ritz vector, DAV:516-606); at nev + block the subspace collapses every
+    iteration and the levels still converge to the dense ones."""
+    from evrhip import hamiltonian as H
+    D, L = 4, 3
+    ham = H.smolyak_config(1, D)
+    s = oracle_mod.SG4(D, L)
+    V = ham.potential(s.grid())
+    M = s.oppsi(oracle_mod.Op(10, V=V, G0=ham.G0, Ga=ham.Ga_fortran()), np.eye(s.nb_packed))
+    ref = np.linalg.eigvalsh(0.5 * (M + M.T))[:8]
+    plan = evr.Plan(evr.Basis.sg4(D, L), type=10, V=V, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=8,
+                    hermitian=True)
+    with pytest.raises(RuntimeError, match="max_sub must be at least nev"):
+        plan.davidson(8, block=8, max_sub=9)
+    for prec in (evr.FP64, evr.FP32):
+        ev, _, info = plan.davidson(8, block=8, max_sub=16, conv_resi=1e-7, conv_ene=1e-9, precision=prec,
+                                    max_it=400)
+        print("max_sub 16 precision %d: it %d fp64 %d fp32 %d" % (prec, info.iterations, info.hpsi_fp64,
+                                                                 info.hpsi_fp32))
+        assert info.converged == 1
+        assert np.abs(ev - ref).max() < 1e-8 * np.abs(ref).max()
+
+
+def test_davidson_save_and_restart_files(evr, oracle_mod, tmp_path):
+    """The Ritz vectors saved with sub_save_psi's format (save_file) restart a
+    Davidson run (guess_file, read_WP of the reference): the second run
+    converges in fewer iterations to the same levels."""
+    from evrhip import hamiltonian as H
+    D, L = 4, 3
+    ham = H.smolyak_config(1, D)
+    b = evr.Basis.sg4(D, L)
+    s = oracle_mod.SG4(D, L)
+    V = ham.potential(s.grid())
+    plan = evr.Plan(b, type=10, V=V, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=8, hermitian=True)
+    path = str(tmp_path / "dav_wp")
+    ev1, _, info1 = plan.davidson(10, block=8, conv_resi=1e-3, conv_ene=1e-6, save_file=path)
+    ev2, vec2, info2 = plan.davidson(10, block=8, conv_resi=1e-8, conv_ene=1e-10, guess_file=path,
+                                     want_vectors=True)
+    ev3, _, info3 = plan.davidson(10, block=8, conv_resi=1e-8, conv_ene=1e-10)
+    print("restart: it %d (cold start %d)" % (info2.iterations, info3.iterations))
+    assert info2.converged == 1
+    assert np.abs(ev2 - ev3).max() < 1e-8 * np.abs(ev3).max()
+    assert info2.iterations < info3.iterations
+    got = b.read_psi(path, cplx=False, max_wp=20)
+    assert got.shape[1] == 10

@@ -169,3 +169,41 @@ def test_cheby_graph(evr, sg4_case, with_field):
     assert np.abs(graph - eager).max() < 1e-12
     assert np.abs(ac_g - ac_e).max() < 1e-12
     assert ig.max_order == ie.max_order
+
+
+@pytest.mark.parametrize("method", ["taylor", "rk4"])
+def test_time_dependent_field_integrators(evr, sg4_case, method):
+    """march_noD_field (Taylor with the field derivatives) and
+    march_RK4_field on the GPU with the control field E(t) = sin^2(pi t/T)
+    sum_k a_k sin(k pi t/T) (samples from evr_control_field_sub), against the
+    oracle's restatement of the same integrator (1e-10) and, for Taylor,
+    against the time-dependent solution (fine Magnus, 1e-9)."""
+    import control
+    c = sg4_case
+    nwp, nst = 2, 5
+    psi = rand_psi(c["nb"], nwp, 11)
+    rng = np.random.default_rng(12)
+    amp = rng.uniform(-200.0, 200.0, (6, nwp))      # field in cm^-1 per unit of mu
+    T = 0.01
+    dt = T / 20 if method == "taylor" else T / 400
+    m = evr.TAYLOR if method == "taylor" else evr.RK4
+    nsub = 14 if method == "taylor" else 3
+    fs = evr.control_field_sub(amp, T, int(round(T / dt)), m, nsub)[:nst]
+    ev = np.linalg.eigvalsh(c["Hs"])
+    out, _, info = c["plan"].propagate(psi, dt, nst, method=m, npoly=60, poly_tol=1e-26,
+                                       Hmin=ev[0], Hmax=ev[-1], mu=c["mplan"], field_sub=fs)
+    E0 = 0.5 * (ev[0] + ev[-1])
+    ref = psi.copy()
+    for n in range(nst):
+        if m == evr.TAYLOR:
+            ref, _ = propa.taylor_field_step(c["Hs"], ref, dt, E0, 60, 1e-26, dnE=fs[n], M=c["Mu"])
+        else:
+            ref = propa.rk4_field_step(c["Hs"], ref, dt, E0, fs[n], c["Mu"])
+    err = np.abs(out - ref).max()
+    print(method, "orders", info.max_order, "vs oracle", err)
+    assert err < 1e-10
+    if m == evr.TAYLOR:
+        for w in range(nwp):
+            f = lambda t: control.field_derivs(amp[:, w], T, t, 0)[0, 0]
+            ex = propa.magnus4_propagate(c["Hs"], psi[:, w:w + 1], 0.0, nst * dt, 400, f, c["Mu"])[:, 0]
+            assert np.abs(out[:, w] - ex).max() < 1e-9

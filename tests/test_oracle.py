@@ -147,6 +147,52 @@ def test_smolyak_matches_dense_kronecker(oracle_mod):
     assert np.abs(H - Hd).max() < 1e-11 * np.abs(Hd).max()
 
 
+def test_smolyak_type10_d3_generator_matches_dense_kronecker(oracle_mod):
+    """D=3 (the smallest D with three distinct modes in G_ij(q) = G0_ij +
+    sum_k a_ijk q_k, the coupling the benchmark operator carries): the SG4
+    type-10 operator with the generator potential evaluated per grid point
+    (Vpoly) equals (i) the same operator given V explicitly and (ii) the
+    combination of per-term dense Kronecker operators built from the 1D
+    matrices, V from the host formula.  A transposed Ga index, a wrong
+    potential coefficient or a dropped cross term fails (ii)."""
+    from evrhip import hamiltonian as Hm
+    D, L = 3, 3
+    s = oracle_mod.SG4(D, L)
+    ham = Hm.smolyak_config(1, D)
+    q = s.grid()
+    V = ham.potential(q)
+    Ga = ham.Ga.copy()
+    Ga[0, 1, 0] += 3.0           # break the symmetry: an (i, j) transposition shows
+    Gaf = Ga.ravel(order="F")
+    cf = oracle_mod.poly_coefs(ham.v2, ham.c3, ham.d22, ham.e4)
+    X = np.eye(s.nb_packed)
+    Hp = s.oppsi(oracle_mod.Op(10, G0=ham.G0, Ga=Gaf, Vpoly=cf), X)
+    Hv = s.oppsi(oracle_mod.Op(10, V=V, G0=ham.G0, Ga=Gaf), X)
+    assert np.abs(Hp - Hv).max() < 1e-12 * np.abs(Hv).max()
+    lv, w, onb, onq = s.terms()
+    m = s.map()
+    Hd = np.zeros((s.nb_packed, s.nb_packed))
+    Hx = np.zeros_like(Hd)
+    for t in range(s.nterms):
+        one = [s.one_d(d, lv[t, d]) for d in range(D)]
+        K = lambda A0, A1, A2: np.kron(A2, np.kron(A1, A0))     # first mode fastest
+        Bt = K(*[o["B"] for o in one]); Wt = K(*[o["W"] for o in one])
+        I = [np.eye(o["nq"]) for o in one]
+        Dm = [K(one[0]["D1"], I[1], I[2]), K(I[0], one[1]["D1"], I[2]), K(I[0], I[1], one[2]["D1"])]
+        qt = q[onq[t]:onq[t + 1]]
+        op = np.diag(V[onq[t]:onq[t + 1]]).copy()
+        opx = op.copy()
+        for i in range(D):
+            for j in range(D):
+                op += -0.5 * Dm[i] @ np.diag(ham.G0[j, i] + qt @ Ga[j, i, :]) @ Dm[j]
+                opx += -0.5 * Dm[i] @ np.diag(ham.G0[i, j] + qt @ Ga[i, j, :]) @ Dm[j]
+        rows = m[onb[t]:onb[t + 1]]
+        Hd[np.ix_(rows, rows)] += w[t] * (Wt @ op @ Bt)
+        Hx[np.ix_(rows, rows)] += w[t] * (Wt @ opx @ Bt)
+    assert np.abs(Hp - Hd).max() < 1e-11 * np.abs(Hd).max()
+    assert np.abs(Hp - Hx).max() > 1e-6 * np.abs(Hd).max()
+
+
 def tabulated_keo(q, seed):
     """A per-point G (not symmetric in i, j, so a transposed index shows), a
     Jacobian and s = sqrt(rho/J) on the grid points q."""
@@ -191,6 +237,64 @@ def test_smolyak_tabulated_keo_matches_dense_kronecker(oracle_mod):
         Hx[np.ix_(rows, rows)] += w[t] * (Wt @ opx @ Bt)
     assert np.abs(H - Hd).max() < 1e-11 * np.abs(Hd).max()
     assert np.abs(H - Hx).max() > 1e-4 * np.abs(Hd).max()
+
+
+def type1_grids(q, seed):
+    """Per-point type-1 operator grids: F2[p, i, j] (not symmetric, so a
+    transposed index shows) multiplying d_i d_j, and F1[p, i] multiplying d_i."""
+    rng = np.random.default_rng(seed)
+    npt, D = q.shape
+    A = -0.5 * np.diag(rng.uniform(0.8, 1.3, D)) + 0.05 * rng.standard_normal((D, D))
+    B = 0.02 * rng.standard_normal((D, D, D))
+    F2 = A[None] + np.einsum("pk,ijk->pij", q, B)
+    F1 = 0.1 * np.tanh(q @ rng.standard_normal((D, D)))
+    return F2, F1
+
+
+def test_smolyak_type1_grids_match_dense_kronecker(oracle_mod):
+    """D=2, type 1 with the operator grids per point (get_OpGrid_type1,
+    OPSG4:1490-1546): sum_ij F2(:,i,j) d_i d_j psi + sum_i F1(:,i) d_i psi + V psi
+    per term, d_i d_i with dnRGG%d2 and d_i d_j with d1 on both modes, against
+    dense Kronecker operators (d1 d1 in place of d2, or F1 on the wrong mode,
+    differs); a constant table equals the constant-F2 path."""
+    D, L = 2, 3
+    s = oracle_mod.SG4(D, L)
+    q = s.grid()
+    F2, F1 = type1_grids(q, 21)
+    V = 0.5 * (q ** 2).sum(1)
+    I_ = np.eye(s.nb_packed)
+    H = s.oppsi(oracle_mod.Op(1, V=V, F2q=F2, F1q=F1), I_)
+    lv, w, onb, onq = s.terms()
+    m = s.map()
+    Hd = np.zeros((s.nb_packed, s.nb_packed))
+    Hx = np.zeros_like(Hd)
+    for t in range(s.nterms):
+        one = [s.one_d(d, lv[t, d]) for d in range(D)]
+        K = lambda A0, A1: np.kron(A1, A0)
+        Bt = K(one[0]["B"], one[1]["B"]); Wt = K(one[0]["W"], one[1]["W"])
+        I0, I1 = np.eye(one[0]["nq"]), np.eye(one[1]["nq"])
+        D1 = [K(one[0]["D1"], I1), K(I0, one[1]["D1"])]
+        D2 = [K(one[0]["D2"], I1), K(I0, one[1]["D2"])]
+        sl = slice(onq[t], onq[t + 1])
+        op = np.diag(V[sl]).copy()
+        opx = op.copy()
+        for i in range(D):
+            op += np.diag(F1[sl, i]) @ D1[i]
+            opx += np.diag(F1[sl, D - 1 - i]) @ D1[i]        # mistake: F1 mode index swapped
+            for j in range(D):
+                Dij = D2[i] if i == j else D1[i] @ D1[j]
+                op += np.diag(F2[sl, i, j]) @ Dij
+                opx += np.diag(F2[sl, i, j]) @ (D1[i] @ D1[j])  # mistake: d1 d1 for d2
+        rows = m[onb[t]:onb[t + 1]]
+        Hd[np.ix_(rows, rows)] += w[t] * (Wt @ op @ Bt)
+        Hx[np.ix_(rows, rows)] += w[t] * (Wt @ opx @ Bt)
+    assert np.abs(H - Hd).max() < 1e-11 * np.abs(Hd).max()
+    assert np.abs(H - Hx).max() > 1e-4 * np.abs(Hd).max()
+    F2c = np.array([[-0.6, 0.04], [0.03, -0.45]])
+    Hc = s.oppsi(oracle_mod.Op(1, V=V, F2=F2c, F1=np.array([0.1, -0.2])), I_)
+    Ht = s.oppsi(oracle_mod.Op(1, V=V, F2q=np.broadcast_to(F2c, (s.sum_nq, D, D)),
+                               F1q=np.broadcast_to([0.1, -0.2], (s.sum_nq, D))), I_)
+    assert np.abs(Hc - Ht).max() < 1e-13 * np.abs(Hc).max()
 
 
 def test_tabulated_g_equals_polynomial(oracle_mod):
@@ -248,3 +352,60 @@ def test_dp_matches_dense(oracle_mod):
     Dm = [np.kron(one[0][2], np.eye(nq[1])), np.kron(np.eye(nq[0]), one[1][2])]
     op = np.diag(V) - 0.5 * sum(G0[j, i] * Dm[i] @ Dm[j] for i in range(2) for j in range(2))
     assert np.abs(H - Wt @ op @ Bt).max() < 1e-11 * np.abs(H).max()
+
+
+def test_grid_derivative_closed_form(oracle_mod):
+    """orc_sg4_deriv / orc_dp_deriv on the grid image of a basis vector equal
+    the closed-form derivatives of the Hermite functions: d/dq_i and
+    d2/dq_i2 of phi_n(q_i) = H_n e^{-q^2/2}/sqrt(2^n n! sqrt(pi)), and the
+    mixed d_i d_j (dnRGG%d1 B = B', dnRGG%d2 B = B'' exactly for functions
+    of the basis).  A wrong mode index or d1 for d2 fails."""
+    from numpy.polynomial.hermite import hermval
+    from math import factorial, pi, sqrt
+
+    def phi(n, x, der):
+        c = np.zeros(n + 1); c[n] = 1.0
+        f = hermval(x, c) * np.exp(-x * x / 2) / sqrt(2.0 ** n * factorial(n) * sqrt(pi))
+        if der == 0:
+            return f
+        c1 = np.zeros(n + 2); c1[n + 1] = 1.0
+        fp1 = hermval(x, c1) * np.exp(-x * x / 2) / sqrt(2.0 ** (n + 1) * factorial(n + 1) * sqrt(pi))
+        fm1 = 0.0
+        if n > 0:
+            cm = np.zeros(n); cm[n - 1] = 1.0
+            fm1 = hermval(x, cm) * np.exp(-x * x / 2) / sqrt(2.0 ** (n - 1) * factorial(n - 1) * sqrt(pi))
+        if der == 1:                     # phi_n' = sqrt(n/2) phi_{n-1} - sqrt((n+1)/2) phi_{n+1}
+            return sqrt(n / 2.0) * fm1 - sqrt((n + 1) / 2.0) * fp1
+        return (x * x - 2 * n - 1) * f   # phi_n''
+
+    D, L = 3, 3
+    s = oracle_mod.SG4(D, L)
+    q = s.grid()
+    P = s.packed()
+    k = int(np.argmax(P.sum(1)))         # a packed function with several nonzero indices
+    e = np.zeros(s.nb_packed); e[k] = 1.0
+    g = s.b2g(e)
+    n = P[k]
+    # the grid image is phi_n on the grids of the terms that hold function k
+    _, _, onb, onq = s.terms()
+    m = s.map()
+    on = np.zeros(len(q), bool)
+    for t in range(s.nterms):
+        if (m[onb[t]:onb[t + 1]] == k).any():
+            on[onq[t]:onq[t + 1]] = True
+    assert on.sum() > 0 and np.abs(g[~on]).max() == 0.0
+    for (i, j) in [(0, -1), (2, -1), (1, 1), (0, 2), (-1, -1)]:
+        want = np.ones(len(q))
+        for d in range(D):
+            der = (d == i) + (d == j)
+            want = want * phi(n[d], q[:, d], der)
+        got = s.deriv(g, i, j)
+        assert np.abs(got[on] - want[on]).max() < 1e-11, (i, j)
+        assert np.abs(got[~on]).max() == 0.0
+    assert np.abs(s.deriv(g, 1, -1) - s.deriv(g, 0, -1)).max() > 1e-3
+    d = oracle_mod.DP([4, 5, 3], [6, 7, 5])
+    qd = d.grid()
+    e = np.zeros(d.nb_tot); e[7] = 1.0      # last mode fastest: (0, 2, 1)
+    gd = d.b2g(e)
+    want = phi(0, qd[:, 0], 0) * phi(2, qd[:, 1], 1) * phi(1, qd[:, 2], 1)
+    assert np.abs(d.deriv(gd, 1, 2) - want).max() < 1e-11

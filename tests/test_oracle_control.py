@@ -78,3 +78,21 @@ def test_potential_gradient_fd(small_system):
         fd = (control.objective(c["H"] + h * P, c["Mu"], c["psi0"], c["phi1"], E, dt)
               - control.objective(c["H"] - h * P, c["Mu"], c["psi0"], c["phi1"], E, dt)) / (2 * h)
         assert abs(fd[0] - gV[g, 0]) < 1e-5 * np.abs(gV).max()
+
+
+@pytest.mark.parametrize("method,nsub", [(3, 6), (4, 3)])
+def test_control_field_sub_matches_oracle(evr, method, nsub):
+    """evr_control_field_sub (host code of the library) against the oracle's
+    closed-form field derivatives: Taylor derivatives at t_n, RK4 values at
+    t_n, t_n + dt/2, t_n + dt."""
+    rng = np.random.default_rng(9)
+    amp = rng.uniform(-1e-3, 1e-3, (8, 3))
+    T, N = 2000.0, 20
+    got = evr.control_field_sub(amp, T, N, method, nsub)
+    dt = T / N
+    for n in range(N):
+        if method == 3:
+            ref = control.field_derivs(amp, T, n * dt, nsub - 1)
+        else:
+            ref = np.stack([control.field_derivs(amp, T, n * dt + j * dt / 2, 0)[0] for j in range(3)])
+        assert np.abs(got[n] - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())

@@ -124,3 +124,73 @@ def test_errors(evr, basis, tmp_path):
         basis.read_psi(bad)
     with pytest.raises(RuntimeError, match="unformatted"):
         basis.save_psi(tmp_path / "x", np.zeros(basis.nb_packed), version=0, formatted=False)
+
+
+@pytest.mark.parametrize("value", [-5, 2 ** 31 - 1])
+def test_corrupted_unformatted_header_is_an_error(evr, basis, tmp_path, value):
+    """nb_tot read from a file is checked against the file before anything is
+    sized from it: a negative or huge count returns EVR_ERR_IO (the host
+    process survives)."""
+    path = tmp_path / "wpu"
+    basis.save_psi(path, _psi(basis.nb_packed, 1, True), version=1, formatted=False)
+    raw = bytearray(open(path, "rb").read())
+    # records: [4]'X'[4] [4]nb_psi[4] [4]nb_tot[4] ...: nb_tot at byte 25
+    assert int.from_bytes(raw[25:29], "little", signed=True) == basis.nb_packed
+    raw[25:29] = int(value).to_bytes(4, "little", signed=True)
+    open(path, "wb").write(bytes(raw))
+    with pytest.raises(RuntimeError, match="do not fit the file"):
+        basis.read_psi(path, cplx=True)
+
+
+@pytest.mark.parametrize("field,value", [("nb_tot", 2000000000), ("nb_tot", -3), ("nb_psi", -1)])
+def test_corrupted_namelist_header_is_an_error(evr, basis, tmp_path, field, value):
+    path = tmp_path / "wpn"
+    basis.save_psi(path, _psi(basis.nb_packed, 1, True), version=1, formatted=True)
+    text = open(path).read()
+    import re
+    text, n = re.subn(r"(?i)(%s\s*=\s*)(-?\d+)" % field, lambda m: m.group(1) + str(value), text, count=1)
+    assert n == 1
+    open(path, "w").write(text)
+    with pytest.raises(RuntimeError, match="do not fit the file"):
+        basis.read_psi(path, cplx=True)
+
+
+def test_restart_round_trip(evr, tmp_path):
+    """SaveWP_restart / ReadWP_restart: "T nb_WP size_WP" then the complex
+    coefficients, read back exactly."""
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal((37, 3)) + 1j * rng.standard_normal((37, 3))
+    path = tmp_path / "restart"
+    evr.save_restart(path, 123.456, psi)
+    first = open(path).readline().split()
+    assert float(first[0]) == 123.456 and first[1:] == ["3", "37"]
+    T, got, found = evr.read_restart(path, 3, 37)
+    assert found and T == 123.456
+    assert np.array_equal(got, psi)
+
+
+def test_restart_gfortran_list_directed(evr, tmp_path):
+    """A restart file in gfortran's list-directed layout (several complex
+    constants per record, blanks inside the parentheses) is read."""
+    psi = np.array([1.5 - 2j, 0.25 + 0j, -3e-17 + 4.5j, 2 + 1j])
+    items = ["(%25.17E,%25.17E)" % (z.real, z.imag) for z in psi]
+    text = "   2.0000000000000000                1           4\n" + \
+        " " + "  ".join(items[:3]) + "\n " + items[3] + "\n"
+    path = tmp_path / "restart_gf"
+    open(path, "w").write(text.replace("E", "E"))
+    T, got, found = evr.read_restart(path, 1, 4)
+    assert found and T == 2.0
+    assert np.allclose(got[:, 0], psi, rtol=0, atol=0)
+
+
+def test_restart_missing_header_and_size_mismatch(evr, tmp_path):
+    """No readable "T nb_WP size_WP" record: no restart (T0 = 0, found =
+    False), as the reference's WARNING; wrong sizes: an error (the reference
+    STOPs)."""
+    path = tmp_path / "r0"
+    open(path, "w").write("not a restart file\n")
+    T, _, found = evr.read_restart(path, 1, 4)
+    assert not found and T == 0.0
+    evr.save_restart(path, 1.0, np.ones(5, complex))
+    with pytest.raises(RuntimeError, match="restart file holds 1 wavepackets of 5"):
+        evr.read_restart(path, 1, 4)
