@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1638,6 +1639,66 @@ int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
     HIP_TRY(hipStreamSynchronize(p->stream));
   }
   return EVR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------- allocation-free H|psi> --
+// Work memory of evr_oppsi_noalloc: one file-scope device array (each GPU
+// holds its own copy from module load to process end).  Host code never uses
+// the array's name as an address: its device address is looked up once per
+// device with hipGetSymbolAddress and kept in g_scratch_ptr.
+namespace {
+constexpr size_t kScratchBytes = (size_t)EVR_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_device(int dev, double **out)
+{
+  if (dev < 0 || dev >= kMaxDevices) return fail(EVR_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    double *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(EVR_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return EVR_OK;
+}
+
+// 256-byte aligned word offsets of X, R and P inside the scratch array
+int64_t scratch_words(int64_t words) { return (words + 31) & ~(int64_t)31; }
+}  // namespace
+
+extern "C" {
+
+int evr_oppsi_noalloc(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags)
+{
+  if (!p || !psi || !hpsi) return fail(EVR_ERR_ARG, "NULL argument");
+  if (!(flags & EVR_DEVICE_PTRS))
+    return fail(EVR_ERR_ARG, "evr_oppsi_noalloc takes device pointers only (EVR_DEVICE_PTRS)");
+  if (nvec < 1 || nvec > p->max_nvec) return fail(EVR_ERR_ARG, "nvec must be in 1..max_nvec");
+  if (p->keo_grid) return fail(EVR_ERR_ARG, "evr_oppsi_noalloc: plans with operator grids per point use evr_oppsi");
+  if ((flags & EVR_FP32) && !p->V32_valid && !p->d_A32.ptr)
+    return fail(EVR_ERR_STATE, "evr_oppsi_noalloc: the FP32 potential copy is made by a first FP32 evr_oppsi call");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != p->device) return fail(EVR_ERR_ARG, "evr_oppsi_noalloc: the plan's device is not the current device");
+  const int64_t wx = scratch_words(p->nb_packed * nvec);
+  const int64_t wr = scratch_words(std::max<int64_t>(p->nrows, 1) * nvec);
+  const int64_t wp = scratch_words(std::max<int64_t>(p->nseg, 1) * nvec);
+  const int64_t need = 8 * (wx + wr + wp);
+  if (need > (int64_t)kScratchBytes)
+    return fail(EVR_ERR_SIZE, "evr_oppsi_noalloc: " + std::to_string(need) + " bytes of work memory for " +
+                                  std::to_string(nvec) + " vectors exceed the " +
+                                  std::to_string(kScratchBytes) + "-byte scratch (use evr_oppsi)");
+  double *base = nullptr;
+  const int rc = scratch_for_device(dev, &base);
+  if (rc) return rc;
+  return oppsi_core(p, nvec, psi, hpsi, flags, base, base + wx, base + wx + wr);
 }
 
 int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG)

@@ -24,6 +24,7 @@ FP32 = 1
 DEVICE_PTRS = 2
 MATRIX_FREE = 4
 AUTO_PATH = 8
+NOALLOC_SCRATCH_BYTES = 64 << 20   # EVR_NOALLOC_SCRATCH_BYTES of evr_hip.h
 ASM_FP64 = 1
 ASM_FP32 = 2
 CHEBY = 1
@@ -242,6 +243,12 @@ class Plan:
         """H|psi> on device buffers (column-major), queued on the plan stream."""
         check(load().evr_oppsi(self.h, nvec, ctypes.c_void_p(psi_ptr), ctypes.c_void_p(hpsi_ptr),
                                flags | DEVICE_PTRS))
+
+    def TabOpPsi_noalloc_device(self, psi_ptr, hpsi_ptr, nvec, flags=FP64):
+        """evr_oppsi_noalloc: the same H|psi> with the work memory in the
+        library's static per-GPU scratch (no device allocation)."""
+        check(load().evr_oppsi_noalloc(self.h, nvec, ctypes.c_void_p(psi_ptr), ctypes.c_void_p(hpsi_ptr),
+                                       flags | DEVICE_PTRS))
 
     def stream(self):
         return load().evr_plan_stream(self.h)

@@ -101,6 +101,8 @@ _SIGS = {
                                            ctypes.c_int]),
     "evr_grid_deriv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "evr_oppsi_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int]),
     "evr_basis_to_grid": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "evr_plan_diagonal": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     "evr_davidson": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DavidsonOpts), _dp, _dp,

@@ -292,6 +292,23 @@ int evr_plan_kernel_stats(evr_plan *p, int max, char *names, double *ms, double 
  * optionally | EVR_DEVICE_PTRS. */
 int evr_oppsi(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags);
 
+/* H|psi> with no device allocation (no counterpart in the reference; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same kernels and results as
+ * evr_oppsi with EVR_DEVICE_PTRS (required in flags), but the packed staging
+ * block, the term rows and the segment sums live in one static device array
+ * of EVR_NOALLOC_SCRATCH_BYTES per GPU instead of the plan's buffers:
+ * 8 x (nb_packed + term rows + segments) x nvec bytes, rounded to 256-byte
+ * blocks, must fit (EVR_ERR_SIZE otherwise; e.g. D=6 SG4 L=5 takes 32
+ * vectors, D=9 SG4 L=8 none).  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (on different streams or host threads) race.
+ * No part of the array needs clearing (each call writes what it reads).  The
+ * plan's device must be current; FP32 needs a prior FP32 evr_oppsi call (the
+ * FP32 potential copy) or an FP32 assembled operator; plans with G, J, s per
+ * grid point are refused. */
+#define EVR_NOALLOC_SCRATCH_BYTES (64u << 20)
+int evr_oppsi_noalloc(evr_plan *p, int nvec, const double *psi, double *hpsi, int flags);
+
 /* Diagonal of H in the packed basis (nb_packed values): V part exact
  * (sum_t w_t W^(x) diag(V) B^(x) diagonal, one grid->basis pass of V with
  * W o B^T), kinetic part from the diagonals of the per-mode factors.  Used as
