@@ -244,16 +244,44 @@ extern "C" int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control
   }
   HTRY(hipStreamSynchronize(P.st));
 
-  // ---- forward, storing psi every ck steps
+  // ---- forward, storing psi every ck steps.  With opts->graph the steps
+  // after the first are replays of one captured Chebyshev step of the first
+  // step's order (the field values picked on the device by a step counter);
+  // a step whose exit test fails is redone adaptively and the graph rebuilt.
+  // Each step's record (order, E0, Esc) is what the backward sweep recomputes
+  // with, so the gradients are those of the computed propagation either way.
   std::vector<ChebRecord> rec(N);
   int retries = 0, max_order = 0;
+  const bool use_graph = o->graph && N > 1;
+  ChebGraph CG;
+  DBuf field_all, ctr;
+  if (use_graph) {
+    HTRY(field_all.alloc((int64_t)N * nf));
+    HTRY(ctr.alloc(1));
+    HTRY(evr::upload_now(field_all.p, field.data(), sizeof(double) * field.size(), P.st));
+  }
   for (int n = 0; n < N; ++n) {
     if (n % ck == 0)
       HTRY(hipMemcpyAsync(CK.p + nblk * (n / ck), PSI.p, bytes, hipMemcpyDeviceToDevice, P.st));
+    if (CG.x) {
+      bool ok = false;
+      RTRY(replay_cheb_graph(P, cw, PSI.p, o->poly_tol, CG, ok));
+      if (ok) {
+        rec[n] = CG.rec;
+        max_order = std::max(max_order, rec[n].order);
+        continue;
+      }
+    }
     RTRY(P.set_field(field.data() + (size_t)n * nf));
     RTRY(cheb_step(P, cw, PSI.p, P.dE.p, MW.p, dt, o->npoly, o->poly_tol, hmin, hmax, rec[n], retries,
                    nullptr, false));
     max_order = std::max(max_order, rec[n].order);
+    if (use_graph && n + 1 < N &&
+        (!CG.x || rec[n].order > CG.rec.order || rec[n].e0 != CG.rec.e0 || rec[n].esc != CG.rec.esc)) {
+      const double next = (double)(n + 1);
+      HTRY(evr::upload_now(ctr.p, &next, sizeof(double), P.st));
+      RTRY(build_cheb_graph(P, cw, PSI.p, MW.p, dt, field_all.p, ctr.p, rec[n], CG));
+    }
   }
   std::vector<cplx> ov;
   RTRY(P.dots(PHI.p, 0, 1, PSI.p, ov));

@@ -10,6 +10,8 @@
 // is full (DAV:516-606), NewVec_type 2 (Davidson correction r/(E - Hdiag),
 // DAV:1392-1413) when a diagonal is given, plain residuals (type 4) otherwise.
 #include <hip/hip_runtime.h>
+
+#include <map>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -233,15 +235,52 @@ struct Dav {
   int64_t hpsi64 = 0, hpsi32 = 0;
 
   // HX = H X for nc columns (chunks of the plan's max_nvec)
-  int apply(const double *X, double *HX, int nc, int flags)
+  int apply(const double *X, double *HX, int nc, int flags, bool count = true)
   {
     for (int c0 = 0; c0 < nc; c0 += max_nvec) {
       const int k = std::min(max_nvec, nc - c0);
       RTRY(op.plan ? evr_oppsi(op.plan, k, X + nb * c0, HX + nb * c0, flags | EVR_DEVICE_PTRS)
                    : evr_coupled_oppsi(op.coupled, k, X + nb * c0, HX + nb * c0, flags | EVR_DEVICE_PTRS));
-      if (flags & EVR_FP32) hpsi32 += k; else hpsi64 += k;
+      if (count) {
+        if (flags & EVR_FP32) hpsi32 += k; else hpsi64 += k;
+      }
     }
     return EVR_OK;
+  }
+  // The same for the iteration's new-vector block (X, HX the fixed buffers of
+  // that block) as a captured HIP graph per (columns, precision): the first
+  // call runs eagerly (it also makes lazy state such as the FP32 potential
+  // copy) and records the launches; later calls replay them on the stream.
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  int apply_captured(const double *X, double *HX, int nc, int flags)
+  {
+    const auto key = std::make_pair(nc, flags);
+    auto it = graphs.find(key);
+    if (it != graphs.end()) {
+      HTRY(hipGraphLaunch(it->second, st));
+      if (flags & EVR_FP32) hpsi32 += nc; else hpsi64 += nc;
+      return EVR_OK;
+    }
+    RTRY(apply(X, HX, nc, flags));
+    hipGraph_t g = nullptr;
+    HTRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = apply(X, HX, nc, flags, false);
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != EVR_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc != EVR_OK) return rc;
+      HTRY(e);
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HTRY(ei);
+    graphs[key] = ex;
+    return EVR_OK;
+  }
+  ~Dav()
+  {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
   }
 
   // C (m x n) = A^T B with A nb x m, B nb x n.  The result is small and the
@@ -639,7 +678,8 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
     RTRY(d.orthonormalise(V.p, m, T.p, nc, tmpC.p, tmpT.p, kept));
     if (o->verbose) fprintf(stderr, "  new vectors %d kept %d (m %d)\n", nc, kept, m);
     if (kept == 0) break;  // the subspace cannot grow (DAV:1534 drops tiny vectors)
-    RTRY(d.apply(T.p, HT.p, kept, flags));
+    if (o->graph) RTRY(d.apply_captured(T.p, HT.p, kept, flags));
+    else RTRY(d.apply(T.p, HT.p, kept, flags));
     HTRY(hipMemcpyAsync(V.p + nb * m, T.p, sizeof(double) * nb * kept, hipMemcpyDeviceToDevice, d.st));
     HTRY(hipMemcpyAsync(HV.p + nb * m, HT.p, sizeof(double) * nb * kept, hipMemcpyDeviceToDevice, d.st));
     if (!mixed) nex = m + kept;

@@ -1489,6 +1489,112 @@ static void recount_flops(evr_plan *p)
   for (int64_t t = 0; t < p->nterms; ++t) p->flops_class[p->tclass[t]] += term_flops(p, p->terms[t]);
 }
 
+int evr_plan_autotune(evr_plan *p, int nvec, int flags, int reps, int *chosen, double *ms_static,
+                      double *ms_tuned)
+{
+  if (!p) return fail(EVR_ERR_ARG, "NULL plan");
+  if (nvec < 1 || nvec > p->max_nvec) return fail(EVR_ERR_ARG, "nvec must be in 1..max_nvec");
+  if (p->keo_grid) return fail(EVR_ERR_ARG, "plans with operator grids per point are not tuned");
+  if (reps < 1) reps = 3;
+  const bool fp32 = (flags & EVR_FP32) != 0;
+  const int64_t n = p->nb_packed * nvec;
+  DevBuf<double> X, Y;
+  HIP_TRY(X.alloc((size_t)n));
+  HIP_TRY(Y.alloc((size_t)n));
+  hipLaunchKernelGGL(k_unit_cols, dim3(blocks_for(n)), dim3(256), 0, p->stream, p->nb_packed, nvec, (int64_t)0,
+                     X.ptr);
+  HIP_TRY(hipGetLastError());
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  struct EvFree {
+    hipEvent_t a, b;
+    ~EvFree()
+    {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } evfree{e0, e1};
+  const bool prof = p->profiling;
+  p->profiling = false;
+  const int fl = EVR_DEVICE_PTRS | EVR_MATRIX_FREE | (fp32 ? EVR_FP32 : EVR_FP64);
+  // median of reps timed sum-factorised H|psi> after one warm-up call
+  auto time_layout = [&](double &ms) -> int {
+    int rc = evr_oppsi(p, nvec, X.ptr, Y.ptr, fl);
+    if (rc) return rc;
+    std::vector<double> t(reps);
+    for (int r = 0; r < reps; ++r) {
+      HIP_TRY(hipEventRecord(e0, p->stream));
+      rc = evr_oppsi(p, nvec, X.ptr, Y.ptr, fl);
+      if (rc) return rc;
+      HIP_TRY(hipEventRecord(e1, p->stream));
+      HIP_TRY(hipEventSynchronize(e1));
+      float f = 0.f;
+      HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+      t[r] = f;
+    }
+    std::sort(t.begin(), t.end());
+    ms = t[reps / 2];
+    return EVR_OK;
+  };
+  // buckets: terms that share a class under the static rule (smallest class
+  // that holds nb_t); each bucket in turn tries the larger classes that hold
+  // it, the other buckets keeping their current choice
+  std::vector<int> nat(p->nterms);
+  for (int64_t t = 0; t < p->nterms; ++t) {
+    int c = 0;
+    while (kClasses[c].max_nbt() < p->terms[t].nbt) ++c;
+    nat[t] = c;
+  }
+  p->tclass = nat;
+  int rc = apply_classes(p);
+  if (rc) return rc;
+  double best = 0.0;
+  rc = time_layout(best);
+  if (rc) return rc;
+  const double t_static = best;
+  int pick[kNumClasses];
+  for (int b = 0; b < kNumClasses; ++b) pick[b] = b;
+  for (int b = 0; b < kNumClasses; ++b) {
+    bool present = false;
+    for (int64_t t = 0; t < p->nterms && !present; ++t) present = nat[t] == b;
+    if (!present) continue;
+    for (int c = b + 1; c < kNumClasses && c <= b + 2; ++c) {
+      for (int64_t t = 0; t < p->nterms; ++t)
+        if (nat[t] == b) p->tclass[t] = c;
+      double ms = 0.0;
+      rc = apply_classes(p);
+      if (rc == EVR_OK) rc = time_layout(ms);
+      if (rc == EVR_ERR_SIZE) {  // this class cannot hold the bucket's slabs
+        ms = 1e300;
+        rc = EVR_OK;
+      }
+      if (rc) return rc;
+      // keep a larger class only when it is faster by more than the noise
+      if (ms < 0.97 * best) {
+        best = ms;
+        pick[b] = c;
+      } else {
+        break;  // a slower class up: larger ones are not tried (the reference's thread scan stops likewise)
+      }
+    }
+    for (int64_t t = 0; t < p->nterms; ++t)
+      if (nat[t] == b) p->tclass[t] = pick[b];
+    rc = apply_classes(p);
+    if (rc) return rc;
+  }
+  rc = time_layout(best);
+  if (rc) return rc;
+  recount_flops(p);
+  p->profiling = prof;
+  if (chosen)
+    for (int b = 0; b < kNumClasses; ++b) chosen[b] = pick[b];
+  if (ms_static) *ms_static = t_static;
+  if (ms_tuned) *ms_tuned = best;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return EVR_OK;
+}
+
 int evr_plan_set_profiling(evr_plan *p, int on)
 {
   if (!p) return fail(EVR_ERR_ARG, "NULL plan");

@@ -702,24 +702,6 @@ int cheb_step(Batch &P, ChebBufs &cw, double *PSI, const double *dEp, double *MW
 // inside.  Replaying it advances PSI by one step; with a field the graph
 // first picks the step's values from all steps' values by a device counter
 // and advances the counter last.
-struct ChebGraph {
-  hipGraph_t g = nullptr;
-  hipGraphExec_t x = nullptr;
-  ChebRecord rec;
-  double cK = 0.0;             // 2 J_K: the exit test's coefficient
-  int64_t hpsi_per_step = 0;   // H|psi> columns one replay applies
-  ChebGraph() = default;
-  ChebGraph(const ChebGraph &) = delete;
-  ChebGraph &operator=(const ChebGraph &) = delete;
-  void reset()
-  {
-    if (x) (void)hipGraphExecDestroy(x);
-    if (g) (void)hipGraphDestroy(g);
-    x = nullptr;
-    g = nullptr;
-  }
-  ~ChebGraph() { reset(); }
-};
 
 static int capture_cheb_step(Batch &P, ChebBufs &cw, double *PSI, double *MWp, double dt,
                              const double *field_all, double *ctr, const std::vector<double> &J,
@@ -760,7 +742,7 @@ static int capture_cheb_step(Batch &P, ChebBufs &cw, double *PSI, double *MWp, d
   return EVR_OK;
 }
 
-static int build_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double *MWp, double dt,
+int build_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double *MWp, double dt,
                             const double *field_all, double *ctr, const ChebRecord &rec, ChebGraph &G)
 {
   G.reset();
@@ -786,7 +768,7 @@ static int build_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double *MWp, do
 
 // Replay one step; ok = the exit test |c_K| <w_K|w_K> < poly_tol held for
 // every wavepacket (else PSI is restored from the saved step start).
-static int replay_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double poly_tol, ChebGraph &G,
+int replay_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double poly_tol, ChebGraph &G,
                              bool &ok)
 {
   HTRY(hipGraphLaunch(G.x, P.st));

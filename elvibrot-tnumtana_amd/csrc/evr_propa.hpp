@@ -118,5 +118,32 @@ int cheb_step(Batch &P, ChebBufs &cw, double *PSI, const double *dE, double *MW,
               int npoly, double poly_tol, double &hmin, double &hmax, ChebRecord &rec, int &retries,
               double *store, bool fixed);
 
+// One Chebyshev step of fixed order captured as a HIP graph (the launches of
+// cheb_step with rec's order, E0 and Esc): with field_all (all steps' field
+// values on the device) the graph picks its step's values by the device
+// counter ctr and advances it.  replay_cheb_graph runs it and checks the exit
+// test |c_K| <w_K|w_K> < poly_tol (ok = false: PSI restored to the step start).
+struct ChebGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  ChebRecord rec;
+  double cK = 0.0;             // 2 J_K: the exit test's coefficient
+  int64_t hpsi_per_step = 0;   // H|psi> columns one replay applies
+  ChebGraph() = default;
+  ChebGraph(const ChebGraph &) = delete;
+  ChebGraph &operator=(const ChebGraph &) = delete;
+  void reset()
+  {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+  }
+  ~ChebGraph() { reset(); }
+};
+int build_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double *MWp, double dt, const double *field_all,
+                     double *ctr, const ChebRecord &rec, ChebGraph &G);
+int replay_cheb_graph(Batch &P, ChebBufs &cw, double *PSI, double poly_tol, ChebGraph &G, bool &ok);
+
 }  // namespace propa
 }  // namespace evr

@@ -253,6 +253,13 @@ class Plan:
     def stream(self):
         return load().evr_plan_stream(self.h)
 
+    def autotune(self, nvec, flags=FP64, reps=3):
+        """Per-term tile autotuner: (chosen class per bucket, ms static, ms tuned)."""
+        ch = (ctypes.c_int * 8)()
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(load().evr_plan_autotune(self.h, nvec, flags, reps, ch, ctypes.byref(a), ctypes.byref(b)))
+        return list(ch), a.value, b.value
+
     def set_profiling(self, on=True):
         check(load().evr_plan_set_profiling(self.h, 1 if on else 0))
 
@@ -294,7 +301,7 @@ class Plan:
 
     def davidson(self, nev, block=16, max_sub=0, max_it=200, conv_resi=1e-8, conv_ene=1e-9,
                  precond=True, diag=None, guess=None, precision=FP64, refine_resi=1e-3,
-                 want_vectors=False, verbose=False, guess_file=None, save_file=None):
+                 want_vectors=False, verbose=False, guess_file=None, save_file=None, graph=False):
         """Lowest nev levels by block Davidson (sub_propagation_Davidson).
         guess_file / save_file: wavepacket files (sub_save_psi formats) read as
         start vectors / written with the nev Ritz vectors."""
@@ -307,7 +314,7 @@ class Plan:
                          min(block, self.max_nvec), int(verbose),
                          ctypes.cast(self.basis.ptr, ctypes.c_void_p) if (guess_file or save_file) else None,
                          None if guess_file is None else os.fsencode(guess_file),
-                         None if save_file is None else os.fsencode(save_file))
+                         None if save_file is None else os.fsencode(save_file), 1 if graph else 0)
         ev = np.zeros(nev)
         vec = np.zeros((self.nb_packed, nev), order="F") if want_vectors else None
         info = DavidsonInfo()
@@ -353,7 +360,7 @@ class Plan:
         return out, (None if ac is None else ac[..., 0] + 1j * ac[..., 1]), info
 
     def control_gradient(self, mu, amp, psi0, phi1, T, nsteps, checkpoint=32, npoly=200, poly_tol=1e-14,
-                         Hmin=0.0, Hmax=0.0, want=("a",), precision=FP64):
+                         Hmin=0.0, Hmax=0.0, want=("a",), precision=FP64, graph=False):
         """J_f = |<phi1|psi_f(T)>|^2 for the fields of amp (nk x nf) and its gradients.
 
         want: any of "a" (nk x nf), "E" (nsteps x nf), "V" (sum_nq x nf).
@@ -361,7 +368,7 @@ class Plan:
         Returns (J, {name: gradient}, info)."""
         amp = np.asfortranarray(np.asarray(amp, dtype=np.float64).reshape(np.shape(amp)[0], -1))
         nk, nf = amp.shape
-        o = ControlOpts(T, nsteps, nk, checkpoint, npoly, poly_tol, Hmin, Hmax, precision)
+        o = ControlOpts(T, nsteps, nk, checkpoint, npoly, poly_tol, Hmin, Hmax, precision, 1 if graph else 0)
         p0 = _to_pairs(np.asarray(psi0).reshape(-1, 1))
         p1 = _to_pairs(np.asarray(phi1).reshape(-1, 1))
         J = np.zeros(nf)

@@ -30,7 +30,8 @@ class DavidsonOpts(ctypes.Structure):
                 ("precond", ctypes.c_int), ("diag", _dp), ("guess", _dp), ("nguess", ctypes.c_int),
                 ("precision", ctypes.c_int), ("refine_resi", ctypes.c_double),
                 ("max_nvec_hint", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("basis", ctypes.c_void_p), ("guess_file", ctypes.c_char_p), ("save_file", ctypes.c_char_p)]
+                ("basis", ctypes.c_void_p), ("guess_file", ctypes.c_char_p), ("save_file", ctypes.c_char_p),
+                ("graph", ctypes.c_int)]
 
 
 class DavidsonInfo(ctypes.Structure):
@@ -55,7 +56,8 @@ class PropaInfo(ctypes.Structure):
 class ControlOpts(ctypes.Structure):
     _fields_ = [("T", ctypes.c_double), ("nsteps", ctypes.c_int), ("nk", ctypes.c_int),
                 ("checkpoint", ctypes.c_int), ("npoly", ctypes.c_int), ("poly_tol", ctypes.c_double),
-                ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double), ("precision", ctypes.c_int)]
+                ("Hmin", ctypes.c_double), ("Hmax", ctypes.c_double), ("precision", ctypes.c_int),
+                ("graph", ctypes.c_int)]
 
 
 class ControlInfo(ctypes.Structure):
@@ -95,6 +97,8 @@ _SIGS = {
     "evr_plan_kernel_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, _dp, _dp, _i64p,
                                              ctypes.POINTER(ctypes.c_int)]),
     "evr_oppsi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "evr_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), _dp, _dp]),
     "evr_basis_to_grid_n": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_int]),
     "evr_grid_to_basis_n": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

@@ -6,7 +6,9 @@
 !   1. H psi for 4 fixed real vectors and 2 complex ones,
 !   2. the 8 lowest levels by Davidson,
 !   3. 5 Chebyshev steps of a complex wavepacket,
-!   4. H psi again on the assembled per-term operators (evr_plan_assemble).
+!   4. H psi again on the assembled per-term operators (evr_plan_assemble),
+!   5. BasisTOGrid of the 4 vectors (evr_BasisRep_TO_GridRep_n) and the grid
+!      derivative d_1 d_2 of those grids (evr_DerivOp_TO_RVecG).
 ! Results go to the text file named by the first argument.
 !===============================================================================
 program evr_f_host
@@ -20,6 +22,7 @@ program evr_f_host
   type(evr_op_desc) :: op
   integer(c_int64_t) :: sz(4)
   real(c_double), allocatable, target :: q(:, :), V(:), psi(:, :), Hpsi(:, :), Vec(:, :), Hasm(:, :)
+  real(c_double), allocatable, target :: psiG(:, :), dG(:, :)
   real(c_double) :: abytes
   real(c_double), target :: F2(D, D)
   real(c_double) :: Ene(nev)
@@ -69,6 +72,9 @@ program evr_f_host
   if (evr_plan_assembled(plan, abytes) /= EVR_ASM_FP64) stop 'evr_plan_assembled'
   call evr_sub_TabOpPsi(plan, psi, Hasm)
   call evr_check(evr_plan_release_assembled(plan), 'evr_plan_release_assembled')
+  allocate(psiG(npts, nvec), dG(npts, nvec))
+  call evr_BasisRep_TO_GridRep_n(plan, psi, psiG)
+  call evr_DerivOp_TO_RVecG(plan, psiG, dG, [1, 2])
 
   open(newunit=u, file=trim(out), status='replace', action='write')
   write(u, '(i0,1x,i0)') nb, npts
@@ -91,6 +97,11 @@ program evr_f_host
   do k = 1, nvec
     do i = 1, nb
       write(u, '(es25.17)') Hasm(i, k)
+    end do
+  end do
+  do k = 1, nvec
+    do i = 1, npts
+      write(u, '(2es25.17)') psiG(i, k), dG(i, k)
     end do
   end do
   close(u)

@@ -48,6 +48,7 @@ module evr_hip_f
     real(c_double) :: refine_resi = 1.0d-3
     integer(c_int) :: max_nvec_hint = 0, verbose = 0
     type(c_ptr)    :: basis = c_null_ptr, guess_file = c_null_ptr, save_file = c_null_ptr
+    integer(c_int) :: graph = 0
   end type evr_davidson_opts
 
   type, bind(C) :: evr_davidson_info
@@ -77,7 +78,7 @@ module evr_hip_f
     real(c_double) :: T = 0.0d0
     integer(c_int) :: nsteps = 1, nk = 1, checkpoint = 0, npoly = 200
     real(c_double) :: poly_tol = 1.0d-14, Hmin = 0.0d0, Hmax = 0.0d0
-    integer(c_int) :: precision = EVR_FP64
+    integer(c_int) :: precision = EVR_FP64, graph = 0
   end type evr_control_opts
 
   type, bind(C) :: evr_control_info
@@ -236,6 +237,65 @@ module evr_hip_f
       real(c_double), intent(in) :: psiG(*)
       real(c_double), intent(out) :: psiB(*)
       integer(c_int) :: evr_grid_to_basis
+    end function
+    function evr_basis_to_grid_n(p, ncol, psiB, psiG, flags) bind(C, name="evr_basis_to_grid_n")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p, psiB, psiG
+      integer(c_int), value :: ncol, flags
+      integer(c_int) :: evr_basis_to_grid_n
+    end function
+    function evr_grid_to_basis_n(p, ncol, psiG, psiB, flags) bind(C, name="evr_grid_to_basis_n")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p, psiG, psiB
+      integer(c_int), value :: ncol, flags
+      integer(c_int) :: evr_grid_to_basis_n
+    end function
+    function evr_grid_deriv(p, ncol, tab_der, psiG, dpsiG, flags) bind(C, name="evr_grid_deriv")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p, psiG, dpsiG
+      integer(c_int), value :: ncol, flags
+      integer(c_int), intent(in) :: tab_der(2)
+      integer(c_int) :: evr_grid_deriv
+    end function
+    function evr_plan_set_type1_grid(p, F2, F1) bind(C, name="evr_plan_set_type1_grid")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p, F2, F1
+      integer(c_int) :: evr_plan_set_type1_grid
+    end function
+    function evr_propa_save_restart(T, nwp, size_wp, psi, path) bind(C, name="evr_propa_save_restart")
+      import :: c_int, c_int64_t, c_double, c_ptr, c_char
+      real(c_double), value :: T
+      integer(c_int), value :: nwp
+      integer(c_int64_t), value :: size_wp
+      type(c_ptr), value :: psi
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: evr_propa_save_restart
+    end function
+    function evr_propa_read_restart(path, nwp, size_wp, psi, T, found) bind(C, name="evr_propa_read_restart")
+      import :: c_int, c_int64_t, c_double, c_ptr, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: nwp
+      integer(c_int64_t), value :: size_wp
+      type(c_ptr), value :: psi
+      real(c_double), intent(out) :: T
+      integer(c_int), intent(out) :: found
+      integer(c_int) :: evr_propa_read_restart
+    end function
+    function evr_control_field_sub(opts, nf, amp, method, nsub, field_sub) bind(C, name="evr_control_field_sub")
+      import :: c_int, c_double, evr_control_opts
+      type(evr_control_opts), intent(in) :: opts
+      integer(c_int), value :: nf, method, nsub
+      real(c_double), intent(in) :: amp(*)
+      real(c_double), intent(out) :: field_sub(*)
+      integer(c_int) :: evr_control_field_sub
+    end function
+    function evr_plan_autotune(p, nvec, flags, reps, chosen, ms_static, ms_tuned) bind(C, name="evr_plan_autotune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      integer(c_int), value :: nvec, flags, reps
+      integer(c_int), intent(out) :: chosen(*)
+      real(c_double), intent(out) :: ms_static, ms_tuned
+      integer(c_int) :: evr_plan_autotune
     end function
     function evr_davidson(p, opts, evals, evecs, info) bind(C, name="evr_davidson")
       import :: c_int, c_ptr, c_double, evr_davidson_opts, evr_davidson_info
@@ -489,6 +549,37 @@ contains
     real(c_double), intent(out) :: psiB(:)
     call evr_check(evr_grid_to_basis(plan, psiG, psiB), 'evr_GridRep_TO_BasisRep')
   end subroutine evr_GridRep_TO_BasisRep
+
+  ! BasisTOGrid / GridTOBasis of the columns of a table of psi (host arrays):
+  ! psiB(nb_packed, ncol) <-> psiG(sum_nq, ncol).
+  subroutine evr_BasisRep_TO_GridRep_n(plan, psiB, psiG)
+    type(c_ptr), intent(in) :: plan
+    real(c_double), intent(in), target, contiguous :: psiB(:, :)
+    real(c_double), intent(out), target, contiguous :: psiG(:, :)
+    call evr_check(evr_basis_to_grid_n(plan, int(size(psiB, 2), c_int), c_loc(psiB), c_loc(psiG), 0_c_int), &
+                   'evr_BasisRep_TO_GridRep_n')
+  end subroutine evr_BasisRep_TO_GridRep_n
+
+  subroutine evr_GridRep_TO_BasisRep_n(plan, psiG, psiB)
+    type(c_ptr), intent(in) :: plan
+    real(c_double), intent(in), target, contiguous :: psiG(:, :)
+    real(c_double), intent(out), target, contiguous :: psiB(:, :)
+    call evr_check(evr_grid_to_basis_n(plan, int(size(psiG, 2), c_int), c_loc(psiG), c_loc(psiB), 0_c_int), &
+                   'evr_GridRep_TO_BasisRep_n')
+  end subroutine evr_GridRep_TO_BasisRep_n
+
+  ! DerivOp_TO_RVecG(RVecG, nq, basis, tab_der) on the columns of psiG
+  ! (tab_der: 1-based modes, 0 for none).
+  subroutine evr_DerivOp_TO_RVecG(plan, psiG, dpsiG, tab_der)
+    type(c_ptr), intent(in) :: plan
+    real(c_double), intent(in), target, contiguous :: psiG(:, :)
+    real(c_double), intent(out), target, contiguous :: dpsiG(:, :)
+    integer, intent(in) :: tab_der(2)
+    integer(c_int) :: td(2)
+    td = int(tab_der, c_int)
+    call evr_check(evr_grid_deriv(plan, int(size(psiG, 2), c_int), td, c_loc(psiG), c_loc(dpsiG), 0_c_int), &
+                   'evr_DerivOp_TO_RVecG')
+  end subroutine evr_DerivOp_TO_RVecG
 
   ! nev lowest levels (and vectors, nb_packed x nev) by block Davidson.
   subroutine evr_propagation_Davidson(plan, nev, block, conv_resi, conv_ene, Ene, Vec, info)

@@ -275,6 +275,25 @@ void *evr_coupled_stream(evr_coupled *c);
 /* hpsi = H psi for nvec columns of nb0 x nb_packed rows; flags as evr_oppsi
  * (EVR_DEVICE_PTRS: device pointers on evr_coupled_stream). */
 int evr_coupled_oppsi(evr_coupled *c, int nvec, const double *psi, double *hpsi, int flags);
+/* Per-term tile autotuner, the counterpart of the reference's run-time tuning
+ * of its term loop (Tune_SG4threads_HPsi, Source_ElVibRot/sub_main/vib.f90:
+ * 1598-1705, which times H|psi> for each OpenMP thread count and keeps the
+ * fastest).  The sum-factorised term kernels come in EVR_NCLASSES work
+ * classes (threads x elements per thread x vectors per workgroup, tile
+ * shapes of growing size); the static rule gives each term the smallest
+ * class that holds its nb_t functions.  The tuner takes the terms of each
+ * static class (a bucket) in turn, times the whole sum-factorised H|psi> of
+ * nvec vectors (flags EVR_FP64 or EVR_FP32; HIP events on the plan stream,
+ * median of reps after a warm-up) with the bucket moved one and two classes
+ * up, keeps a larger class only when it is more than 3% faster, and stops
+ * at the first slower one.  The layout chosen stays with the plan
+ * (evr_oppsi, the assembly).  chosen (EVR_NCLASSES ints, or NULL): the class
+ * of each bucket; ms_static / ms_tuned: the H|psi> times of the static rule
+ * and of the tuned layout.  Results are unchanged by the layout (the same
+ * arithmetic per term). */
+#define EVR_NCLASSES 8
+int evr_plan_autotune(evr_plan *p, int nvec, int flags, int reps, int *chosen, double *ms_static,
+                      double *ms_tuned);
 /* Profiling (no counterpart in the reference, whose analogue is the
  * nb_mult_BTOG/GTOB counters, SYS:120): when on, every term-kernel launch is
  * bracketed by HIP events on the plan stream; evr_plan_kernel_stats returns,
@@ -381,6 +400,10 @@ typedef struct evr_davidson_opts {
   const evr_basis_desc *basis;
   const char *guess_file;
   const char *save_file;
+  int graph;               /* 1: the H|psi> of each iteration's new vectors is
+                              one captured HIP graph per (block size,
+                              precision), replayed on the operator's stream
+                              (results identical to graph = 0)               */
 } evr_davidson_opts;
 
 typedef struct evr_davidson_info {
@@ -540,6 +563,9 @@ typedef struct evr_control_opts {
                               recurrences and accumulations stay FP64), and J
                               is refined by one FP64 forward sweep; the
                               gradients are those of the FP32 sweeps           */
+  int graph;               /* 1: forward steps after the first replayed as one
+                              captured Chebyshev step (as evr_propa_opts.graph);
+                              results identical to graph = 0                  */
 } evr_control_opts;
 
 typedef struct evr_control_info {

@@ -135,3 +135,14 @@ def test_fp32_refined_gradient(evr, case):
     print("FP32-refined control: |dJ| %.2e, grad_a rel err %.2e" % (np.abs(J32 - J64).max(), ea))
     assert np.abs(J32 - J64).max() < 1e-12
     assert ea < 1e-4
+
+
+def test_graph_forward_sweep_identical(evr, case):
+    """Forward steps replayed as a captured Chebyshev step (graph = 1): J and
+    the gradients equal the eager sweep's to the last bit."""
+    c = case
+    J0, g0, _ = run(c, c["amp"], want=("a", "E"))
+    J1, g1, _ = c["plan"].control_gradient(c["mplan"], c["amp"], c["psi0"], c["phi1"], T, N, checkpoint=8,
+                                           npoly=100, poly_tol=1e-15, want=("a", "E"), graph=True)
+    assert np.array_equal(J0, J1)
+    assert np.array_equal(g0["a"], g1["a"]) and np.array_equal(g0["E"], g1["E"])

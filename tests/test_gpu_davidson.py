@@ -137,3 +137,25 @@ def test_davidson_save_and_restart_files(evr, oracle_mod, tmp_path):
     assert info2.iterations < info3.iterations
     got = b.read_psi(path, cplx=False, max_wp=20)
     assert got.shape[1] == 10
+
+
+def test_davidson_graph_replays_match_eager(evr):
+    """The captured H|psi> of the new-vector block (graph = 1) replayed every
+    iteration, with the driver's own eager work in between, gives the same
+    levels to the last bit as the eager driver, in FP64 and in the mixed
+    precision (whose FP32 graph and FP64 corrections alternate)."""
+    from evrhip import hamiltonian as H
+    D, L = 5, 4
+    ham = H.smolyak_config(9, D)
+    b = evr.Basis.sg4(D, L)
+    plan = evr.Plan(b, type=10, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=8, hermitian=True)
+    plan.gen_potential_poly(ham)
+    for prec in (evr.FP64, evr.FP32):
+        e0, _, i0 = plan.davidson(12, block=8, conv_resi=1e-6, conv_ene=1e-9, precision=prec, refine_resi=1e-1)
+        e1, _, i1 = plan.davidson(12, block=8, conv_resi=1e-6, conv_ene=1e-9, precision=prec, refine_resi=1e-1,
+                                  graph=True)
+        print("graph: precision %d, %d iterations, %d / %d H|psi>" % (prec, i1.iterations, i1.hpsi_fp64,
+                                                                      i1.hpsi_fp32))
+        assert i0.converged == 1 and i1.converged == 1
+        assert np.array_equal(e0, e1)
+        assert (i0.iterations, i0.hpsi_fp64, i0.hpsi_fp32) == (i1.iterations, i1.hpsi_fp64, i1.hpsi_fp32)
