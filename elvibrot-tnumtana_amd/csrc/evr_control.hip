@@ -187,6 +187,28 @@ extern "C" int evr_control_field_sub(const evr_control_opts *o, int nf, const do
   return EVR_OK;
 }
 
+extern "C" int evr_control_build_field(const evr_control_opts *o, int nf, const double *grad_E, double alpha,
+                                       double Tenv, int krotov, const double *field_old, double *field_new)
+{
+  if (!o || !grad_E || !field_new || nf < 1 || o->nsteps < 1 || !(o->T > 0.0) || !(alpha != 0.0))
+    return set_error(EVR_ERR_ARG, "bad argument");
+  if (krotov && !field_old) return set_error(EVR_ERR_ARG, "the Krotov update adds to field_old");
+  // build_field (CTRL:902-971): E_new(t) = -(1/alpha) sum_j Im(<psi_j|chi_j>
+  // <chi_j|mu|psi_j>) [x envelopp(t, Tenv) = sin^2(pi t/Tenv)], added to the
+  // old field for Krotov.  For a field held at the mid-step values that
+  // quantity is (dJ/dE_n) / (2 dt) at t_n = (n + 1/2) dt.
+  const double dt = o->T / o->nsteps;
+  for (int n = 0; n < o->nsteps; ++n) {
+    const double t = (n + 0.5) * dt;
+    const double env = Tenv > 0.0 ? std::pow(std::sin(M_PI * t / Tenv), 2) : 1.0;
+    for (int f = 0; f < nf; ++f) {
+      const size_t k = (size_t)n * nf + f;
+      field_new[k] = (krotov ? field_old[k] : 0.0) + env * grad_E[k] / (2.0 * alpha * dt);
+    }
+  }
+  return EVR_OK;
+}
+
 extern "C" int evr_control_gradient(evr_plan *H, evr_plan *mu, const evr_control_opts *o, int nf,
                                     const double *amp, const double *psi0, const double *phi1,
                                     double *J, double *grad_a, double *grad_E, double *grad_V,

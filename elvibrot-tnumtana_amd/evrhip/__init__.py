@@ -58,6 +58,19 @@ def control_field(amp, T, nsteps):
     return out
 
 
+def control_build_field(grad_E, T, alpha, Tenv=0.0, field_old=None):
+    """build_field: the Zhu-Rabitz field (or, with field_old, the Krotov update)
+    at the mid-steps from dJ/dE_n (nsteps, nf)."""
+    g = np.ascontiguousarray(grad_E, dtype=np.float64)
+    nsteps, nf = g.shape
+    o = ControlOpts(T, nsteps, 1, 0, 3, 1e-12, 0.0, 0.0)
+    old = None if field_old is None else np.ascontiguousarray(field_old, dtype=np.float64)
+    out = np.zeros((nsteps, nf))
+    check(load().evr_control_build_field(ctypes.byref(o), nf, _d(g), alpha, Tenv, 0 if old is None else 1,
+                                         _d(old), _d(out)))
+    return out
+
+
 def control_field_sub(amp, T, nsteps, method, nsub):
     """The control fields sampled for the TAYLOR (derivatives at t_n) or RK4
     (t_n, t_n + dt/2, t_n + dt) integrators: (nsteps, nsub, nf)."""

@@ -120,6 +120,8 @@ _SIGS = {
     "evr_plan_release_assembled": (ctypes.c_int, [ctypes.c_void_p]),
     "evr_plan_bounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
     "evr_control_field": (ctypes.c_int, [ctypes.POINTER(ControlOpts), ctypes.c_int, _dp, _dp]),
+    "evr_control_build_field": (ctypes.c_int, [ctypes.POINTER(ControlOpts), ctypes.c_int, _dp, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "evr_control_field_sub": (ctypes.c_int, [ctypes.POINTER(ControlOpts), ctypes.c_int, _dp, ctypes.c_int,
                                              ctypes.c_int, _dp]),
     "evr_control_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ControlOpts),

@@ -583,6 +583,16 @@ typedef struct evr_control_info {
  * layout of evr_propa_opts.field_sub: field_sub[(n*nsub + j)*nf + f]. */
 int evr_control_field_sub(const evr_control_opts *opts, int nf, const double *amp, int method, int nsub,
                           double *field_sub);
+/* The reference's field update, build_field (CTRL:902-971) with the
+ * Zhu-Rabitz / Krotov choice of sub_Opt_control (CTRL:290-520):
+ * E_new(t_n) = [E_old(t_n) if krotov] - (1/alpha) s(t_n) Im(<psi|chi>
+ * <chi|mu|psi>)(t_n), chi the target propagated back, s = envelopp(t, Tenv) =
+ * sin^2(pi t/Tenv) (Tenv <= 0: s = 1), at the mid-steps t_n = (n + 1/2) dt of
+ * the field evr_control_gradient holds; the bracket equals (dJ/dE_n)/(2 dt)
+ * with grad_E from evr_control_gradient (nsteps x nf, field[n*nf + f]).
+ * field_old / field_new: nsteps x nf. */
+int evr_control_build_field(const evr_control_opts *opts, int nf, const double *grad_E, double alpha,
+                            double Tenv, int krotov, const double *field_old, double *field_new);
 /* Field values at the mid-steps: field[n*nf + f] (amp: nk x nf, column per field). */
 int evr_control_field(const evr_control_opts *opts, int nf, const double *amp, double *field);
 /* H: symmetric plan (hermitian=1 or type 1); mu: plan of the coupling on the

@@ -125,3 +125,34 @@ def gradient(H, Mu, psi0, phi1, field, dt, Y=None, gw=None):
                 gV[:, f] += gw * np.real(np.sum((YV @ X) * YV, axis=1))
             g = V @ (np.conj(fv) * gt)                           # U_n^dagger g
     return J, gE, gV
+
+
+def build_field(H, Mu, psi0, phi1, field, dt, alpha, Tenv=0.0, field_old=None):
+    """build_field (CTRL:902-971) for one field held at mid-step values:
+    E_new(t_n) = [E_old] - (1/alpha) s(t_n) Im(<psi|chi> <chi|mu|psi>) at the
+    mid-steps t_n = (n + 1/2) dt, psi propagated forward from psi0 and chi
+    backward from phi1 exactly (exp(-i (H - E mu) dt) per step, half steps to
+    reach the middle), s = sin^2(pi t/Tenv) (envelopp) or 1."""
+    field = np.asarray(field, dtype=np.float64).ravel()
+    N = len(field)
+
+    def U(E, tau):
+        ev, V = np.linalg.eigh(H - E * Mu)
+        return (V * np.exp(-1j * ev * tau)) @ V.conj().T
+    psi = [np.asarray(psi0, complex)]
+    for n in range(N):
+        psi.append(U(field[n], dt) @ psi[-1])
+    chi = [None] * (N + 1)
+    chi[N] = np.asarray(phi1, complex)
+    for n in range(N - 1, -1, -1):
+        chi[n] = U(field[n], -dt) @ chi[n + 1]
+    out = np.zeros(N)
+    for n in range(N):
+        ph = U(field[n], dt / 2) @ psi[n]
+        ch = U(field[n], -dt / 2) @ chi[n + 1]
+        S = np.vdot(ph, ch)
+        avmu = np.vdot(ch, Mu @ ph)
+        t = (n + 0.5) * dt
+        env = np.sin(np.pi * t / Tenv) ** 2 if Tenv > 0 else 1.0
+        out[n] = -np.imag(S * avmu) / alpha * env + (0.0 if field_old is None else field_old[n])
+    return out

@@ -146,3 +146,35 @@ def test_graph_forward_sweep_identical(evr, case):
                                            npoly=100, poly_tol=1e-15, want=("a", "E"), graph=True)
     assert np.array_equal(J0, J1)
     assert np.array_equal(g0["a"], g1["a"]) and np.array_equal(g0["E"], g1["E"])
+
+
+def test_build_field_matches_reference_update(evr, case):
+    """build_field from the GPU adjoint gradient, (dJ/dE_n)/(2 alpha dt)
+    s(t_n) (the reference formula -(1/alpha) s Im(<psi|chi><chi|mu|psi>), to
+    which the oracle ties it), against the same from the oracle's exact
+    gradient (1e-9); a small step along it raises J; the Krotov form adds the
+    old field."""
+    c = case
+    N2 = 100
+    amp = c["amp"][:, :1]
+    J, g, _ = c["plan"].control_gradient(c["mplan"], amp, c["psi0"], c["phi1"], T, N2, checkpoint=20,
+                                         npoly=100, poly_tol=1e-15, want=("E",))
+    alpha = 0.5
+    Enew = evr.control_build_field(g["E"], T, alpha, Tenv=T)
+    E = evr.control_field(amp, T, N2)
+    _, gEr, _ = control.gradient(c["H"], c["Mu"], c["psi0"], c["phi1"], E, T / N2)
+    tn = (np.arange(N2) + 0.5) * (T / N2)
+    ref = gEr[:, 0] / (2 * alpha * T / N2) * np.sin(np.pi * tn / T) ** 2
+    err = np.abs(Enew[:, 0] - ref).max() / np.abs(ref).max()
+    print("build_field vs oracle: rel err %.2e" % err)
+    assert err < 1e-9
+    kro = evr.control_build_field(g["E"], T, alpha, Tenv=T, field_old=E)
+    assert np.array_equal(kro, E + Enew)
+    # ascent: J(E + eps E_new) > J(E) for a small eps
+    psi = c["psi0"][:, None]
+    J0 = np.abs(c["phi1"].conj() @ c["plan"].propagate(psi, T / N2, N2, npoly=100, poly_tol=1e-15, field=E,
+                                                       mu=c["mplan"])[0]) ** 2
+    eps = 1e-3 * np.abs(E).max() / np.abs(Enew).max()
+    J1 = np.abs(c["phi1"].conj() @ c["plan"].propagate(psi, T / N2, N2, npoly=100, poly_tol=1e-15,
+                                                       field=E + eps * Enew, mu=c["mplan"])[0]) ** 2
+    assert J1[0] > J0[0]

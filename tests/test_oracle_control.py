@@ -96,3 +96,30 @@ def test_control_field_sub_matches_oracle(evr, method, nsub):
         else:
             ref = np.stack([control.field_derivs(amp, T, n * dt + j * dt / 2, 0)[0] for j in range(3)])
         assert np.abs(got[n] - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_build_field_is_the_scaled_gradient():
+    """The oracle's build_field (-(1/alpha) Im(<psi|chi><chi|mu|psi>) at the
+    mid-steps, CTRL:902-971) equals (dJ/dE_n)/(2 alpha dt) of the oracle's
+    exact gradient (pinned by finite differences above) up to the midpoint
+    rule's error, which falls as dt^2; a sign error in either fails."""
+    rng = np.random.default_rng(3)
+    n = 12
+    A = rng.standard_normal((n, n))
+    H = (A + A.T) / 4 + np.diag(np.linspace(0, 1, n))
+    B = rng.standard_normal((n, n))
+    M = 0.2 * (B + B.T)
+    psi0 = np.zeros(n, complex); psi0[0] = 1.0
+    phi1 = np.zeros(n, complex); phi1[1] = 1.0
+    T, alpha = 20.0, 0.7
+    errs = []
+    for N in (200, 400):
+        dt = T / N
+        E = (0.3 * np.sin(np.pi * (np.arange(N) + 0.5) * dt / T))[:, None]
+        _, gE, _ = control.gradient(H, M, psi0, phi1, E, dt)
+        bf = control.build_field(H, M, psi0, phi1, E[:, 0], dt, alpha)
+        ref = gE[:, 0] / (2 * alpha * dt)
+        errs.append(np.abs(bf - ref).max() / np.abs(ref).max())
+        assert np.abs(bf + ref).max() > 1.0 * np.abs(ref).max()
+    assert errs[1] < 2e-3
+    assert 3.5 < errs[0] / errs[1] < 4.5          # the midpoint rule's dt^2
