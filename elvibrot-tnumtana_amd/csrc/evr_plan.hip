@@ -76,7 +76,9 @@ struct KClass {
 constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {1024, 1, 4}, {512, 2, 2},
                                {1024, 2, 2}, {1024, 2, 1}, {1024, 4, 1}, {1024, 8, 1}};
 constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
-constexpr size_t kLdsLimit = 160 * 1024 - 1024;
+// dynamic LDS per workgroup: 160 KiB less a reserve for the kernels' static
+// arrays (under 2 KiB for the term kernels)
+constexpr size_t kLdsLimit = 160 * 1024 - 4096;
 
 }  // namespace
 
@@ -298,7 +300,7 @@ bool layout_term(const evr_plan *p, evr::TermDev &td, int c, std::vector<evr::St
   }
   const int64_t nbt = td.nbt;
   const int nvw = kClasses[c].nvw;
-  const size_t keo_bufs = kClasses[c].ept <= 2 ? 3 : 1;  // LDS buffers of the KEO path
+  const size_t keo_bufs = kClasses[c].ept <= 2 ? (evr::keo_pencil(kClasses[c].nt, kClasses[c].ept, nvw) ? 6 : 3) : 1;
   const size_t base = td.lds_mats + (size_t)nbt * nvw;
   const size_t keo_words = p->keo_layout ? keo_bufs * nbt * nvw : 0;
   // half the LDS when the term fits in it, so two workgroups share a CU
@@ -2402,3 +2404,4 @@ int evr::plan_grid_to_user(const evr_plan *p, const double *term_grid, double *u
     std::copy(term_grid, term_grid + p->sum_nq, user);
   return EVR_OK;
 }
+

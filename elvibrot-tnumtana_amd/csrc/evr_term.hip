@@ -17,6 +17,19 @@
 
 namespace evr {
 
+// Term classes whose kinetic part runs as pencil passes: those with two
+// units per thread (nb_t from 257 to 2048 at one or two vectors).  Measured
+// on the D=9 L=8 operator (32 vectors, FP64): 512x2x2 21.9 -> 16.2 ms,
+// 1024x2x2 45.7 -> 42.7 ms, 1024x2x1 52.5 -> 45.6 ms; the one-unit classes
+// (small terms, few pencils per barrier) are faster per unit (0.98 against
+// 1.68 ms at 256x1x4).
+constexpr bool keo_pencil(int nt, int ept, int nvw)
+{
+  (void)nt;
+  (void)nvw;
+  return ept == 2;
+}
+
 template <typename T>
 __device__ __forceinline__ T apply_mode(const T *__restrict__ M, const T *__restrict__ buf,
                                         int e, int bd, int nbd, int st)
@@ -43,6 +56,23 @@ __device__ __forceinline__ T apply_k(const T *__restrict__ row, const T *__restr
   T s = T(0);
   for (int b = 0; b < nbd; ++b) s = fma(row[nbd * b], col[b * st], s);
   return s;
+}
+
+// A contraction step read from LDS, made wave-uniform (scalar registers:
+// the step's matrix is then read with scalar loads)
+__device__ __forceinline__ StepDev uniform_step(const StepDev &s)
+{
+  StepDev r;
+  r.n1 = __builtin_amdgcn_readfirstlane(s.n1);
+  r.n3 = __builtin_amdgcn_readfirstlane(s.n3);
+  r.nin = __builtin_amdgcn_readfirstlane(s.nin);
+  r.nout = __builtin_amdgcn_readfirstlane(s.nout);
+  r.rn1 = s.rn1;
+  r.pad = 0;
+  const uint64_t mat = (uint64_t)s.mat;
+  r.mat = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(mat >> 32)) << 32) |
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)mat));
+  return r;
 }
 
 // Integer division of x < 2^24 by a wave-uniform d via the float reciprocal
@@ -135,6 +165,69 @@ __device__ __forceinline__ void contract(const T *__restrict__ in, T *__restrict
   contract_gen<T, NT, NVW>(in, out, M, n1, nin, nout, n3);
 }
 
+// out (=, or += scale *) the factor K (nb x nb, K[a + nb b] = K(a, b))
+// applied along one mode of the unit tensor `in` (mode stride st in units):
+// one thread per pencil; the pencil is read from LDS once and the factor, at
+// a wave-uniform global address, sits in scalar registers.
+template <typename T, int NT, int NB>
+__device__ __forceinline__ void pencil_fix(const T *__restrict__ K, const T *__restrict__ in,
+                                           T *__restrict__ out, int st, int npen, bool add, T scale)
+{
+  T k[NB * NB];
+#pragma unroll
+  for (int i = 0; i < NB * NB; ++i) k[i] = K[i];
+  const float rst = 1.0f / (float)st;
+  for (int pp = threadIdx.x; pp < npen; pp += NT) {
+    const int base = pp + fdiv(pp, st, rst) * st * (NB - 1);
+    T v[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) v[b] = in[base + b * st];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      T s = k[a] * v[0];
+#pragma unroll
+      for (int b = 1; b < NB; ++b) s = fma(k[a + NB * b], v[b], s);
+      T *o = out + base + a * st;
+      *o = add ? fma(scale, s, *o) : s;
+    }
+  }
+}
+
+// pencil_fix for any nb: the factor read from its LDS copy and the pencil
+// re-read per result (few registers; sizes without a fixed variant are rare)
+template <typename T, int NT>
+__device__ __forceinline__ void pencil_gen(const T *__restrict__ K, const T *__restrict__ in,
+                                           T *__restrict__ out, int nb, int st, int npen, bool add,
+                                           T scale)
+{
+  const float rst = 1.0f / (float)st;
+  for (int pp = threadIdx.x; pp < npen; pp += NT) {
+    const int base = pp + fdiv(pp, st, rst) * st * (nb - 1);
+    for (int a = 0; a < nb; ++a) {
+      T s = T(0);
+      for (int b = 0; b < nb; ++b) s = fma(K[a + nb * b], in[base + b * st], s);
+      T *o = out + base + a * st;
+      *o = add ? fma(scale, s, *o) : s;
+    }
+  }
+}
+
+// One factor along one mode over nunit units; Kg: the factor in global
+// memory (wave-uniform address), Kl: its LDS copy.  Fixed variants for the
+// odd sizes of the Hermite levels that fill the large terms (3 and 5
+// functions); more fixed sizes cost registers in every term class.
+template <typename T, int NT>
+__device__ __forceinline__ void pencil_pass(const T *Kg, const T *Kl, const T *in, T *out, int nb, int st,
+                                            int nunit, bool add, T scale)
+{
+  const int npen = nunit / nb;
+  switch (nb) {
+    case 3: pencil_fix<T, NT, 3>(Kg, in, out, st, npen, add, scale); return;
+    case 5: pencil_fix<T, NT, 5>(Kg, in, out, st, npen, add, scale); return;
+    default: pencil_gen<T, NT>(Kl, in, out, nb, st, npen, add, scale); return;
+  }
+}
+
 // Unit u = e*NVW + v handled by thread tid in its r-th slot.  When each thread
 // holds as many units as the tile has vectors, it takes the NVW vectors of one
 // element (element-major: shared digits, factor rows and coefficient loads);
@@ -190,35 +283,44 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   const int nunit = nbt * NVW;
   const int64_t nvec = asmb ? nbt : p.nvec;  // assembly: columns of the term
 
+  // slot table of the term: one load per mode in parallel, then the LDS
+  // layout (prefix sums) by one thread from LDS
+  __shared__ int64_t s_gB[kMaxDev], s_gW[kMaxDev], s_gK[kMaxDev];
+  __shared__ int s_mats;
+  if (tid < D) {
+    const SlotDev &s = p.slots[td.slot[tid]];
+    s_nb[tid] = s.nb;
+    s_nq[tid] = s.nq;
+    s_gB[tid] = s.offB;
+    s_gW[tid] = s.offW;
+    s_gK[tid] = s.offK;
+  }
+  __syncthreads();
   if (tid == 0) {
     int off = 0, st = NVW;
     for (int d = 0; d < D; ++d) {
-      const SlotDev &s = p.slots[td.slot[d]];
-      s_nb[d] = s.nb;
-      s_nq[d] = s.nq;
+      const int nb = s_nb[d], nq = s_nq[d];
       s_st[d] = st;  // unit stride of mode d
-      st *= s.nb;
+      st *= nb;
       s_offB[d] = off;
-      off += s.nq * s.nb;
+      off += nq * nb;
       s_offW[d] = off;
-      off += s.nq * s.nb;
+      off += nq * nb;
       s_offK[d] = off;
-      if (p.hasKeo) off += kKeoFactors * s.nb * s.nb;
+      if (p.hasKeo) off += kKeoFactors * nb * nb;
     }
+    s_mats = off;
   }
   __syncthreads();
-  // stage the 1D matrices of this term in LDS
-  for (int d = 0; d < D; ++d) {
-    const SlotDev &s = p.slots[td.slot[d]];
-    const int nbq = s.nb * s.nq;
-    for (int i = tid; i < nbq; i += NT) {
-      smem[s_offB[d] + i] = p.bw[s.offB + i];
-      smem[s_offW[d] + i] = p.bw[s.offW + i];
-    }
-    if (p.hasKeo) {
-      const int nk = kKeoFactors * s.nb * s.nb;
-      for (int i = tid; i < nk; i += NT) smem[s_offK[d] + i] = p.keo[s.offK + i];
-    }
+  // stage the 1D matrices of this term in LDS: one flat pass over all modes
+  // (every load in flight at once, not one mode after another)
+  for (int i = tid; i < s_mats; i += NT) {
+    int d = 0;
+    while (d + 1 < D && s_offB[d + 1] <= i) ++d;
+    const int nbq = s_nb[d] * s_nq[d];
+    const int l = i - s_offB[d];
+    smem[i] = l < nbq ? p.bw[s_gB[d] + l]
+                      : (l < 2 * nbq ? p.bw[s_gW[d] + (l - nbq)] : p.keo[s_gK[d] + (l - 2 * nbq)]);
   }
   T *sx = smem + td.lds_mats;
   T *work = sx + nunit;
@@ -262,11 +364,200 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   }
 
   // ---------------------------------------------------------------- KEO --
-  // Accumulators M_i, N_i, O_i (final factors P_i, R_i, T_i or T2_i) of a
-  // group of IG modes live in registers; the coefficient strips are
-  // pre-zeroed on the host (no branches, no index tests) and read with
-  // wave-uniform scalar loads.
+  // Two forms of the kinetic part for up to two units per thread, chosen per
+  // term class (keo_pencil): one-mode factors applied by pencil passes
+  // through LDS buffers (a thread per pencil, the factor in scalar
+  // registers; fewer LDS reads, more barriers), or per unit (each result
+  // reads its pencil and factor row; no extra buffers).  Accumulators M_i,
+  // N_i, O_i of a group of IG active modes live in registers per unit; the
+  // coefficient strips are pre-zeroed on the host (no branches, no index
+  // tests) and read at wave-uniform addresses.
   if constexpr (EPT <= 2) {
+  if constexpr (keo_pencil(NT, EPT, NVW)) {
+    if (hmode && p.hasKeo) {
+      T *Zb = work;              // P_j x
+      T *Sb = work + nunit;      // S_j x; M_i stage
+      T *Qb = work + 2 * nunit;  // Q_j x; N_i stage
+      T *Wb = work + 3 * nunit;  // Q_k Z_j (two buffers, alternating); O_i stage
+      T *Yb = work + 5 * nunit;  // sum of the final factor passes
+      // this term's coefficient set (strips in its mode positions)
+      const double *kS = p.keoc + (int64_t)td.cix * p.keoc_stride;  // [j*Dp + i]
+      const double *kN = kS + kDp * kDp;            // [j*Dp + i]
+      const double *kO = kN + kDp * kDp;            // [k*Dp + i]
+      const double *kC = kO + kDp * kDp;            // [j*Dp + i]
+      const double *kCd = kC + kDp * kDp;           // [i]
+      const double *kU = kCd + kDp;                 // [i]
+      const double *kF = kU + kDp;                  // [i]
+      const double *kA = kF + kDp;                  // [(j*Dp + k)*Dp + i], with Ga only
+      const int fdiag = p.diagT2 ? 6 : 2;
+      // Modes outside amode[] have one function and vanishing P, Q, R (odd
+      // integrands on the symmetric quadrature): every product holding one of
+      // those factors drops, the others fold into scalars of this term:
+      //   c_ii Td_i + a_iii U_i           -> s_f0 x
+      //   a_iik Td_i Q_k  (k active)      -> s_fq[k] Q_k x
+      //   a_ijj P_i S_j   (i active)      -> s_fs[i] added to M_i
+      __shared__ T s_fq[kMaxDev], s_fs[kMaxDev], s_f0;
+      __shared__ int s_koff[kMaxDev];  // offset of mode d's factors in p.keo
+      const int na = td.na;
+      if (tid == 0) {
+        bool act[kMaxDev];
+        for (int d = 0; d < kMaxDev; ++d) {
+          act[d] = false;
+          s_fq[d] = T(0);
+          s_fs[d] = T(0);
+          s_koff[d] = d < D ? p.slots[td.slot[d]].offK : 0;
+        }
+        for (int a = 0; a < na; ++a) act[td.amode[a]] = true;
+        T f0 = T(0);
+        for (int i = 0; i < D; ++i) {
+          if (act[i]) continue;
+          const T *Ki = smem + s_offK[i];  // 1 x 1 factors
+          f0 += T(kCd[i]) * Ki[fdiag];
+          if (p.hasA) {
+            f0 += T(kU[i]) * Ki[5];
+            for (int k = 0; k < D; ++k)
+              if (act[k]) {
+                s_fq[k] += T(kO[k * kDp + i]) * Ki[fdiag];
+                s_fs[k] += T(kS[i * kDp + k]) * Ki[4];
+              }
+          }
+        }
+        s_f0 = f0;
+      }
+      for (int u = tid; u < nunit; u += NT) Yb[u] = T(0);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        const int u = unit_of<NT, EPT, NVW>(tid, r);
+        if (u < nunit) yown[r] = s_f0 * sx[u];
+      }
+      // factor f of mode d applied along d: out = K in, or out += scale K in
+      auto pen = [&](int d, int f, const T *in, T *out, bool add, T scale) {
+        const int nbd = s_nb[d];
+        const int off = __builtin_amdgcn_readfirstlane(s_koff[d] + f * nbd * nbd);
+        pencil_pass<T, NT>(p.keo + off, smem + s_offK[d] + f * nbd * nbd, in, out, nbd, s_st[d], nunit,
+                           add, scale);
+      };
+      auto keo_groups = [&](auto igc) {
+      constexpr int IG = decltype(igc)::value;
+      for (int i0 = 0; i0 < na; i0 += IG) {
+        T accM[IG][EPT], accN[IG][EPT], accO[IG][EPT];
+#pragma unroll
+        for (int ii = 0; ii < IG; ++ii)
+#pragma unroll
+          for (int r = 0; r < EPT; ++r) accM[ii][r] = accN[ii][r] = accO[ii][r] = T(0);
+        // strip column of each accumulator: the active mode, or the all-zero
+        // padding column kDp - 1 past the end of the list
+        int icol[IG];
+#pragma unroll
+        for (int ii = 0; ii < IG; ++ii) icol[ii] = i0 + ii < na ? td.amode[i0 + ii] : kDp - 1;
+        for (int ja = 0; ja < na; ++ja) {
+          const int j = td.amode[ja];
+          pen(j, 0, sx, Zb, false, T(1));              // Z_j = P_j x
+          if (p.hasA) {
+            pen(j, 4, sx, Sb, false, T(1));            // S_j x
+            pen(j, 1, sx, Qb, false, T(1));            // Q_j x
+          }
+          __syncthreads();
+          {
+            T cz[IG], cs[IG], cn[IG], co[IG];
+            const T fq = s_fq[j];
+#pragma unroll
+            for (int ii = 0; ii < IG; ++ii) {
+              cz[ii] = T(kC[j * kDp + icol[ii]]);
+              cs[ii] = T(kS[j * kDp + icol[ii]]);
+              cn[ii] = T(kN[j * kDp + icol[ii]]);
+              co[ii] = T(kO[j * kDp + icol[ii]]);
+            }
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) {
+              const int u = unit_of<NT, EPT, NVW>(tid, r);
+              if (u < nunit) {
+                const T zj = Zb[u];
+#pragma unroll
+                for (int ii = 0; ii < IG; ++ii) accM[ii][r] = fma(cz[ii], zj, accM[ii][r]);
+                if (p.hasA) {
+                  const T sxv = Sb[u], qxv = Qb[u];
+#pragma unroll
+                  for (int ii = 0; ii < IG; ++ii) {
+                    accM[ii][r] = fma(cs[ii], sxv, accM[ii][r]);
+                    accN[ii][r] = fma(cn[ii], zj, accN[ii][r]);
+                    accO[ii][r] = fma(co[ii], qxv, accO[ii][r]);
+                  }
+                  if (i0 == 0) yown[r] = fma(fq, qxv, yown[r]);
+                }
+              }
+            }
+          }
+          if (p.hasA) {
+            // Q_k Z_j for the other active modes, into two alternating
+            // buffers: a buffer is rewritten only after the barrier that
+            // follows the pass after its last read
+            int nk = 0;
+            for (int ka = 0; ka < na; ++ka) {
+              const int k = td.amode[ka];
+              if (k == j) continue;
+              T *Wk = Wb + (nk & 1) * nunit;
+              ++nk;
+              pen(k, 1, Zb, Wk, false, T(1));
+              __syncthreads();
+              T ca[IG];
+#pragma unroll
+              for (int ii = 0; ii < IG; ++ii) ca[ii] = T(kA[(j * kDp + k) * kDp + icol[ii]]);
+#pragma unroll
+              for (int r = 0; r < EPT; ++r) {
+                const int u = unit_of<NT, EPT, NVW>(tid, r);
+                if (u < nunit) {
+                  const T qz = Wk[u];
+#pragma unroll
+                  for (int ii = 0; ii < IG; ++ii) accM[ii][r] = fma(ca[ii], qz, accM[ii][r]);
+                }
+              }
+            }
+          }
+          __syncthreads();
+        }
+        // Y += P_i (M_i + (s_fs_i + F1_i) x) + R_i N_i + Tdiag_i (O_i + c_ii x) + a_iii U_i x
+#pragma unroll
+        for (int ii = 0; ii < IG; ++ii) {
+          if (i0 + ii >= na) break;
+          const int i = icol[ii];
+          const T cd = T(kCd[i]), cu = T(kU[i]), fs = s_fs[i] + (p.hasF1 ? T(kF[i]) : T(0));
+#pragma unroll
+          for (int r = 0; r < EPT; ++r) {
+            const int u = unit_of<NT, EPT, NVW>(tid, r);
+            if (u < nunit) {
+              Sb[u] = fma(fs, sx[u], accM[ii][r]);
+              Qb[u] = accN[ii][r];
+              Wb[u] = fma(cd, sx[u], accO[ii][r]);
+            }
+          }
+          __syncthreads();
+          pen(i, 0, Sb, Yb, true, T(1));              // P_i M_i
+          pen(i, fdiag, Wb, Yb, true, T(1));          // T_i O_i
+          if (p.hasA) {
+            pen(i, 3, Qb, Yb, true, T(1));            // R_i N_i
+            pen(i, 5, sx, Yb, true, cu);              // a_iii U_i x
+          }
+          __syncthreads();
+        }
+      }
+      };
+      // group size fitted to the active modes (no accumulators for absent ones)
+      if constexpr (EPT == 1) {
+        if (na <= 4) keo_groups(std::integral_constant<int, 4>{});
+        else keo_groups(std::integral_constant<int, 8>{});
+      } else {
+        if (na <= 3) keo_groups(std::integral_constant<int, 3>{});
+        else keo_groups(std::integral_constant<int, 6>{});
+      }
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        const int u = unit_of<NT, EPT, NVW>(tid, r);
+        if (u < nunit) yown[r] += Yb[u];
+      }
+    }
+  } else {
     if (hmode && p.hasKeo) {
       T *zb = work;
       T *zb2 = work + nunit;
@@ -454,6 +745,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         else keo_groups(std::integral_constant<int, 5>{});
       }
     }
+  }
   } else {
   if (hmode && p.hasKeo) {
     constexpr int IG = IGroup<EPT>::value;
@@ -638,7 +930,13 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         rb = q;
       }
     }
-    const StepDev *stp = p.steps + (red ? td.vstep0 : td.step0);
+    // the 2m contraction steps of the slab, staged in LDS by one parallel
+    // load (read back as wave-uniform values)
+    __shared__ StepDev s_stp[2 * kMaxDev];
+    {
+      const StepDev *stp = p.steps + (red ? td.vstep0 : td.step0);
+      if (tid < 2 * m) s_stp[tid] = stp[tid];
+    }
     __syncthreads();
     for (int qB = 0; qB < QBg; ++qB) {
       for (int bB = tid; bB < nbB; bB += NT) {
@@ -684,7 +982,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         }
         __syncthreads();
         for (int d = 0; d < m; ++d) {
-          const StepDev st = stp[d];
+          const StepDev st = uniform_step(s_stp[d]);
           contract<T, NT, NVW>(cur, nxt, p.bw + st.mat, st.n1, st.nin, st.nout, st.n3);
           T *tmp = cur; cur = nxt; nxt = tmp;
           __syncthreads();
@@ -708,7 +1006,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       }
       if (p.mode != kModeB2G) {
         for (int d = 0; d < m; ++d) {
-          const StepDev st = stp[m + d];
+          const StepDev st = uniform_step(s_stp[m + d]);
           contract<T, NT, NVW>(cur, nxt, p.bw + st.mat, st.n1, st.nin, st.nout, st.n3);
           T *tmp = cur; cur = nxt; nxt = tmp;
           __syncthreads();

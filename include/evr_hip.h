@@ -377,7 +377,8 @@ int evr_grid_deriv(evr_plan *p, int ncol, const int *tab_der, const double *psiG
 typedef struct evr_davidson_opts {
   int nev;                 /* levels wanted                                     */
   int block;               /* new vectors per iteration (<= plan max_nvec)      */
-  int max_sub;             /* subspace size before a collapse (0: auto)         */
+  int max_sub;             /* subspace size before a collapse (0: the larger of
+                              4 nev and nev + 4 block)                          */
   int max_it;              /* iterations (0: 200)                               */
   double conv_resi;        /* ||H x - E x|| threshold (energy units)            */
   double conv_ene;         /* |E(it) - E(it-1)| threshold                       */
