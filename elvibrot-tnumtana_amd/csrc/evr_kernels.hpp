@@ -7,7 +7,9 @@
 namespace evr {
 
 constexpr int kMaxDev = 12;   // modes supported by the term kernel (5-bit digits in a u64)
-constexpr int kMaxN = 20;     // largest nb or nq of a 1D slot
+constexpr int kMaxNb = 31;    // largest nb of a 1D slot (5-bit basis digits)
+constexpr int kMaxNq = 64;    // largest nq of a 1D slot
+constexpr int kGenCache = 20; // slab contractions of up to this many inputs keep them in registers
 constexpr int kKeoFactors = 7;
 constexpr int kDp = 16;       // padded stride of the KEO coefficient strips
 
@@ -16,15 +18,15 @@ struct TermDev {
   int64_t off_nb;             // start of the term in the concatenated basis (map, R)
   int64_t off_nq;             // start of the term in the concatenated grid (V)
   int nbt, nqt;               // basis / grid size of the term
-  int m;                      // modes 0..m-1 are contracted inside an LDS slab
-  int nbA, QA, nbB, QB;       // inner/outer basis and grid sizes
   int lds_mats;               // doubles of 1D matrices staged in LDS
   double w;                   // Smolyak weight (SGT2:754-854)
   int slot[kMaxDev];          // 1D slot of each mode
-  int step0;                  // first of the 2m slab contraction steps (StepDev table)
-  // Reduced grid of the H|psi> V part: one-function modes (nb = 1) are summed
-  // into V_eff once per potential (k_fold_v), so the slab walks only the Dv
-  // modes vmode[0..Dv-1] (split fields as above, over those modes).
+  // Reduced grid of the slab: the Dv modes vmode[0..Dv-1] with more than one
+  // function.  H|psi> has the one-function modes summed into V_eff once per
+  // potential (k_fold_v); the transforms expand / contract them on the fly.
+  // Reduced modes 0..vm-1 are contracted inside an LDS slab (vnbA functions,
+  // vQA points), the others (vnbB, vQB) through one Kronecker row per outer
+  // point; vstep0: first of the 2 vm slab contraction steps (StepDev table).
   int Dv;
   int vmode[kMaxDev];
   int vm, vnbA, vQA, vnbB, vQB, vstep0;

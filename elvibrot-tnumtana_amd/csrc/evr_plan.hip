@@ -77,7 +77,7 @@ constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {1024, 1, 4}, {512, 2
                                {1024, 2, 2}, {1024, 2, 1}, {1024, 4, 1}, {1024, 8, 1}};
 constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
 // dynamic LDS per workgroup: 160 KiB less a reserve for the kernels' static
-// arrays (under 2 KiB for the term kernels)
+// arrays (about 2 KiB for the term kernels)
 constexpr size_t kLdsLimit = 160 * 1024 - 4096;
 
 }  // namespace
@@ -286,17 +286,17 @@ double term_flops(const evr_plan *p, const evr::TermDev &td)
   return f;
 }
 
-// Slab split of term td for kernel class c (the largest first-m-modes slab
-// of each mode list that fits the class's LDS budget) and its 2m contraction
-// steps, B->G then G->B, appended to steps; words: LDS doubles needed.
+// Slab split of term td for kernel class c over its reduced grid (the modes
+// with more than one function; the largest first-m-modes slab that fits the
+// class's LDS budget) and its 2m contraction steps, B->G then G->B, appended
+// to steps; words: LDS doubles needed.
 bool layout_term(const evr_plan *p, evr::TermDev &td, int c, std::vector<evr::StepDev> &steps, size_t &words)
 {
   const int D = p->D;
-  int nbd[evr::kMaxDev], nqd[evr::kMaxDev], fullm[evr::kMaxDev];
+  int nbd[evr::kMaxDev], nqd[evr::kMaxDev];
   for (int d = 0; d < D; ++d) {
     nbd[d] = p->slots[td.slot[d]].nb;
     nqd[d] = p->slots[td.slot[d]].nq;
-    fullm[d] = d;
   }
   const int64_t nbt = td.nbt;
   const int nvw = kClasses[c].nvw;
@@ -351,8 +351,7 @@ bool layout_term(const evr_plan *p, evr::TermDev &td, int c, std::vector<evr::St
     words = std::max(words, base + std::max<size_t>(keo_words, 2 * (size_t)QA_ * nvw + 2 * nbB_));
     return true;
   };
-  return split(fullm, D, td.m, td.nbA, td.QA, td.nbB, td.QB, td.step0) &&
-         split(td.vmode, td.Dv, td.vm, td.vnbA, td.vQA, td.vnbB, td.vQB, td.vstep0);
+  return split(td.vmode, td.Dv, td.vm, td.vnbA, td.vQA, td.vnbB, td.vQB, td.vstep0);
 }
 
 // Lays out every term for its class p->tclass[t]: slab splits and steps, LDS
@@ -891,8 +890,8 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->slots.resize(ns);
   for (int s = 0; s < ns; ++s) {
     const int nb = bd->nb[s], nq = bd->nq[s];
-    if (nb < 1 || nq < nb || nq > evr::kMaxN)
-      return fail(EVR_ERR_ARG, "each slot needs 1 <= nb <= nq <= 20");
+    if (nb < 1 || nq < nb || nb > evr::kMaxNb || nq > evr::kMaxNq)
+      return fail(EVR_ERR_ARG, "each slot needs 1 <= nb <= nq, nb <= 31, nq <= 64");
     evr::SlotDev &sd = p->slots[s];
     sd.nb = nb;
     sd.nq = nq;

@@ -129,17 +129,31 @@ __device__ __forceinline__ void contract_gen(const T *__restrict__ in, T *__rest
   const int npen = n1 * n3 * NVW;
   const int n1v = n1 * NVW;
   const float rn1 = 1.0f / (float)n1v;
+  if (nin > kGenCache) {
+    // long inputs: each result re-reads its pencil from LDS
+    for (int pp = threadIdx.x; pp < npen; pp += NT) {
+      const int i3 = fdiv(pp, n1v, rn1);
+      const int i1 = pp - i3 * n1v;
+      const T *src = in + i1 + n1v * nin * i3;
+      for (int q = 0; q < nout; ++q) {
+        T s = T(0);
+        for (int k = 0; k < nin; ++k) s = fma(M[q + nout * k], src[n1v * k], s);
+        out[i1 + n1v * (q + nout * i3)] = s;
+      }
+    }
+    return;
+  }
   for (int pp = threadIdx.x; pp < npen; pp += NT) {
     const int i3 = fdiv(pp, n1v, rn1);
     const int i1 = pp - i3 * n1v;  // (i1, v) combined
-    T vin[kMaxN];
+    T vin[kGenCache];
 #pragma unroll
-    for (int k = 0; k < kMaxN; ++k)
+    for (int k = 0; k < kGenCache; ++k)
       if (k < nin) vin[k] = in[i1 + n1v * (k + nin * i3)];
     for (int q = 0; q < nout; ++q) {
       T s = T(0);
 #pragma unroll
-      for (int k = 0; k < kMaxN; ++k)
+      for (int k = 0; k < kGenCache; ++k)
         if (k < nin) s = fma(M[q + nout * k], vin[k], s);
       out[i1 + n1v * (q + nout * i3)] = s;
     }
@@ -887,26 +901,52 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   // ------------------------------------------------------------- V part --
   const bool vpart = (hmode && p.hasV) || p.mode == kModeB2G || p.mode == kModeG2B;
   if (vpart) {
-    // H|psi> walks the reduced grid (one-function modes folded into V_eff,
-    // p.V); the basis <-> grid transforms walk the full term grid.
-    const bool red = hmode;
-    const int Dg = red ? td.Dv : D;
-    const int m = red ? td.vm : td.m, QA = red ? td.vQA : td.QA, nbA = red ? td.vnbA : td.nbA,
-              nbB = red ? td.vnbB : td.nbB, QBg = red ? td.vQB : td.QB;
-    const int64_t goff = red ? td.off_vq : td.off_nq;
+    // The slab walks the reduced grid: the modes with more than one function.
+    // H|psi> has the one-function modes folded into V_eff (p.V); the basis
+    // <-> grid transforms expand the reduced values over those modes on the
+    // way out (B->G: value x prod_d B_d[q_d, 0]) and contract them on the way
+    // in (G->B: sum over their points of value x prod_d W_d[0, q_d]), so the
+    // contractions run on the reduced grid only.
+    const int Dg = td.Dv;
+    const int m = td.vm, QA = td.vQA, nbA = td.vnbA, nbB = td.vnbB, QBg = td.vQB;
+    const int64_t goff = hmode ? td.off_vq : td.off_nq;
     T *slab0 = work;
     T *slab1 = work + QA * NVW;
     T *crow = work + 2 * QA * NVW;
     T *crowW = crow + nbB;
     __shared__ int s_qd[kMaxDev];  // digits of the current outer grid point
     __shared__ int sv_nb[kMaxDev], sv_nq[kMaxDev], sv_offB[kMaxDev], sv_offW[kMaxDev];
+    // transforms: full-grid stride of each reduced position, the one-function
+    // modes (points, full-grid stride, B and W offsets in LDS), the count of
+    // their point combinations and the full-grid offset of the outer point
+    __shared__ int s_fst[kMaxDev], s_inq[kMaxDev], s_ifst[kMaxDev], s_ioB[kMaxDev], s_ioW[kMaxDev];
+    __shared__ int s_ni, s_ncomb, s_bofs;
     if (tid < kMaxDev) s_qd[tid] = 0;
     if (tid < Dg) {
-      const int d = red ? td.vmode[tid] : tid;
+      const int d = td.vmode[tid];
       sv_nb[tid] = s_nb[d];
       sv_nq[tid] = s_nq[d];
       sv_offB[tid] = s_offB[d];
       sv_offW[tid] = s_offW[d];
+    }
+    if (tid == 0 && !hmode) {
+      int fst = 1, ni = 0, nc = 1, k = 0;
+      for (int d = 0; d < D; ++d) {
+        if (k < Dg && td.vmode[k] == d) {
+          s_fst[k++] = fst;
+        } else {
+          s_inq[ni] = s_nq[d];
+          s_ifst[ni] = fst;
+          s_ioB[ni] = s_offB[d];
+          s_ioW[ni] = s_offW[d];
+          nc *= s_nq[d];
+          ++ni;
+        }
+        fst *= s_nq[d];
+      }
+      s_ni = ni;
+      s_ncomb = nc;
+      s_bofs = 0;
     }
     __syncthreads();
     // per-unit offsets of the outer accumulation (no division inside leaves)
@@ -934,7 +974,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
     // load (read back as wave-uniform values)
     __shared__ StepDev s_stp[2 * kMaxDev];
     {
-      const StepDev *stp = p.steps + (red ? td.vstep0 : td.step0);
+      const StepDev *stp = p.steps + td.vstep0;
       if (tid < 2 * m) s_stp[tid] = stp[tid];
     }
     __syncthreads();
@@ -988,19 +1028,63 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
           __syncthreads();
         }
         if (p.mode == kModeB2G) {
-          for (int i2 = tid; i2 < QA * NVW; i2 += NT) {
-            const int i = i2 / NVW, c = i2 - i * NVW;
-            if (v0 + c < nvec) p.G[(int64_t)(v0 + c) * p.gld + gbase + i] = cur[i2];
+          // reduced point i of this slab -> every full-grid point over the
+          // one-function modes (combination outermost: neighbouring threads
+          // write neighbouring reduced points)
+          const int ni = s_ni, nc = s_ncomb;
+          for (int cb = 0; cb < nc; ++cb) {
+            int r2 = cb, coff = s_bofs;
+            T sc = T(1);
+            for (int k = 0; k < ni; ++k) {
+              const int q = r2 % s_inq[k];
+              r2 /= s_inq[k];
+              coff += q * s_ifst[k];
+              sc *= smem[s_ioB[k] + q];
+            }
+            for (int i2 = tid; i2 < QA * NVW; i2 += NT) {
+              const int i = i2 / NVW, c = i2 - i * NVW;
+              if (v0 + c >= nvec) continue;
+              int rem = i, off = coff;
+              for (int k = 0; k < m; ++k) {
+                const int q = rem % sv_nq[k];
+                rem /= sv_nq[k];
+                off += q * s_fst[k];
+              }
+              p.G[(int64_t)(v0 + c) * p.gld + goff + off] = sc * cur[i2];
+            }
           }
         } else {
           for (int i2 = tid; i2 < QA * NVW; i2 += NT) cur[i2] *= p.V[gbase + i2 / NVW];
         }
         __syncthreads();
       } else {
-        // grid input: column v0 + c at Gin + (v0 + c) * gld (lanes past nvec read 0)
+        // grid input: column v0 + c at Gin + (v0 + c) * gld (lanes past nvec
+        // read 0), contracted over the one-function modes with their W rows
+        const int ni = s_ni, nc = s_ncomb;
         for (int i2 = tid; i2 < QA * NVW; i2 += NT) {
           const int i = i2 / NVW, c = i2 - i * NVW;
-          cur[i2] = v0 + c < nvec ? p.Gin[(int64_t)(v0 + c) * p.gld + gbase + i] : T(0);
+          T acc = T(0);
+          if (v0 + c < nvec) {
+            int rem = i, off = s_bofs;
+            for (int k = 0; k < m; ++k) {
+              const int q = rem % sv_nq[k];
+              rem /= sv_nq[k];
+              off += q * s_fst[k];
+            }
+            const T *Gc = p.Gin + (int64_t)(v0 + c) * p.gld + goff + off;
+            for (int cb = 0; cb < nc; ++cb) {
+              int r2 = cb, coff = 0;
+              T sc = T(1);
+              for (int k = 0; k < ni; ++k) {
+                const int q = r2 % s_inq[k];
+                r2 /= s_inq[k];
+                coff += q * s_ifst[k];
+                sc *= smem[s_ioW[k] + q];
+              }
+              acc = fma(sc, Gc[coff], acc);
+            }
+          }
+          cur[i2] = acc;
         }
         __syncthreads();
       }
@@ -1022,6 +1106,11 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         for (int d = m; d < Dg; ++d) {
           if (++s_qd[d] < sv_nq[d]) break;
           s_qd[d] = 0;
+        }
+        if (!hmode) {
+          int bo = 0;
+          for (int d = m; d < Dg; ++d) bo += s_qd[d] * s_fst[d];
+          s_bofs = bo;
         }
       }
       __syncthreads();
