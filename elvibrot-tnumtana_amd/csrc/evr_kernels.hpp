@@ -36,7 +36,7 @@ struct TermDev {
   int na;
   int amode[kMaxDev];
   // Assembled operator: start of the term's tiled nb_t x nb_t matrix (MFMA
-  // A-fragment order, see asm_index2) in the assembled pools.
+  // A-fragment order, see asm_index2 / asm_index4) in the assembled pools.
   int64_t off_asm;
   // Coefficient set of the term (index into the per-set KEO coefficients:
   // keoc + cix * keoc_stride, coefC + cix * D * D, coefF1 + cix * D).  All
@@ -57,6 +57,15 @@ __host__ __device__ __forceinline__ int64_t asm_index2(int row, int col, int ks)
   const int kstep = col >> 2;
   return (((int64_t)(row >> 4) * (ks >> 1) + (kstep >> 1)) * 64 + (col & 3) * 16 + (row & 15)) * 2 +
          (kstep & 1);
+}
+
+// The same for the FP32 pool with four consecutive K-steps per lane: a lane
+// reads the A operands of a group of four MFMAs with one 16-byte load.
+__host__ __device__ __forceinline__ int64_t asm_index4(int row, int col, int ks)
+{
+  const int kstep = col >> 2;
+  return (((int64_t)(row >> 4) * (ks >> 2) + (kstep >> 2)) * 64 + (col & 3) * 16 + (row & 15)) * 4 +
+         (kstep & 3);
 }
 
 // K-steps (4 columns each) stored per 16-row tile of an nb_t x nb_t matrix:

@@ -300,7 +300,7 @@ bool layout_term(const evr_plan *p, evr::TermDev &td, int c, std::vector<evr::St
   }
   const int64_t nbt = td.nbt;
   const int nvw = kClasses[c].nvw;
-  const size_t keo_bufs = kClasses[c].ept <= 2 ? (evr::keo_pencil(kClasses[c].nt, kClasses[c].ept, nvw) ? 6 : 3) : 1;
+  const size_t keo_bufs = kClasses[c].ept <= 4 ? (evr::keo_pencil(kClasses[c].nt, kClasses[c].ept, nvw) ? 6 : 3) : 1;
   const size_t base = td.lds_mats + (size_t)nbt * nvw;
   const size_t keo_words = p->keo_layout ? keo_bufs * nbt * nvw : 0;
   // half the LDS when the term fits in it, so two workgroups share a CU

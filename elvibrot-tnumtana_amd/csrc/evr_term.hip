@@ -378,15 +378,18 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
   }
 
   // ---------------------------------------------------------------- KEO --
-  // Two forms of the kinetic part for up to two units per thread, chosen per
-  // term class (keo_pencil): one-mode factors applied by pencil passes
-  // through LDS buffers (a thread per pencil, the factor in scalar
-  // registers; fewer LDS reads, more barriers), or per unit (each result
-  // reads its pencil and factor row; no extra buffers).  Accumulators M_i,
-  // N_i, O_i of a group of IG active modes live in registers per unit; the
+  // Two forms of the kinetic part over the active modes for up to four
+  // units per thread, chosen per term class (keo_pencil): one-mode factors
+  // applied by pencil passes through LDS buffers (a thread per pencil, the
+  // factor in scalar registers; fewer LDS reads, more barriers), or per unit
+  // (each result reads its pencil and factor row; three LDS buffers).
+  // Accumulators M_i, N_i, O_i of a group of IG active modes live in
+  // registers per unit (IG 2 at four units: 127 VGPRs, no spill; 1024x4x1
+  // 19.6 -> 11.9 ms against the all-mode form below at D=9 L=8); the
   // coefficient strips are pre-zeroed on the host (no branches, no index
-  // tests) and read at wave-uniform addresses.
-  if constexpr (EPT <= 2) {
+  // tests) and read at wave-uniform addresses.  Eight units per thread keep
+  // the all-mode form (three buffers of 8,192 units exceed the LDS).
+  if constexpr (EPT <= 4) {
   if constexpr (keo_pencil(NT, EPT, NVW)) {
     if (hmode && p.hasKeo) {
       T *Zb = work;              // P_j x
@@ -585,13 +588,8 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       const double *kU = kCd + kDp;                 // [i]
       const double *kF = kU + kDp;                  // [i]
       const double *kA = kF + kDp;                  // [(j*Dp + k)*Dp + i], with Ga only
-      int base[EPT][kMaxDev];
-#pragma unroll
-      for (int r = 0; r < EPT; ++r) {
-        const int u = unit_of<NT, EPT, NVW>(tid, r);
-        for (int d = 0; d < kMaxDev; ++d)
-          base[r][d] = d < D ? u - (int)((dig[r] >> (5 * d)) & 31) * s_st[d] : 0;
-      }
+      // first unit of the mode-d pencil through unit u (slot r)
+      auto ubase = [&](int r, int u, int d) { return u - (int)((dig[r] >> (5 * d)) & 31) * s_st[d]; };
       const int fdiag = p.diagT2 ? 6 : 2;
       // Modes outside amode[] have one function and vanishing P, Q, R (odd
       // integrands on the symmetric quadrature): every product holding one of
@@ -653,7 +651,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
             const int u = unit_of<NT, EPT, NVW>(tid, r);
             if (u < nunit) {
               const int bj = (int)((dig[r] >> (5 * j)) & 31);
-              zb[u] = apply_k(Kj + bj, sx + base[r][j], nbj, stj);  // Z_j = P_j x
+              zb[u] = apply_k(Kj + bj, sx + ubase(r, u, j), nbj, stj);  // Z_j = P_j x
             }
           }
           __syncthreads();
@@ -676,8 +674,8 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
                 for (int ii = 0; ii < IG; ++ii) accM[ii][r] = fma(cz[ii], zj, accM[ii][r]);
                 if (p.hasA) {
                   const int bj = (int)((dig[r] >> (5 * j)) & 31);
-                  const T sxv = apply_k(Kj + 4 * nnj + bj, sx + base[r][j], nbj, stj);  // S_j x
-                  const T qxv = apply_k(Kj + nnj + bj, sx + base[r][j], nbj, stj);      // Q_j x
+                  const T sxv = apply_k(Kj + 4 * nnj + bj, sx + ubase(r, u, j), nbj, stj);  // S_j x
+                  const T qxv = apply_k(Kj + nnj + bj, sx + ubase(r, u, j), nbj, stj);      // Q_j x
 #pragma unroll
                   for (int ii = 0; ii < IG; ++ii) {
                     accM[ii][r] = fma(cs[ii], sxv, accM[ii][r]);
@@ -703,7 +701,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
                 const int u = unit_of<NT, EPT, NVW>(tid, r);
                 if (u < nunit) {
                   const int bk = (int)((dig[r] >> (5 * k)) & 31);
-                  const T qz = apply_k(Qk + bk, zb + base[r][k], nbk, stk);  // Q_k Z_j
+                  const T qz = apply_k(Qk + bk, zb + ubase(r, u, k), nbk, stk);  // Q_k Z_j
 #pragma unroll
                   for (int ii = 0; ii < IG; ++ii) accM[ii][r] = fma(ca[ii], qz, accM[ii][r]);
                 }
@@ -735,7 +733,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
             const int u = unit_of<NT, EPT, NVW>(tid, r);
             if (u < nunit) {
               const int bi = (int)((dig[r] >> (5 * i)) & 31);
-              const int bo = base[r][i];
+              const int bo = ubase(r, u, i);
               T yv = apply_k(Ki + bi, zb + bo, nbi, sti);                        // P_i M_i
               yv += apply_k(Ki + fdiag * nn + bi, zb3 + bo, nbi, sti);            // T_i O_i
               if (p.hasA) {
@@ -754,9 +752,12 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
       if constexpr (EPT == 1) {
         if (na <= 4) keo_groups(std::integral_constant<int, 4>{});
         else keo_groups(std::integral_constant<int, 8>{});
-      } else {
+      } else if constexpr (EPT == 2) {
         if (na <= 3) keo_groups(std::integral_constant<int, 3>{});
         else keo_groups(std::integral_constant<int, 5>{});
+      } else {
+        if (na <= 2) keo_groups(std::integral_constant<int, 2>{});
+        else keo_groups(std::integral_constant<int, 2>{});
       }
     }
   }
@@ -1147,7 +1148,7 @@ __global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
         if (v0 + c < nbt) {
           const double val = (double)(w * yown[r]);
           if (p.A64) p.A64[td.off_asm + asm_index2(e, v0 + c, ks)] = val;
-          if (p.A32) p.A32[td.off_asm + asm_index2(e, v0 + c, ks)] = (float)val;
+          if (p.A32) p.A32[td.off_asm + asm_index4(e, v0 + c, ks)] = (float)val;
         }
       }
     }
@@ -1217,29 +1218,35 @@ __device__ __forceinline__ void asm_tiles(const T *__restrict__ At, const int *_
     __syncthreads();
   };
   // the 4 x NM matrix fragments of K-steps s..s+3 (ks is a multiple of 4):
-  // two loads of K-step pairs (asm_index2) per tile.  The pool is read once
-  // per call: for FP64, non-temporal loads keep it from evicting the term's
-  // X rows from L2, which the other work items of the term read again (9%
-  // faster apply at D=9 L=8 in a same-box A/B; for FP32 they measured 5%
-  // slower and plain loads are kept).
+  // FP64, two 16-byte loads of K-step pairs (asm_index2) per tile; FP32, one
+  // 16-byte load of the four K-steps (asm_index4; 13.2 against 14.5 ms per
+  // FP32 apply at D=9 L=8, 32 vectors, in a same-box A/B with two 8-byte
+  // loads).  The pool is read once per call: non-temporal loads keep it from
+  // evicting the term's X rows from L2, which the other work items of the
+  // term read again (9% faster FP64 apply in a same-box A/B).
   auto load = [&](T(&af)[4][MA], int s) {
-    using T2 = T __attribute__((ext_vector_type(2)));
-    const T2 *A2 = reinterpret_cast<const T2 *>(At);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      const int64_t b = ((int64_t)(mt0 + m) * (ks >> 1) + (s >> 1)) * 64 + lane;
-      T2 x0, x1;
       if constexpr (sizeof(T) == 8) {
-        x0 = __builtin_nontemporal_load(A2 + b);
-        x1 = __builtin_nontemporal_load(A2 + b + 64);
+        using T2 = T __attribute__((ext_vector_type(2)));
+        const T2 *A2 = reinterpret_cast<const T2 *>(At);
+        const int64_t b = ((int64_t)(mt0 + m) * (ks >> 1) + (s >> 1)) * 64 + lane;
+        const T2 x0 = __builtin_nontemporal_load(A2 + b);
+        const T2 x1 = __builtin_nontemporal_load(A2 + b + 64);
+        af[0][m] = x0.x;
+        af[1][m] = x0.y;
+        af[2][m] = x1.x;
+        af[3][m] = x1.y;
       } else {
-        x0 = A2[b];
-        x1 = A2[b + 64];
+        using T4 = T __attribute__((ext_vector_type(4)));
+        const T4 *A4 = reinterpret_cast<const T4 *>(At);
+        const int64_t b = ((int64_t)(mt0 + m) * (ks >> 2) + (s >> 2)) * 64 + lane;
+        const T4 x = __builtin_nontemporal_load(A4 + b);
+        af[0][m] = x.x;
+        af[1][m] = x.y;
+        af[2][m] = x.z;
+        af[3][m] = x.w;
       }
-      af[0][m] = x0.x;
-      af[1][m] = x0.y;
-      af[2][m] = x1.x;
-      af[3][m] = x1.y;
     }
   };
   auto compute = [&](const T(&af)[4][MA], int s) {
