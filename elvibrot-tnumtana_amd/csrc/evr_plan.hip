@@ -72,9 +72,14 @@ struct KClass {
   int max_nbt() const { return nt * ept / nvw; }
 };
 // (nt, ept, nvw): ept == nvw > 1 is element-major (each thread the nvw vectors
-// of one basis function)
+// of one basis function).  The term kernels are bound by latency and barrier
+// waits, so the classes below 1024 threads are built for four waves per SIMD
+// (two or more workgroups per CU hide each other's barriers): nb_t 513-1024
+// and 1025-2048 run as 512 x 4 rather than 1024 x 2 (38.4 against 42.7 ms
+// and 40.7 against 45.5 ms at D=9 L=8, 32 vectors), 257-512 stays 512 x 2
+// (16.9 ms; 256 x 4 measured 18.4 ms).
 constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {1024, 1, 4}, {512, 2, 2},
-                               {1024, 2, 2}, {1024, 2, 1}, {1024, 4, 1}, {1024, 8, 1}};
+                               {512, 4, 2}, {512, 4, 1}, {1024, 4, 1}, {1024, 8, 1}};
 constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
 // dynamic LDS per workgroup: 160 KiB less a reserve for the kernels' static
 // arrays (about 2 KiB for the term kernels)
@@ -205,8 +210,8 @@ hipError_t launch_class(int c, const evr::TermParams<T> &prm, int64_t units, int
     case 1: return launch_one<T, 512, 1, 4>(prm, units, nvec, lds, s);
     case 2: return launch_one<T, 1024, 1, 4>(prm, units, nvec, lds, s);
     case 3: return launch_one<T, 512, 2, 2>(prm, units, nvec, lds, s);
-    case 4: return launch_one<T, 1024, 2, 2>(prm, units, nvec, lds, s);
-    case 5: return launch_one<T, 1024, 2, 1>(prm, units, nvec, lds, s);
+    case 4: return launch_one<T, 512, 4, 2>(prm, units, nvec, lds, s);
+    case 5: return launch_one<T, 512, 4, 1>(prm, units, nvec, lds, s);
     case 6: return launch_one<T, 1024, 4, 1>(prm, units, nvec, lds, s);
     case 7: return launch_one<T, 1024, 8, 1>(prm, units, nvec, lds, s);
     default: return hipErrorInvalidValue;
@@ -613,8 +618,8 @@ hipError_t launch_items(int c, const evr::TermParams<double> &prm, int64_t nitem
     case 1: return launch_items_one<double, 512, 1, 4>(prm, nitems, lds, s);
     case 2: return launch_items_one<double, 1024, 1, 4>(prm, nitems, lds, s);
     case 3: return launch_items_one<double, 512, 2, 2>(prm, nitems, lds, s);
-    case 4: return launch_items_one<double, 1024, 2, 2>(prm, nitems, lds, s);
-    case 5: return launch_items_one<double, 1024, 2, 1>(prm, nitems, lds, s);
+    case 4: return launch_items_one<double, 512, 4, 2>(prm, nitems, lds, s);
+    case 5: return launch_items_one<double, 512, 4, 1>(prm, nitems, lds, s);
     case 6: return launch_items_one<double, 1024, 4, 1>(prm, nitems, lds, s);
     case 7: return launch_items_one<double, 1024, 8, 1>(prm, nitems, lds, s);
     default: return hipErrorInvalidValue;

@@ -18,11 +18,12 @@
 namespace evr {
 
 // Term classes whose kinetic part runs as pencil passes: those with two
-// units per thread (nb_t from 257 to 2048 at one or two vectors).  Measured
-// on the D=9 L=8 operator (32 vectors, FP64): 512x2x2 21.9 -> 16.2 ms,
-// 1024x2x2 45.7 -> 42.7 ms, 1024x2x1 52.5 -> 45.6 ms; the one-unit classes
+// units per thread.  Measured on the D=9 L=8 operator (32 vectors, FP64):
+// 512x2x2 21.9 -> 16.2 ms (and 1024x2x2 45.7 -> 42.7 ms, 1024x2x1 52.5 ->
+// 45.6 ms, classes since replaced by four-unit ones); the one-unit classes
 // (small terms, few pencils per barrier) are faster per unit (0.98 against
-// 1.68 ms at 256x1x4).
+// 1.68 ms at 256x1x4), and four units per thread need three LDS buffers
+// where the passes need six.
 constexpr bool keo_pencil(int nt, int ept, int nvw)
 {
   (void)nt;
@@ -259,8 +260,11 @@ struct IGroup {
 
 // One workgroup applies one Smolyak term to a tile of NVW vectors.  LDS
 // arrays are [element][vector] (vector fastest); a "unit" u = e*NVW + v.
+// Below 1024 threads the register budget is set for four waves per SIMD
+// (128 VGPRs; a few bytes of spill at one unit per thread): 512x1x4 2.79 ->
+// 1.55 ms, 256x1x4 0.72 -> 0.60 ms at D=9 L=8.
 template <typename T, int NT, int EPT, int NVW>
-__global__ __launch_bounds__(NT) void k_term(TermParams<T> p)
+__global__ __launch_bounds__(NT, NT < 1024 ? 4 : 1) void k_term(TermParams<T> p)
 {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
