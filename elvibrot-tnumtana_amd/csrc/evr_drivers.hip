@@ -431,6 +431,10 @@ static int davidson_impl(OpRef op, const evr_davidson_opts *o, double *evals, do
   BTRY(rocblas_create_handle(&blas.h));
   BTRY(rocblas_set_stream(blas.h, d.st));
   BTRY(rocblas_set_pointer_mode(blas.h, rocblas_pointer_mode_host));
+  // no atomics in the library's gemms (and rocSOLVER's, on this handle): the
+  // same inputs give the same iterates run after run, which the mixed
+  // driver's stall test (a 10% drop within 6 iterations) is sensitive to
+  BTRY(rocblas_set_atomics_mode(blas.h, rocblas_atomics_not_allowed));
   d.bh = blas.h;
 
   const int64_t nb = d.nb;

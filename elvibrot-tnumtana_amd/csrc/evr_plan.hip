@@ -76,9 +76,10 @@ struct KClass {
 // waits, so the classes below 1024 threads are built for four waves per SIMD
 // (two or more workgroups per CU hide each other's barriers): nb_t 513-1024
 // and 1025-2048 run as 512 x 4 rather than 1024 x 2 (38.4 against 42.7 ms
-// and 40.7 against 45.5 ms at D=9 L=8, 32 vectors), 257-512 stays 512 x 2
-// (16.9 ms; 256 x 4 measured 18.4 ms).
-constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {1024, 1, 4}, {512, 2, 2},
+// and 40.7 against 45.5 ms at D=9 L=8, 32 vectors), 129-256 as 512 x 2 at 4
+// vectors rather than 1024 x 1 (6.9 against 8.0 ms; 256 x 4 8.0 ms), 257-512
+// stays 512 x 2 (16.9 ms; 256 x 4 measured 18.4 ms).
+constexpr KClass kClasses[] = {{256, 1, 4},  {512, 1, 4},  {512, 2, 4}, {512, 2, 2},
                                {512, 4, 2}, {512, 4, 1}, {1024, 4, 1}, {1024, 8, 1}};
 constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
 // dynamic LDS per workgroup: 160 KiB less a reserve for the kernels' static
@@ -208,7 +209,7 @@ hipError_t launch_class(int c, const evr::TermParams<T> &prm, int64_t units, int
   switch (c) {
     case 0: return launch_one<T, 256, 1, 4>(prm, units, nvec, lds, s);
     case 1: return launch_one<T, 512, 1, 4>(prm, units, nvec, lds, s);
-    case 2: return launch_one<T, 1024, 1, 4>(prm, units, nvec, lds, s);
+    case 2: return launch_one<T, 512, 2, 4>(prm, units, nvec, lds, s);
     case 3: return launch_one<T, 512, 2, 2>(prm, units, nvec, lds, s);
     case 4: return launch_one<T, 512, 4, 2>(prm, units, nvec, lds, s);
     case 5: return launch_one<T, 512, 4, 1>(prm, units, nvec, lds, s);
@@ -616,7 +617,7 @@ hipError_t launch_items(int c, const evr::TermParams<double> &prm, int64_t nitem
   switch (c) {
     case 0: return launch_items_one<double, 256, 1, 4>(prm, nitems, lds, s);
     case 1: return launch_items_one<double, 512, 1, 4>(prm, nitems, lds, s);
-    case 2: return launch_items_one<double, 1024, 1, 4>(prm, nitems, lds, s);
+    case 2: return launch_items_one<double, 512, 2, 4>(prm, nitems, lds, s);
     case 3: return launch_items_one<double, 512, 2, 2>(prm, nitems, lds, s);
     case 4: return launch_items_one<double, 512, 4, 2>(prm, nitems, lds, s);
     case 5: return launch_items_one<double, 512, 4, 1>(prm, nitems, lds, s);
