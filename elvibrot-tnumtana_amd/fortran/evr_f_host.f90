@@ -27,6 +27,9 @@ program evr_f_host
   real(c_double), target :: F2(D, D)
   real(c_double) :: Ene(nev)
   complex(c_double), allocatable :: zpsi(:, :), zH(:, :)
+  type(evr_control_opts) :: copts
+  integer, parameter :: nfc = 2, nsc = 10
+  real(c_double), target :: gE(nfc, nsc), Eold(nfc, nsc), Ezr(nfc, nsc), Ekr(nfc, nsc)
   type(evr_davidson_info) :: dinfo
   type(evr_propa_info) :: pinfo
   integer :: nb, npts, i, k, u
@@ -102,6 +105,27 @@ program evr_f_host
   do k = 1, nvec
     do i = 1, npts
       write(u, '(2es25.17)') psiG(i, k), dG(i, k)
+    end do
+  end do
+  close(u)
+  ! the control field update (build_field) on a given dJ/dE, Zhu-Rabitz and
+  ! Krotov, written to <out>.field
+  copts%T = 100.0d0
+  copts%nsteps = nsc
+  do k = 1, nsc
+    do i = 1, nfc
+      gE(i, k) = sin(0.3d0 * k + i)
+      Eold(i, k) = 0.01d0 * cos(0.7d0 * k - i)
+    end do
+  end do
+  call evr_check(evr_control_build_field(copts, nfc, gE, 2.0d0, 100.0d0, 0, c_null_ptr, Ezr), &
+                 'evr_control_build_field')
+  call evr_check(evr_control_build_field(copts, nfc, gE, 2.0d0, 100.0d0, 1, c_loc(Eold), Ekr), &
+                 'evr_control_build_field')
+  open(newunit=u, file=trim(out)//'.field', status='replace', action='write')
+  do k = 1, nsc
+    do i = 1, nfc
+      write(u, '(4es25.17)') gE(i, k), Eold(i, k), Ezr(i, k), Ekr(i, k)
     end do
   end do
   close(u)

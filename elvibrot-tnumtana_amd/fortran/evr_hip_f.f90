@@ -459,6 +459,19 @@ module evr_hip_f
       real(c_double), intent(out) :: field(*)
       integer(c_int) :: evr_control_field
     end function
+    ! the reference's field update (build_field, CTRL:902-971): field_old is
+    ! c_loc of the old field (krotov = 1) or c_null_ptr
+    function evr_control_build_field(opts, nf, grad_E, alpha, Tenv, krotov, field_old, field_new) &
+        bind(C, name="evr_control_build_field")
+      import :: c_int, c_double, c_ptr, evr_control_opts
+      type(evr_control_opts), intent(in) :: opts
+      integer(c_int), value :: nf, krotov
+      real(c_double), intent(in) :: grad_E(*)
+      real(c_double), value :: alpha, Tenv
+      type(c_ptr), value :: field_old
+      real(c_double), intent(out) :: field_new(*)
+      integer(c_int) :: evr_control_build_field
+    end function
     function evr_control_gradient(H, mu, opts, nf, amp, psi0, phi1, J, grad_a, grad_E, grad_V, info) &
         bind(C, name="evr_control_gradient")
       import :: c_int, c_ptr, c_double, evr_control_opts, evr_control_info

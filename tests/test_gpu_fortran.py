@@ -4,7 +4,9 @@ evr_f_host builds a D=4 SG4 Henon-Heiles type-1 operator through evr_hip_f,
 applies it to real and complex vectors, runs Davidson and 5 Chebyshev steps,
 builds the assembled per-term operators and applies them again, transforms
 the vectors to the grid and takes a grid derivative (the multi-vector and
-DerivOp wrappers), and writes the results; the oracle recomputes all of them.  evr_f_pyrazine
+DerivOp wrappers), and writes the results; the oracle recomputes all of them.
+It also runs the control field update (build_field, Zhu-Rabitz and Krotov)
+on a given dJ/dE through the Fortran interface.  evr_f_pyrazine
 propagates the two-state 12D pyrazine model and writes the autocorrelation
 file the reference's benchmark holds.
 """
@@ -76,6 +78,14 @@ def test_fortran_host(oracle_mod, tmp_path):
         assert np.abs(gd[c, :, 0] - g).max() < 1e-12 * np.abs(g).max()
         dref = s.deriv(g, 0, 1)
         assert np.abs(gd[c, :, 1] - dref).max() < 1e-12 * np.abs(dref).max()
+    # build_field through the Fortran interface: T = 100, 10 steps, alpha = 2,
+    # envelope sin^2(pi t / 100) at the mid-steps; Krotov adds the old field
+    f = np.loadtxt(str(out) + ".field").reshape(10, 2, 4)
+    gE, Eold, Ezr, Ekr = (f[:, :, j] for j in range(4))
+    dt = 10.0
+    env = np.sin(np.pi * (np.arange(10) + 0.5) * dt / 100.0) ** 2
+    assert np.abs(Ezr - env[:, None] * gE / (2 * 2.0 * dt)).max() < 1e-15
+    assert np.abs(Ekr - (Eold + Ezr)).max() < 1e-15
 
 
 PYRAZINE = os.path.join(ROOT, "elvibrot-tnumtana_amd", "fortran", "evr_f_pyrazine")
