@@ -479,6 +479,47 @@ def transforms_run(args):
     print(json.dumps(line), flush=True)
 
 
+def auto_path_run(args):
+    """EVR_AUTO_PATH against the two forced paths for blocks of 1 .. --nvec
+    vectors on the --D/--L workload (FP64 and FP32): which path the cost model
+    takes and what each costs, 5 calls per case timed with HIP events on the
+    plan stream."""
+    import numpy as np
+    import torch
+    import evrhip
+    from evrhip import hamiltonian as H
+    D, L, nmax = args.D, args.L, args.nvec
+    cfg = config_index(D, L)
+    ham = H.smolyak_config(cfg, D)
+    b = evrhip.Basis.sg4(D, L)
+    plan = evrhip.Plan(b, type=10, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=nmax)
+    plan.gen_potential_poly(ham)
+    plan.assemble(evrhip.ASM_FP64 | evrhip.ASM_FP32)
+    X = torch.from_numpy(np.ascontiguousarray(H.test_vectors(b.nb_packed, nmax, cfg).T)).cuda()
+    Hx = torch.empty_like(X)
+    s = torch.cuda.ExternalStream(plan.stream())
+    rows = []
+    for prec, pf in (("fp64", evrhip.FP64), ("fp32", evrhip.FP32)):
+        for n in range(1, nmax + 1):
+            ms = {}
+            for name, f in (("matrix_free", evrhip.MATRIX_FREE), ("assembled", 0), ("auto", evrhip.AUTO_PATH)):
+                for _ in range(2):
+                    plan.TabOpPsi_device(X.data_ptr(), Hx.data_ptr(), n, f | pf)
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                for _ in range(5):
+                    plan.TabOpPsi_device(X.data_ptr(), Hx.data_ptr(), n, f | pf)
+                e1.record(s)
+                torch.cuda.synchronize()
+                ms[name] = round(e0.elapsed_time(e1) / 5, 3)
+            log("%s nvec %d: matrix-free %.3f assembled %.3f auto %.3f ms"
+                % (prec, n, ms["matrix_free"], ms["assembled"], ms["auto"]))
+            rows.append(dict(precision=prec, nvec=n, **ms))
+    print(json.dumps({"workload": "SG4 D=%d L=%d type-10 H|psi>, EVR_AUTO_PATH" % (D, L), "ms": rows}), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this workload (profiles/traffic_sg4_d9l8.md), or None."""
@@ -537,6 +578,8 @@ def main():
                     help="type 10 with G, J, s per grid point on D=6 SG4 L=5")
     ap.add_argument("--transforms", action="store_true",
                     help="basis <-> grid of one vector on the --D/--L grids, kernel times")
+    ap.add_argument("--auto-path", action="store_true",
+                    help="EVR_AUTO_PATH against both forced paths for 1 .. --nvec vectors")
     ap.add_argument("--nev", type=int, default=200)
     ap.add_argument("--conv-resi", type=float, default=1e-3)
     ap.add_argument("--refine-resi", type=float, default=1e-2)
@@ -569,6 +612,9 @@ def main():
         return
     if args.transforms:
         transforms_run(args)
+        return
+    if args.auto_path:
+        auto_path_run(args)
         return
 
     import numpy as np

@@ -98,6 +98,7 @@ struct evr_desc_owner {
 struct evr_plan {
   int D = 0, type = 0, max_nvec = 0;  // D: the term kernel's mode positions
   int device = 0;                     // HIP device current at creation
+  int ncu = 256;                      // compute units of that device
   int D_full = 0;                     // modes of the basis (> D when compacted)
   int64_t keoc_stride = 0;            // doubles per KEO coefficient set
   bool dp_order = false;
@@ -139,7 +140,8 @@ struct evr_plan {
   DevBuf<float> d_Veff32;
   DevBuf<double> d_coefC, d_coefA, d_coefF1, d_keoc;
   DevBuf<int> d_order[kNumClasses];
-  int64_t nunits[kNumClasses] = {};
+  int64_t nunits[kNumClasses] = {};   // work list lengths (terms padded to a multiple of 8)
+  int64_t nterms_class[kNumClasses] = {};
   std::vector<int> tclass;  // kernel class of every term (static rule or autotuned)
   bool keo_layout = false;  // the term layouts hold the KEO factors in LDS
   size_t lds_bytes64[kNumClasses] = {}, lds_bytes32[kNumClasses] = {};
@@ -369,7 +371,7 @@ int apply_classes(evr_plan *p)
   for (int c = 0; c < kNumClasses; ++c) {
     p->lds_bytes64[c] = p->lds_bytes32[c] = 0;
     p->flops_class[c] = 0.0;
-    p->nunits[c] = 0;
+    p->nunits[c] = p->nterms_class[c] = 0;
     p->d_order[c].release();
   }
   std::vector<double> cost(p->nterms);
@@ -393,6 +395,7 @@ int apply_classes(evr_plan *p)
       if (p->tclass[t] == c) ids.push_back((int)t);
     if (ids.empty()) continue;
     std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    p->nterms_class[c] = (int64_t)ids.size();
     while (ids.size() % 8) ids.push_back(-1);
     p->nunits[c] = (int64_t)ids.size();
     HIP_TRY(p->d_order[c].upload(ids.data(), ids.size(), p->stream));
@@ -885,6 +888,7 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->nterms = bd->nterms;
   p->nb_packed = bd->nb_packed;
   HIP_TRY(hipGetDevice(&p->device));
+  HIP_TRY(hipDeviceGetAttribute(&p->ncu, hipDeviceAttributeMultiprocessorCount, p->device));
   HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
 
   // ---- slots: 1D matrices and basis-space KEO factors
@@ -1650,21 +1654,37 @@ static int ensure_v32(evr_plan *p)
 }
 
 // The assembled operator streams its whole pool once per call whatever the
-// number of vectors; the sum-factorised term kernels cost flops per vector.
-// With EVR_AUTO_PATH the cheaper of the two by this model is taken (rates
-// measured on MI355X: pool streaming about 5.5 TB/s, term kernels about 2
-// TFLOP/s FP64 on small blocks, about 8 us per launch).
+// number of vectors (about 6 TB/s on MI355X); the sum-factorised term kernels
+// cost per vector.  They are bound by latency and barrier waits, so a class
+// takes its number of waves of workgroups (units x ceil(nvec / nvw) over the
+// 1024 / nt workgroups each CU holds, fewer when LDS is short) times one
+// workgroup's time, and a workgroup always does nvw vectors' work, padded or
+// not.  The per-workgroup rates below are the classes' whole-device TFLOP/s
+// measured at D=9 SG4 L=8 with 32 vectors, wave-count corrected; the model
+// then gives 8.7 / 10.4 / 17.0 / 31.2 / 119.7 ms for 1 / 2 / 4 / 8 / 32
+// vectors there (measured 9.1 / 10.8 / 17.2 / 31.6 / 120.5) and overestimates
+// D=12 SG4 L=6 by about 30%.  With EVR_AUTO_PATH the cheaper path by this
+// model is taken: at D=9 L=8 FP64 the sum-factorised one up to 5 vectors (17.3
+// against 22.5 ms at 4, 23.3 against 22.8 at 5), at D=12 L=6 the assembled
+// one from 1 vector (2.49 against 2.79 ms).
 static bool matrix_free_cheaper(const evr_plan *p, int nvec, bool fp32)
 {
-  double flops = 0.0;
-  int launches = 2;  // gather-free pack + reduction
-  for (int c = 0; c < kNumClasses; ++c)
-    if (p->nunits[c]) {
-      flops += p->flops_class[c];
-      ++launches;
-    }
-  const double t_mf = nvec * flops / (fp32 ? 2.5e12 : 2.0e12) + 8e-6 * launches;
-  const double t_asm = (fp32 ? 0.5 : 1.0) * p->asm_bytes64 / 5.5e12 + 8e-6 * 3;
+  constexpr double kClassTflops[] = {1.23, 1.82, 2.56, 2.88, 3.83, 5.17, 4.70, 1.99};
+  static_assert(sizeof(kClassTflops) / sizeof(double) == kNumClasses, "one rate per term class");
+  double t_mf = 8e-6 * 2;  // gather-free pack + reduction
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (!p->nterms_class[c]) continue;
+    const KClass &k = kClasses[c];
+    const size_t lds = fp32 ? p->lds_bytes32[c] : p->lds_bytes64[c];
+    int per_cu = 1024 / k.nt;
+    if (lds) per_cu = std::max(1, std::min<int>(per_cu, (int)(160 * 1024 / lds)));
+    const double cap = (double)p->ncu * per_cu;
+    const double nblk = (double)p->nunits[c] * ((nvec + k.nvw - 1) / k.nvw);
+    const double wg_flops = p->flops_class[c] / (double)p->nterms_class[c] * k.nvw;
+    const double rate = kClassTflops[c] * 1e12 * (fp32 ? 1.06 : 1.0);
+    t_mf += std::ceil(nblk / cap) * cap * wg_flops / rate + 8e-6;
+  }
+  const double t_asm = (fp32 ? 0.5 : 1.0) * p->asm_bytes64 / 6.0e12 + 8e-6 * 3;
   return t_mf < t_asm;
 }
 
