@@ -520,6 +520,47 @@ def auto_path_run(args):
     print(json.dumps({"workload": "SG4 D=%d L=%d type-10 H|psi>, EVR_AUTO_PATH" % (D, L), "ms": rows}), flush=True)
 
 
+def streamed_run(args):
+    """H|psi> on direct-product bases from 3 x 12 functions (one term held in
+    LDS by the term kernels) to 4 x 20 (160,000 functions, streamed over
+    global memory); type 10 with q-linear G, FP64 and FP32, 5 calls timed
+    with HIP events on the plan stream."""
+    import numpy as np
+    import torch
+    import evrhip
+    from evrhip import hamiltonian as H
+    rows = []
+    for D, n, nv in ((3, 12, 32), (3, 16, 32), (3, 21, 32), (3, 30, 32), (3, 40, 32), (4, 20, 32)):
+        b = evrhip.Basis.dp([n] * D, [n + 1] * D)
+        ham = H.smolyak_config(9, D)
+        plan = evrhip.Plan(b, type=10, V=ham.potential(b.grid()), G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=nv)
+        X = torch.from_numpy(np.ascontiguousarray(H.test_vectors(b.nb_packed, nv, 3).T)).cuda()
+        Hx = torch.empty_like(X)
+        s = torch.cuda.ExternalStream(plan.stream())
+        ms = {}
+        for tag, f in (("fp64", evrhip.FP64), ("fp32", evrhip.FP32)):
+            for _ in range(2):
+                plan.TabOpPsi_device(X.data_ptr(), Hx.data_ptr(), nv, f)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(5):
+                plan.TabOpPsi_device(X.data_ptr(), Hx.data_ptr(), nv, f)
+            e1.record(s)
+            torch.cuda.synchronize()
+            ms[tag] = round(e0.elapsed_time(e1) / 5, 4)
+        row = {"basis": "DP %d x %d" % (D, n), "functions": b.nb_packed, "points": b.sum_nq, "nvec": nv,
+               "ms_fp64": ms["fp64"], "ms_fp32": ms["fp32"],
+               "hpsi_per_s_per_vector_fp64": round(1e3 * nv / ms["fp64"], 1)}
+        log("%s (%d functions, %d points), %d vectors: FP64 %.3f ms, FP32 %.3f ms"
+            % (row["basis"], b.nb_packed, b.sum_nq, nv, ms["fp64"], ms["fp32"]))
+        rows.append(row)
+        plan.close()
+    print(json.dumps({"workload": "type-10 H|psi> on direct-product bases, LDS-resident and streamed terms",
+                      "rows": rows}), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary of this workload (profiles/traffic_sg4_d9l8.md), or None."""
@@ -578,6 +619,8 @@ def main():
                     help="type 10 with G, J, s per grid point on D=6 SG4 L=5")
     ap.add_argument("--transforms", action="store_true",
                     help="basis <-> grid of one vector on the --D/--L grids, kernel times")
+    ap.add_argument("--streamed", action="store_true",
+                    help="H|psi> on direct-product bases beyond 8192 functions (streamed terms)")
     ap.add_argument("--auto-path", action="store_true",
                     help="EVR_AUTO_PATH against both forced paths for 1 .. --nvec vectors")
     ap.add_argument("--nev", type=int, default=200)
@@ -615,6 +658,9 @@ def main():
         return
     if args.auto_path:
         auto_path_run(args)
+        return
+    if args.streamed:
+        streamed_run(args)
         return
 
     import numpy as np

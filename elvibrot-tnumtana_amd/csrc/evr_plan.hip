@@ -17,6 +17,7 @@
 #include "evr_kernels.hpp"
 #include "evr_setup.hpp"
 #include "evr_term.hip"
+#include "evr_stream.hip"
 
 namespace {
 
@@ -128,6 +129,14 @@ struct evr_plan {
   int64_t n_gitems = 0;
   DevBuf<double> d_GG, d_Jg, d_Sg, d_Gg, d_Pg, d_Fg;
   DevBuf<int2> d_gitems;
+  // streamed terms (a basis beyond one workgroup's LDS, evr_stream.hip): their
+  // ids, the largest term grid among them, the vectors per pass and the work
+  // block of (8 + D) tensors of big_nq x big_k words
+  std::vector<int64_t> big;
+  int64_t big_nq = 0;
+  int big_k = 0;
+  DevBuf<double> d_big;
+  std::vector<double> hC, hF1;  // host copies of the KEO coefficients (kernel convention)
   DevBuf<int> d_map, d_rowpos;
   // reduction segments: rows seg[s] .. seg[s+1]-1 of R; the segments of packed
   // function pk are fseg[pk] .. fseg[pk+1]-1
@@ -181,7 +190,7 @@ struct evr_plan {
     d_X.release(); d_R.release(); d_psi.release(); d_hpsi.release(); d_G.release();
     d_A64.release(); d_A32.release(); d_asm_items.release(); d_D1.release(); d_D2.release();
     d_GG.release(); d_Jg.release(); d_Sg.release(); d_Gg.release(); d_Pg.release(); d_Fg.release();
-    d_F2g.release(); d_F1g.release(); d_gitems.release(); d_dp_perm.release();
+    d_F2g.release(); d_F1g.release(); d_gitems.release(); d_dp_perm.release(); d_big.release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -378,6 +387,7 @@ int apply_classes(evr_plan *p)
   for (int64_t t = 0; t < p->nterms; ++t) {
     evr::TermDev &td = p->terms[t];
     const int c = p->tclass[t];
+    if (c < 0) continue;  // streamed term
     if (kClasses[c].max_nbt() < td.nbt) return fail(EVR_ERR_ARG, "term class too small for its basis");
     size_t words = 0;
     if (!layout_term(p, td, c, steps, words)) return fail(EVR_ERR_SIZE, "term does not fit in LDS");
@@ -402,6 +412,171 @@ int apply_classes(evr_plan *p)
   }
   HIP_TRY(p->d_terms.upload(p->terms.data(), p->terms.size(), p->stream));
   HIP_TRY(p->d_steps.upload(p->steps.data(), p->steps.size(), p->stream));
+  return EVR_OK;
+}
+
+// The streamed terms of a run_terms call (evr_stream.hip), one term and up to
+// big_k vectors at a time: H|psi> reads X and writes the term rows of R, B2G
+// reads X and writes the grid columns of G, G2B reads the grid columns of Gin
+// and writes R.  Whole-device launches on the plan stream, nothing allocated.
+template <typename T>
+int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *Gin)
+{
+  const int D = p->D;
+  const int64_t S = p->big_nq * p->big_k;
+  T *base = reinterpret_cast<T *>(p->d_big.ptr);
+  T *b_in = base, *b_g = base + S, *b_a = base + 2 * S, *b_b = base + 3 * S, *b_acc = base + 4 * S,
+    *b_t = base + 5 * S, *b_y0 = base + 6 * S, *b_y1 = base + 7 * S, *b_P = base + 8 * S;
+  auto blocks = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); };
+  for (int64_t t : p->big) {
+    const evr::TermDev &td = p->terms[t];
+    int nb[evr::kMaxDev], nq[evr::kMaxDev];
+    const evr::SlotDev *sl[evr::kMaxDev];
+    for (int d = 0; d < D; ++d) {
+      sl[d] = &p->slots[td.slot[d]];
+      nb[d] = sl[d]->nb;
+      nq[d] = sl[d]->nq;
+    }
+    const double *bw = p->d_bw64.ptr;
+    for (int c0 = 0; c0 < nvec; c0 += p->big_k) {
+      const int k = std::min(p->big_k, nvec - c0);
+      const int64_t nbk = (int64_t)td.nbt * k, nqk = (int64_t)td.nqt * k;
+      hipError_t e = hipSuccess;
+      auto pass = [&](const T *in, T *out, const double *M, int ld, int trans, int64_t L, int nin, int nout,
+                      int64_t Rr) {
+        // output chunks per pencil: enough threads for about 1024 per CU
+        const int64_t want = (int64_t)p->ncu * 1024;
+        const int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(nout, want / std::max<int64_t>(1, L * Rr)));
+        const int osz = (nout + nchunk - 1) / nchunk;
+        const int64_t thr = L * Rr * ((nout + osz - 1) / osz);
+        const dim3 g(blocks(thr)), b(256);
+        if (nin <= 16)
+          hipLaunchKernelGGL((evr::k_mode_pass<T, 16>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz);
+        else if (nin <= 32)
+          hipLaunchKernelGGL((evr::k_mode_pass<T, 32>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz);
+        else
+          hipLaunchKernelGGL((evr::k_mode_pass<T, 64>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz);
+        if (e == hipSuccess) e = hipGetLastError();
+      };
+      // basis -> grid over all modes (B, or W^T for the transposed operator)
+      auto b2g = [&](const T *src, T *dst, int trans) {
+        int64_t L = k, Rr = td.nbt;
+        const T *cur = src;
+        for (int d = 0; d < D; ++d) {
+          Rr /= nb[d];
+          T *out = d == D - 1 ? dst : ((d & 1) ? b_b : b_a);
+          if (trans) pass(cur, out, bw + sl[d]->offW, nb[d], 1, L, nb[d], nq[d], Rr);
+          else pass(cur, out, bw + sl[d]->offB, nq[d], 0, L, nb[d], nq[d], Rr);
+          L *= nq[d];
+          cur = out;
+        }
+      };
+      // grid -> basis over all modes (W, or B^T for the transposed operator)
+      auto g2b = [&](const T *src, T *dst, int trans) {
+        int64_t L = k, Rr = td.nqt;
+        const T *cur = src;
+        for (int d = 0; d < D; ++d) {
+          Rr /= nq[d];
+          T *out = d == D - 1 ? dst : ((d & 1) ? b_b : b_a);
+          if (trans) pass(cur, out, bw + sl[d]->offB, nq[d], 1, L, nq[d], nb[d], Rr);
+          else pass(cur, out, bw + sl[d]->offW, nb[d], 0, L, nq[d], nb[d], Rr);
+          L *= nb[d];
+          cur = out;
+        }
+      };
+      // grid derivative on mode d (D1 or D2 of its slot, nq x nq)
+      auto deriv = [&](const T *src, T *dst, int d, const double *Dm, int trans) {
+        int64_t L = k, Rr = 1;
+        for (int m = 0; m < d; ++m) L *= nq[m];
+        for (int m = d + 1; m < D; ++m) Rr *= nq[m];
+        pass(src, dst, Dm + sl[d]->offD1, nq[d], trans, L, nq[d], nq[d], Rr);
+      };
+      auto axpy = [&](double f, const T *x, T *acc) {
+        hipLaunchKernelGGL(evr::k_stream_axpy<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, nqk, f, x, acc);
+        if (e == hipSuccess) e = hipGetLastError();
+      };
+      // the grid operator of the term on b_g into b_acc (OPSG4:1448-1650)
+      auto grid_op = [&](int trans) {
+        const double *V = p->hasV ? p->d_V.ptr + td.off_nq : nullptr;
+        hipLaunchKernelGGL(evr::k_stream_vmul<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, V, b_g,
+                           (int64_t)td.nqt, k, b_acc);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (p->type == 1) {
+          for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) {
+              const double f = p->hC[i * D + j];
+              if (f == 0.0) continue;
+              if (i == j) {
+                deriv(b_g, b_t, i, p->d_D2.ptr, trans);
+              } else {
+                deriv(b_g, b_a, i, p->d_D1.ptr, trans);
+                deriv(b_a, b_t, j, p->d_D1.ptr, trans);
+              }
+              axpy(f, b_t, b_acc);
+            }
+          if (p->hasF1)
+            for (int i = 0; i < D; ++i) {
+              if (p->hF1[i] == 0.0) continue;
+              deriv(b_g, b_t, i, p->d_D1.ptr, trans);
+              axpy(p->hF1[i], b_t, b_acc);
+            }
+        } else if (p->type == 10) {
+          evr::StreamGrid sg{};
+          sg.D = D;
+          for (int d = 0; d < D; ++d) {
+            sg.nq[d] = nq[d];
+            sg.offX[d] = sl[d]->offX;
+          }
+          for (int j = 0; j < D; ++j) deriv(b_g, b_P + (size_t)j * nqk, j, p->d_D1.ptr, trans);
+          for (int i = 0; i < D; ++i) {
+            hipLaunchKernelGGL(evr::k_stream_gcomb<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, sg, i, trans,
+                               p->d_coefC.ptr, p->hasA ? p->d_coefA.ptr : nullptr, p->d_x.ptr, b_P,
+                               (int64_t)td.nqt, k, b_t);
+            if (e == hipSuccess) e = hipGetLastError();
+            deriv(b_t, b_a, i, p->d_D1.ptr, trans);
+            axpy(1.0, b_a, b_acc);
+          }
+        }
+      };
+      if (mode == evr::kModeHpsi || mode == evr::kModeB2G) {
+        hipLaunchKernelGGL(evr::k_stream_gather<T>, dim3(blocks(nbk)), dim3(256), 0, p->stream, X, nvec, c0, k,
+                           p->d_map.ptr + td.off_nb, (int64_t)td.nbt, b_in);
+        if (e == hipSuccess) e = hipGetLastError();
+      }
+      if (mode == evr::kModeB2G) {
+        b2g(b_in, b_g, 0);
+        hipLaunchKernelGGL(evr::k_stream_grid_out<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, b_g,
+                           (int64_t)td.nqt, k, c0, G + td.off_nq, p->sum_nq);
+        if (e == hipSuccess) e = hipGetLastError();
+      } else if (mode == evr::kModeG2B) {
+        hipLaunchKernelGGL(evr::k_stream_grid_in<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream,
+                           Gin + td.off_nq, p->sum_nq, (int64_t)td.nqt, k, c0, b_g);
+        if (e == hipSuccess) e = hipGetLastError();
+        g2b(b_g, b_y0, 0);
+        const double w = std::fabs(td.w) < 1e-6 ? 0.0 : td.w;  // BGSG4:1004
+        hipLaunchKernelGGL(evr::k_stream_scatter<T>, dim3(blocks(nbk)), dim3(256), 0, p->stream, b_y0,
+                           (const T *)nullptr, w, nvec, c0, k, p->d_rowpos.ptr + td.off_nb, (int64_t)td.nbt, R);
+        if (e == hipSuccess) e = hipGetLastError();
+      } else {
+        b2g(b_in, b_g, 0);
+        grid_op(0);
+        g2b(b_acc, b_y0, 0);
+        // the hermitian form: (H_t + H_t^T) / 2, the transpose through W^T,
+        // the transposed grid operator and B^T (the V part is symmetric)
+        const bool sym = p->hermitian && p->hasKeo;
+        if (sym) {
+          b2g(b_in, b_g, 1);
+          grid_op(1);
+          g2b(b_acc, b_y1, 1);
+        }
+        hipLaunchKernelGGL(evr::k_stream_scatter<T>, dim3(blocks(nbk)), dim3(256), 0, p->stream, b_y0,
+                           sym ? (const T *)b_y1 : (const T *)nullptr, td.w, nvec, c0, k,
+                           p->d_rowpos.ptr + td.off_nb, (int64_t)td.nbt, R);
+        if (e == hipSuccess) e = hipGetLastError();
+      }
+      if (e != hipSuccess) return fail(EVR_ERR_HIP, std::string("streamed term: ") + hipGetErrorString(e));
+    }
+  }
   return EVR_OK;
 }
 
@@ -436,6 +611,10 @@ int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *
       p->flops_acc[c] += p->flops_class[c] * nvec;
       p->launches[c] += 1;
     }
+  }
+  if (!p->big.empty()) {
+    if (diagKeo) return fail(EVR_ERR_ARG, "the kinetic diagonal of a streamed term is not formed here");
+    return run_streamed<T>(p, mode, nvec, X, R, G, Gin);
   }
   return EVR_OK;
 }
@@ -900,8 +1079,10 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->slots.resize(ns);
   for (int s = 0; s < ns; ++s) {
     const int nb = bd->nb[s], nq = bd->nq[s];
-    if (nb < 1 || nq < nb || nb > evr::kMaxNb || nq > evr::kMaxNq)
-      return fail(EVR_ERR_ARG, "each slot needs 1 <= nb <= nq, nb <= 31, nq <= 64");
+    if (nb < 1 || nq < nb || nq > evr::kMaxNq) return fail(EVR_ERR_ARG, "each slot needs 1 <= nb <= nq <= 64");
+    // above 31 functions (the term kernels' 5-bit basis digits) a term runs streamed
+    if (nb > evr::kMaxNb && D > evr::kMaxDev)
+      return fail(EVR_ERR_ARG, "with more than 12 modes each slot needs nb <= 31");
     evr::SlotDev &sd = p->slots[s];
     sd.nb = nb;
     sd.nq = nq;
@@ -1113,7 +1294,15 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
     td.lds_mats = mats + (mats & 1);  // keep the data arrays 16-byte aligned
     int c = 0;
     while (c < kNumClasses && kClasses[c].max_nbt() < nbt) ++c;
-    if (c == kNumClasses) return fail(EVR_ERR_SIZE, "term basis larger than 8192 functions");
+    bool wide = false;  // a slot beyond the term kernels' 5-bit basis digits
+    for (int d = 0; d < D; ++d) wide = wide || nbd[d] > evr::kMaxNb;
+    if (c == kNumClasses || wide) {
+      // beyond one workgroup: the streamed passes over global memory
+      if (compact) return fail(EVR_ERR_SIZE, "with more than 12 modes a term may have at most 8192 functions");
+      c = -1;
+      p->big.push_back(t);
+      p->big_nq = std::max<int64_t>(p->big_nq, nqt);
+    }
     cls[t] = c;
     // mode lists: all modes (basis <-> grid transforms), the modes with more
     // than one function (the H|psi> V part, one-function modes folded into
@@ -1145,9 +1334,35 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->sum_vq = onvq;
   if (onb > INT32_MAX) return fail(EVR_ERR_SIZE, "sum of term basis sizes exceeds int32");
 
+  // ---- a term whose slabs do not fit one workgroup's LDS in its class (a
+  // long grid next to many functions) runs streamed as well
+  {
+    std::vector<evr::StepDev> probe;
+    for (int64_t t = 0; t < p->nterms; ++t) {
+      if (cls[t] < 0) continue;
+      size_t words = 0;
+      probe.clear();
+      evr::TermDev td = p->terms[t];
+      if (layout_term(p.get(), td, cls[t], probe, words)) continue;
+      if (compact) return fail(EVR_ERR_SIZE, "term does not fit in LDS");
+      cls[t] = -1;
+      p->big.push_back(t);
+      p->big_nq = std::max<int64_t>(p->big_nq, td.nqt);
+    }
+    std::sort(p->big.begin(), p->big.end());
+  }
+
   // ---- slab layouts for the classes (the static rule: the smallest class
   // that holds the term) and the per-class work lists
   p->tclass = cls;
+  p->hC = cC;
+  p->hF1 = cF1;
+  if (!p->big.empty()) {
+    // vectors per streamed pass: the (8 + D) work tensors within 4 GiB
+    const double per_vec = 8.0 * (8 + Dk) * (double)p->big_nq;
+    p->big_k = (int)std::max<int64_t>(1, std::min<int64_t>(max_nvec, (int64_t)(4294967296.0 / per_vec)));
+    HIP_TRY(p->d_big.alloc((size_t)(8 + Dk) * p->big_nq * p->big_k));
+  }
   {
     const int rc = apply_classes(p.get());
     if (rc) return rc;
@@ -1351,6 +1566,7 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
   if (p->D != p->D_full) return fail(EVR_ERR_ARG, "G per grid point needs D <= 12");
   if (p->dp_order) return fail(EVR_ERR_ARG, "G per grid point is for SG4 bases");
   if (p->hermitian) return fail(EVR_ERR_ARG, "G per grid point has no symmetric form (hermitian = 0)");
+  if (!p->big.empty()) return fail(EVR_ERR_ARG, "G per grid point needs terms of at most 8192 functions");
   const int D = p->D;
   const int64_t nq = p->sum_nq;
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1381,6 +1597,7 @@ int evr_plan_set_type1_grid(evr_plan *p, const double *F2, const double *F1)
   if (p->D != p->D_full) return fail(EVR_ERR_ARG, "operator grids per point need D <= 12");
   if (p->dp_order) return fail(EVR_ERR_ARG, "operator grids per point are for SG4 bases");
   if (p->hermitian) return fail(EVR_ERR_ARG, "operator grids per point have no symmetric form (hermitian = 0)");
+  if (!p->big.empty()) return fail(EVR_ERR_ARG, "operator grids per point need terms of at most 8192 functions");
   const int D = p->D;
   const int64_t nq = p->sum_nq;
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1422,6 +1639,8 @@ int evr_plan_assemble(evr_plan *p, int which)
     return fail(EVR_ERR_ARG, "which must be EVR_ASM_FP64 and/or EVR_ASM_FP32");
   if (p->keo_grid)
     return fail(EVR_ERR_ARG, "a plan with operator grids per point runs the grid form (not assembled)");
+  if (!p->big.empty())
+    return fail(EVR_ERR_ARG, "terms above 8192 functions run streamed over global memory (not assembled)");
   const size_t n = (size_t)p->asm_elems;
   if ((which & EVR_ASM_FP64) && !p->d_A64.ptr) {
     if (p->d_A64.alloc(n) != hipSuccess) {
@@ -1497,7 +1716,8 @@ int evr_plan_release_assembled(evr_plan *p)
 static void recount_flops(evr_plan *p)
 {
   for (int c = 0; c < kNumClasses; ++c) p->flops_class[c] = 0;
-  for (int64_t t = 0; t < p->nterms; ++t) p->flops_class[p->tclass[t]] += term_flops(p, p->terms[t]);
+  for (int64_t t = 0; t < p->nterms; ++t)
+    if (p->tclass[t] >= 0) p->flops_class[p->tclass[t]] += term_flops(p, p->terms[t]);
 }
 
 int evr_plan_autotune(evr_plan *p, int nvec, int flags, int reps, int *chosen, double *ms_static,
@@ -1554,8 +1774,8 @@ int evr_plan_autotune(evr_plan *p, int nvec, int flags, int reps, int *chosen, d
   std::vector<int> nat(p->nterms);
   for (int64_t t = 0; t < p->nterms; ++t) {
     int c = 0;
-    while (kClasses[c].max_nbt() < p->terms[t].nbt) ++c;
-    nat[t] = c;
+    while (c < kNumClasses && kClasses[c].max_nbt() < p->terms[t].nbt) ++c;
+    nat[t] = c < kNumClasses && p->tclass[t] >= 0 ? c : -1;  // -1: streamed, not tuned
   }
   p->tclass = nat;
   int rc = apply_classes(p);
@@ -1860,10 +2080,11 @@ int evr_basis_to_grid(evr_plan *p, const double *psiB, double *psiG)
 int evr_plan_diagonal(evr_plan *p, double *diag, int flags)
 {
   if (!p || !diag) return fail(EVR_ERR_ARG, "NULL argument");
-  if (p->keo_grid) {
+  if (p->keo_grid || !p->big.empty()) {
     // G, J, s per grid point: the per-point factors do not separate over the
     // modes, so the diagonal is read off H applied to blocks of unit vectors
-    // (nb_packed / max_nvec applications: exact, and costly for large bases)
+    // (nb_packed / max_nvec applications: exact, and costly for large bases);
+    // likewise with streamed terms, whose passes form no kinetic diagonal
     const int64_t nb = p->nb_packed;
     const int nv = std::max(1, p->max_nvec);
     DevBuf<double> X, Y, out;

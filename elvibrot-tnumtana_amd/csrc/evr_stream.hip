@@ -1,0 +1,185 @@
+// Streamed terms: a term whose basis exceeds what one workgroup holds in LDS
+// (more than 8192 functions, e.g. a direct-product basis of 3 modes x 21
+// functions, or an SG4 term of 6 modes at level 2) runs the reference's
+// per-term sequence as whole-device passes over global memory, one mode at a
+// time:
+//   basis -> grid   BDP_TO_GDP_OF_SmolyakRep (BGSG4:2385-2496)
+//   grid operator   OpPsi_OF_ONEDP_FOR_SG4 (OPSG4:1448-1650): V, the type-1
+//                   F2/F1 derivative sums or the type-10 -1/2 d_i G_ji(q) d_j
+//   grid -> basis   GDP_TO_BDP_OF_SmolyakRep (BGSG4:2497-2581)
+// with the grid derivative of DerivOp_TO_RDP_OF_SmolaykRep (BGSG4:2690-2795).
+// Tensors are [point or function][vector]: the vectors of the chunk fastest,
+// then mode 0, mode 1, ... (the term's own basis and grid orders).
+#pragma once
+
+#include "evr_kernels.hpp"
+
+namespace evr {
+
+// one mode pass: out[l + L (o + nout r)] = sum_i M(o, i) in[l + L (i + nin r)]
+// with M(o, i) = M[o + ld i], or M[i + ld o] when trans; the matrix (at most
+// 64 x 64, FP64 in memory) is staged in LDS in T as sM[o nin + i].  A thread
+// keeps the nin inputs of its pencil (l, r) in registers and writes a chunk
+// of osz consecutive outputs (nchunk chunks per pencil, the chunk index
+// slowest), so every input is read nchunk times and consecutive threads take
+// consecutive l (coalesced whenever L >= 64).  MAXIN >= nin.
+template <typename T, int MAXIN>
+__global__ __launch_bounds__(256) void k_mode_pass(const T *__restrict__ in, T *__restrict__ out,
+                                                   const double *__restrict__ M, int ld, int trans,
+                                                   int64_t L, int nin, int nout, int64_t R, int osz)
+{
+  __shared__ T sM[kMaxNq * kMaxNq];
+  for (int e = threadIdx.x; e < nin * nout; e += blockDim.x) {
+    const int o = e / nin, i = e - o * nin;
+    sM[e] = T(trans ? M[i + (int64_t)ld * o] : M[o + (int64_t)ld * i]);
+  }
+  __syncthreads();
+  const int64_t LR = L * R;
+  const int nchunk = (nout + osz - 1) / osz;
+  const int64_t total = LR * nchunk;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ch = idx / LR;
+    const int64_t lr = idx - ch * LR;
+    const int64_t r = lr / L;
+    const int64_t l = lr - r * L;
+    const T *src = in + l + L * nin * r;
+    T *dst = out + l + L * nout * r;
+    T x[MAXIN];
+#pragma unroll
+    for (int i = 0; i < MAXIN; ++i) x[i] = i < nin ? src[L * i] : T(0);
+    const int o1 = min(nout, (int)(ch + 1) * osz);
+    for (int o = (int)ch * osz; o < o1; ++o) {
+      const T *m = sM + o * nin;
+      T acc = T(0);
+#pragma unroll
+      for (int i = 0; i < MAXIN; ++i)
+        if (i < nin) acc = fma(m[i], x[i], acc);
+      dst[L * o] = acc;
+    }
+  }
+}
+
+// out[r k + v] = X[map[r] nvec + c0 + v] (0 for a pruned function, map < 0)
+template <typename T>
+__global__ __launch_bounds__(256) void k_stream_gather(const T *__restrict__ X, int nvec, int c0, int k,
+                                                       const int *__restrict__ map, int64_t nbt,
+                                                       T *__restrict__ out)
+{
+  const int64_t n = nbt * k;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / k;
+    const int v = (int)(i - r * k);
+    const int pk = map[r];
+    out[i] = pk >= 0 ? X[(int64_t)pk * nvec + c0 + v] : T(0);
+  }
+}
+
+// R[rowpos[r] nvec + c0 + v] = w Y[r k + v], or w (Y + Yt) / 2 with the
+// transposed pass Yt (rows of pruned functions, rowpos < 0, are skipped)
+template <typename T>
+__global__ __launch_bounds__(256) void k_stream_scatter(const T *__restrict__ Y, const T *__restrict__ Yt,
+                                                        double w, int nvec, int c0, int k,
+                                                        const int *__restrict__ rowpos, int64_t nbt,
+                                                        T *__restrict__ R)
+{
+  const int64_t n = nbt * k;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / k;
+    const int v = (int)(i - r * k);
+    const int rp = rowpos[r];
+    if (rp < 0) continue;
+    const T y = Yt ? T(0.5) * (Y[i] + Yt[i]) : Y[i];
+    R[(int64_t)rp * nvec + c0 + v] = T(w) * y;
+  }
+}
+
+// grid columns <-> [point][vector]: G[(c0 + v) gld + q] with G at the term's
+// first point
+template <typename T>
+__global__ __launch_bounds__(256) void k_stream_grid_out(const T *__restrict__ Gt, int64_t nqt, int k, int c0,
+                                                         T *__restrict__ G, int64_t gld)
+{
+  const int64_t n = nqt * k;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = i / k;
+    const int v = (int)(i - q * k);
+    G[(int64_t)(c0 + v) * gld + q] = Gt[i];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_stream_grid_in(const T *__restrict__ G, int64_t gld, int64_t nqt, int k,
+                                                        int c0, T *__restrict__ Gt)
+{
+  const int64_t n = nqt * k;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = i / k;
+    const int v = (int)(i - q * k);
+    Gt[i] = G[(int64_t)(c0 + v) * gld + q];
+  }
+}
+
+// out[q k + v] = V[q] g[q k + v] (V NULL: 0)
+template <typename T>
+__global__ __launch_bounds__(256) void k_stream_vmul(const double *__restrict__ V, const T *__restrict__ g,
+                                                     int64_t nqt, int k, T *__restrict__ out)
+{
+  const int64_t n = nqt * k;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = V ? T(V[i / k]) * g[i] : T(0);
+}
+
+// acc[i] += f t[i]
+template <typename T>
+__global__ __launch_bounds__(256) void k_stream_axpy(int64_t n, double f, const T *__restrict__ t,
+                                                     T *__restrict__ acc)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc[i] = fma(T(f), t[i], acc[i]);
+}
+
+// grid of one streamed term: extents and coordinates of its modes
+struct StreamGrid {
+  int D;
+  int nq[kMaxDev];
+  int64_t offX[kMaxDev];  // first grid coordinate of each mode in the x pool
+};
+
+// type 10, one i: Q[q k + v] = sum_j c_ij(q) P_j[q k + v] with
+// c_ij(q) = C[i D + j] + sum_m A[(i D + j) D + m] x_m(q) (= -1/2 G_ji(q),
+// OPSG4:1590-1610); trans takes c_ji (the transposed operator)
+template <typename T>
+__global__ __launch_bounds__(256) void k_stream_gcomb(StreamGrid sg, int i, int trans,
+                                                      const double *__restrict__ C,
+                                                      const double *__restrict__ A,
+                                                      const double *__restrict__ xpool,
+                                                      const T *__restrict__ P, int64_t nqt, int k,
+                                                      T *__restrict__ Q)
+{
+  const int D = sg.D;
+  const int64_t n = nqt * k;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = e / k;
+    double x[kMaxDev];
+    if (A) {
+      int64_t rem = q;
+      for (int d = 0; d < D; ++d) {
+        const int64_t nx = rem / sg.nq[d];
+        x[d] = xpool[sg.offX[d] + (rem - nx * sg.nq[d])];
+        rem = nx;
+      }
+    }
+    T acc = T(0);
+    for (int j = 0; j < D; ++j) {
+      const int ij = trans ? j * D + i : i * D + j;
+      double c = C[ij];
+      if (A)
+        for (int m = 0; m < D; ++m) c += A[ij * D + m] * x[m];
+      acc = fma(T(c), P[(int64_t)j * n + e], acc);
+    }
+    Q[e] = acc;
+  }
+}
+
+}  // namespace evr

@@ -179,7 +179,15 @@ int evr_psi_read(const evr_basis_desc *basis, int nb0, int cplx, int max_wp, dou
  * reference's 21D and 24D working tests): the modes of a term with one
  * function on one grid point merge into one scalar position, so each term
  * may have at most 11 other modes; not for a DP basis, a Ga, or
- * evr_plan_gen_potential_poly. */
+ * evr_plan_gen_potential_poly.  1D slots: 1 <= nb <= nq <= 64.  A term runs
+ * in one workgroup with its tensors in LDS when it has at most 8192
+ * functions, slots of at most 31 functions and slabs that fit; any other
+ * term (D <= 12) runs streamed: the reference's basis -> grid, grid
+ * operator and grid -> basis sequence as whole-device passes over global
+ * memory, one mode at a time, with a work block of (8 + D) x nq_t x k
+ * values made here (k vectors per pass, within 4 GiB).  A plan with
+ * streamed terms is not assembled and takes no operator grids per point;
+ * its diagonal is read off H on blocks of unit vectors. */
 int evr_plan_create(const evr_basis_desc *basis, const evr_op_desc *op, int max_nvec,
                     evr_plan **out);
 int evr_plan_destroy(evr_plan *p);

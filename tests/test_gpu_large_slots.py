@@ -1,7 +1,8 @@
 """1D slots beyond 20 points: SG4 levels up to nb = 31 functions (the most a
-5-bit basis digit holds) and nq = 32 points, H|psi> (sum-factorised and
-assembled, FP64) and the basis <-> grid transforms against the oracle; a
-slot with 32 functions is refused with its message."""
+5-bit basis digit of the term kernels holds) and nq = 32 points, H|psi>
+(sum-factorised and assembled, FP64) and the basis <-> grid transforms
+against the oracle; slots of 32 to 64 functions run streamed (checked
+against the oracle), a slot of 65 is refused with its message."""
 import numpy as np
 import pytest
 
@@ -45,7 +46,23 @@ def test_large_slots_transforms(evr, oracle_mod):
     assert relerr(back, x) < 1e-12
 
 
-def test_slot_above_31_functions_refused(evr):
-    b = evr.Basis.sg4(2, 1, 1, 32, 1, 33, 1)   # nb(0) = 32 functions
-    with pytest.raises(RuntimeError, match="nb <= 31"):
+def test_slots_above_31_functions_streamed(evr, oracle_mod):
+    from evrhip import hamiltonian as H
+    # nb(l) = 32 + 8 l, nq(l) = 33 + 8 l: every term has a slot beyond 31
+    s = oracle_mod.SG4(2, 2, 2, 32, 8, 33, 8)
+    b = evr.Basis.sg4(2, 2, 2, 32, 8, 33, 8)
+    ham = H.smolyak_config(9, 2)
+    V = ham.potential(s.grid())
+    X = H.test_vectors(b.nb_packed, 2, 6)
+    ref = s.oppsi(oracle_mod.Op(10, V=V, G0=ham.G0, Ga=ham.Ga_fortran()), X)
+    plan = evr.Plan(b, type=10, V=V, G0=ham.G0, Ga=ham.Ga_fortran(), max_nvec=2)
+    e = relerr(plan.TabOpPsi(X), ref)
+    g = plan.BasisRep_TO_GridRep(X[:, 0])
+    print("slots of 32..48 functions (streamed): H|psi> %.2e, B->G %.2e" % (e, relerr(g, s.b2g(X[:, 0]))))
+    assert e < TOL64 and relerr(g, s.b2g(X[:, 0])) < TOL64
+
+
+def test_slot_above_64_functions_refused(evr):
+    b = evr.Basis.sg4(2, 1, 1, 65, 1, 66, 1)   # nb(0) = 65 functions
+    with pytest.raises(RuntimeError, match="nb <= nq <= 64"):
         evr.Plan(b, type=0, V=np.ones(b.sum_nq), max_nvec=1)
