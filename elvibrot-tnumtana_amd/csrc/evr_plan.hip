@@ -443,7 +443,7 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
       const int64_t nbk = (int64_t)td.nbt * k, nqk = (int64_t)td.nqt * k;
       hipError_t e = hipSuccess;
       auto pass = [&](const T *in, T *out, const double *M, int ld, int trans, int64_t L, int nin, int nout,
-                      int64_t Rr) {
+                      int64_t Rr, int accum = 0, double scale = 1.0) {
         // output chunks per pencil: enough threads for about 1024 per CU
         const int64_t want = (int64_t)p->ncu * 1024;
         const int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(nout, want / std::max<int64_t>(1, L * Rr)));
@@ -451,11 +451,14 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
         const int64_t thr = L * Rr * ((nout + osz - 1) / osz);
         const dim3 g(blocks(thr)), b(256);
         if (nin <= 16)
-          hipLaunchKernelGGL((evr::k_mode_pass<T, 16>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz);
+          hipLaunchKernelGGL((evr::k_mode_pass<T, 16>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz,
+                             accum, scale);
         else if (nin <= 32)
-          hipLaunchKernelGGL((evr::k_mode_pass<T, 32>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz);
+          hipLaunchKernelGGL((evr::k_mode_pass<T, 32>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz,
+                             accum, scale);
         else
-          hipLaunchKernelGGL((evr::k_mode_pass<T, 64>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz);
+          hipLaunchKernelGGL((evr::k_mode_pass<T, 64>), g, b, 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, osz,
+                             accum, scale);
         if (e == hipSuccess) e = hipGetLastError();
       };
       // basis -> grid over all modes (B, or W^T for the transposed operator)
@@ -484,16 +487,14 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
           cur = out;
         }
       };
-      // grid derivative on mode d (D1 or D2 of its slot, nq x nq)
-      auto deriv = [&](const T *src, T *dst, int d, const double *Dm, int trans) {
+      // grid derivative on mode d (D1 or D2 of its slot, nq x nq); with
+      // accum, dst += scale d(src)
+      auto deriv = [&](const T *src, T *dst, int d, const double *Dm, int trans, int accum = 0,
+                       double scale = 1.0) {
         int64_t L = k, Rr = 1;
         for (int m = 0; m < d; ++m) L *= nq[m];
         for (int m = d + 1; m < D; ++m) Rr *= nq[m];
-        pass(src, dst, Dm + sl[d]->offD1, nq[d], trans, L, nq[d], nq[d], Rr);
-      };
-      auto axpy = [&](double f, const T *x, T *acc) {
-        hipLaunchKernelGGL(evr::k_stream_axpy<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, nqk, f, x, acc);
-        if (e == hipSuccess) e = hipGetLastError();
+        pass(src, dst, Dm + sl[d]->offD1, nq[d], trans, L, nq[d], nq[d], Rr, accum, scale);
       };
       // the grid operator of the term on b_g into b_acc (OPSG4:1448-1650)
       auto grid_op = [&](int trans) {
@@ -507,19 +508,15 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
               const double f = p->hC[i * D + j];
               if (f == 0.0) continue;
               if (i == j) {
-                deriv(b_g, b_t, i, p->d_D2.ptr, trans);
+                deriv(b_g, b_acc, i, p->d_D2.ptr, trans, 1, f);
               } else {
                 deriv(b_g, b_a, i, p->d_D1.ptr, trans);
-                deriv(b_a, b_t, j, p->d_D1.ptr, trans);
+                deriv(b_a, b_acc, j, p->d_D1.ptr, trans, 1, f);
               }
-              axpy(f, b_t, b_acc);
             }
           if (p->hasF1)
-            for (int i = 0; i < D; ++i) {
-              if (p->hF1[i] == 0.0) continue;
-              deriv(b_g, b_t, i, p->d_D1.ptr, trans);
-              axpy(p->hF1[i], b_t, b_acc);
-            }
+            for (int i = 0; i < D; ++i)
+              if (p->hF1[i] != 0.0) deriv(b_g, b_acc, i, p->d_D1.ptr, trans, 1, p->hF1[i]);
         } else if (p->type == 10) {
           evr::StreamGrid sg{};
           sg.D = D;
@@ -533,8 +530,7 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
                                p->d_coefC.ptr, p->hasA ? p->d_coefA.ptr : nullptr, p->d_x.ptr, b_P,
                                (int64_t)td.nqt, k, b_t);
             if (e == hipSuccess) e = hipGetLastError();
-            deriv(b_t, b_a, i, p->d_D1.ptr, trans);
-            axpy(1.0, b_a, b_acc);
+            deriv(b_t, b_acc, i, p->d_D1.ptr, trans, 1, 1.0);
           }
         }
       };

@@ -22,11 +22,13 @@ namespace evr {
 // keeps the nin inputs of its pencil (l, r) in registers and writes a chunk
 // of osz consecutive outputs (nchunk chunks per pencil, the chunk index
 // slowest), so every input is read nchunk times and consecutive threads take
-// consecutive l (coalesced whenever L >= 64).  MAXIN >= nin.
+// consecutive l (coalesced whenever L >= 64).  MAXIN >= nin.  With accum
+// the pass adds scale times its result to out instead of storing it.
 template <typename T, int MAXIN>
 __global__ __launch_bounds__(256) void k_mode_pass(const T *__restrict__ in, T *__restrict__ out,
                                                    const double *__restrict__ M, int ld, int trans,
-                                                   int64_t L, int nin, int nout, int64_t R, int osz)
+                                                   int64_t L, int nin, int nout, int64_t R, int osz,
+                                                   int accum, double scale)
 {
   __shared__ T sM[kMaxNq * kMaxNq];
   for (int e = threadIdx.x; e < nin * nout; e += blockDim.x) {
@@ -55,7 +57,8 @@ __global__ __launch_bounds__(256) void k_mode_pass(const T *__restrict__ in, T *
 #pragma unroll
       for (int i = 0; i < MAXIN; ++i)
         if (i < nin) acc = fma(m[i], x[i], acc);
-      dst[L * o] = acc;
+      if (accum) dst[L * o] = fma(T(scale), acc, dst[L * o]);
+      else dst[L * o] = acc;
     }
   }
 }
@@ -128,15 +131,6 @@ __global__ __launch_bounds__(256) void k_stream_vmul(const double *__restrict__ 
   const int64_t n = nqt * k;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     out[i] = V ? T(V[i / k]) * g[i] : T(0);
-}
-
-// acc[i] += f t[i]
-template <typename T>
-__global__ __launch_bounds__(256) void k_stream_axpy(int64_t n, double f, const T *__restrict__ t,
-                                                     T *__restrict__ acc)
-{
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    acc[i] = fma(T(f), t[i], acc[i]);
 }
 
 // grid of one streamed term: extents and coordinates of its modes
