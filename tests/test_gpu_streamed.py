@@ -6,8 +6,9 @@ oracle on direct-product bases of 3 x 22 functions (10,648 > 8192; type 10
 with q-linear G, type 1 with F2 and F1, FP64 and FP32), on an SG4 basis
 whose four level-(1,1,1,0) terms have 13^3 x 5 = 10,985 functions beside
 LDS-resident terms (H|psi>, basis <-> grid, the diagonal), on the hermitian
-form, and through block Davidson on a coupled harmonic direct-product
-operator whose levels are known in closed form."""
+form, through block Davidson on a coupled harmonic direct-product operator
+whose levels are known in closed form, and through Chebyshev propagation
+(eager and replayed as captured HIP graphs) against march_cheby on the CPU."""
 import itertools
 
 import numpy as np
@@ -145,3 +146,43 @@ def test_dp_streamed_davidson_harmonic(evr):
     print("coupled harmonic DP Davidson", info.iterations, np.abs(ev - levels).max())
     assert info.converged == 1
     assert np.abs(ev - levels).max() < 1e-9
+
+
+def test_dp_streamed_chebyshev_graph(evr, oracle_mod):
+    """Chebyshev propagation through the streamed path (the hermitian form),
+    eager and with the steps after the first replayed as captured HIP graphs,
+    against march_cheby restated on the CPU with the oracle's H: 3 steps of
+    two wavepackets on the coupled harmonic operator of 3 x 22 functions."""
+    import propa
+    w = np.array([1.0, 1.3, 1.7])
+    K = np.diag(w)
+    K[0, 1] = K[1, 0] = 0.15
+    K[1, 2] = K[2, 1] = -0.1
+    d, b = _dp(evr, oracle_mod)
+    q = d.grid()
+    V = 0.5 * np.einsum("pi,ij,pj->p", q, K, q)
+    F2 = -0.5 * np.diag(w)
+    plan = evr.Plan(b, type=1, V=V, F2=F2, max_nvec=4, hermitian=True)
+    lo, hi = plan.bounds()
+    hmin, hmax = lo - 1.0, hi + 1.0
+    rng = np.random.default_rng(21)
+    psi = rng.standard_normal((b.nb_packed, 2)) + 1j * rng.standard_normal((b.nb_packed, 2))
+    psi /= np.linalg.norm(psi, axis=0)
+    dt, nst = 0.05, 3
+    kw = dict(npoly=400, poly_tol=1e-14, Hmin=hmin, Hmax=hmax)
+    eager, _, ie = plan.propagate(psi, dt, nst, **kw)
+    graph, _, ig = plan.propagate(psi, dt, nst, graph=True, **kw)
+    op = oracle_mod.Op(1, V=V, F2=F2)
+
+    class Hop:
+        def __matmul__(self, x):
+            return d.oppsi(op, np.ascontiguousarray(x.real)) + 1j * d.oppsi(op, np.ascontiguousarray(x.imag))
+
+    ref = psi
+    for _ in range(nst):
+        ref, order, _ = propa.cheby_step(Hop(), ref, dt, hmin, hmax, 400, 1e-14)
+    print("streamed Chebyshev: order %d, eager %.2e, graph %.2e"
+          % (order, np.abs(eager - ref).max(), np.abs(graph - ref).max()))
+    assert np.abs(eager - ref).max() < 1e-10
+    assert np.abs(graph - ref).max() < 1e-10
+    assert np.abs(np.linalg.norm(eager, axis=0) - 1.0).max() < 1e-10
