@@ -131,7 +131,7 @@ struct evr_plan {
   DevBuf<int2> d_gitems;
   // streamed terms (a basis beyond one workgroup's LDS, evr_stream.hip): their
   // ids, the largest term grid among them, the vectors per pass and the work
-  // block of (8 + D) tensors of big_nq x big_k words
+  // block of (7 + D) tensors of big_nq x big_k words
   std::vector<int64_t> big;
   int64_t big_nq = 0;
   int big_k = 0;
@@ -426,7 +426,7 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
   const int64_t S = p->big_nq * p->big_k;
   T *base = reinterpret_cast<T *>(p->d_big.ptr);
   T *b_in = base, *b_g = base + S, *b_a = base + 2 * S, *b_b = base + 3 * S, *b_acc = base + 4 * S,
-    *b_t = base + 5 * S, *b_y0 = base + 6 * S, *b_y1 = base + 7 * S, *b_P = base + 8 * S;
+    *b_y0 = base + 5 * S, *b_y1 = base + 6 * S, *b_P = base + 7 * S;
   auto blocks = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); };
   for (int64_t t : p->big) {
     const evr::TermDev &td = p->terms[t];
@@ -441,9 +441,17 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
     for (int c0 = 0; c0 < nvec; c0 += p->big_k) {
       const int k = std::min(p->big_k, nvec - c0);
       const int64_t nbk = (int64_t)td.nbt * k, nqk = (int64_t)td.nqt * k;
+      if (nqk >= INT32_MAX) return fail(EVR_ERR_SIZE, "streamed term grid too large for one pass");
       hipError_t e = hipSuccess;
       auto pass = [&](const T *in, T *out, const double *M, int ld, int trans, int64_t L, int nin, int nout,
                       int64_t Rr, int accum = 0, double scale = 1.0) {
+        if (L >= 16) {  // tiles of 16 pencils on MFMA
+          const int64_t tiles = (L + 15) / 16 * Rr;
+          hipLaunchKernelGGL(evr::k_mode_mfma<T>, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 65536))),
+                             dim3(256), 0, p->stream, in, out, M, ld, trans, L, nin, nout, Rr, accum, scale);
+          if (e == hipSuccess) e = hipGetLastError();
+          return;
+        }
         // output chunks per pencil: enough threads for about 1024 per CU
         const int64_t want = (int64_t)p->ncu * 1024;
         const int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(nout, want / std::max<int64_t>(1, L * Rr)));
@@ -500,7 +508,7 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
       auto grid_op = [&](int trans) {
         const double *V = p->hasV ? p->d_V.ptr + td.off_nq : nullptr;
         hipLaunchKernelGGL(evr::k_stream_vmul<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, V, b_g,
-                           (int64_t)td.nqt, k, b_acc);
+                           (unsigned)nqk, k, b_acc);
         if (e == hipSuccess) e = hipGetLastError();
         if (p->type == 1) {
           for (int i = 0; i < D; ++i)
@@ -528,30 +536,30 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
           for (int i = 0; i < D; ++i) {
             hipLaunchKernelGGL(evr::k_stream_gcomb<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, sg, i, trans,
                                p->d_coefC.ptr, p->hasA ? p->d_coefA.ptr : nullptr, p->d_x.ptr, b_P,
-                               (int64_t)td.nqt, k, b_t);
+                               (unsigned)nqk, k, b_a);
             if (e == hipSuccess) e = hipGetLastError();
-            deriv(b_t, b_acc, i, p->d_D1.ptr, trans, 1, 1.0);
+            deriv(b_a, b_acc, i, p->d_D1.ptr, trans, 1, 1.0);
           }
         }
       };
       if (mode == evr::kModeHpsi || mode == evr::kModeB2G) {
         hipLaunchKernelGGL(evr::k_stream_gather<T>, dim3(blocks(nbk)), dim3(256), 0, p->stream, X, nvec, c0, k,
-                           p->d_map.ptr + td.off_nb, (int64_t)td.nbt, b_in);
+                           p->d_map.ptr + td.off_nb, (unsigned)nbk, b_in);
         if (e == hipSuccess) e = hipGetLastError();
       }
       if (mode == evr::kModeB2G) {
         b2g(b_in, b_g, 0);
         hipLaunchKernelGGL(evr::k_stream_grid_out<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, b_g,
-                           (int64_t)td.nqt, k, c0, G + td.off_nq, p->sum_nq);
+                           (unsigned)nqk, k, c0, G + td.off_nq, p->sum_nq);
         if (e == hipSuccess) e = hipGetLastError();
       } else if (mode == evr::kModeG2B) {
         hipLaunchKernelGGL(evr::k_stream_grid_in<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream,
-                           Gin + td.off_nq, p->sum_nq, (int64_t)td.nqt, k, c0, b_g);
+                           Gin + td.off_nq, p->sum_nq, (unsigned)nqk, k, c0, b_g);
         if (e == hipSuccess) e = hipGetLastError();
         g2b(b_g, b_y0, 0);
         const double w = std::fabs(td.w) < 1e-6 ? 0.0 : td.w;  // BGSG4:1004
         hipLaunchKernelGGL(evr::k_stream_scatter<T>, dim3(blocks(nbk)), dim3(256), 0, p->stream, b_y0,
-                           (const T *)nullptr, w, nvec, c0, k, p->d_rowpos.ptr + td.off_nb, (int64_t)td.nbt, R);
+                           (const T *)nullptr, w, nvec, c0, k, p->d_rowpos.ptr + td.off_nb, (unsigned)nbk, R);
         if (e == hipSuccess) e = hipGetLastError();
       } else {
         b2g(b_in, b_g, 0);
@@ -567,7 +575,7 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
         }
         hipLaunchKernelGGL(evr::k_stream_scatter<T>, dim3(blocks(nbk)), dim3(256), 0, p->stream, b_y0,
                            sym ? (const T *)b_y1 : (const T *)nullptr, td.w, nvec, c0, k,
-                           p->d_rowpos.ptr + td.off_nb, (int64_t)td.nbt, R);
+                           p->d_rowpos.ptr + td.off_nb, (unsigned)nbk, R);
         if (e == hipSuccess) e = hipGetLastError();
       }
       if (e != hipSuccess) return fail(EVR_ERR_HIP, std::string("streamed term: ") + hipGetErrorString(e));
@@ -1354,10 +1362,10 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->hC = cC;
   p->hF1 = cF1;
   if (!p->big.empty()) {
-    // vectors per streamed pass: the (8 + D) work tensors within 4 GiB
-    const double per_vec = 8.0 * (8 + Dk) * (double)p->big_nq;
+    // vectors per streamed pass: the (7 + D) work tensors within 4 GiB
+    const double per_vec = 8.0 * (7 + Dk) * (double)p->big_nq;
     p->big_k = (int)std::max<int64_t>(1, std::min<int64_t>(max_nvec, (int64_t)(4294967296.0 / per_vec)));
-    HIP_TRY(p->d_big.alloc((size_t)(8 + Dk) * p->big_nq * p->big_k));
+    HIP_TRY(p->d_big.alloc((size_t)(7 + Dk) * p->big_nq * p->big_k));
   }
   {
     const int rc = apply_classes(p.get());

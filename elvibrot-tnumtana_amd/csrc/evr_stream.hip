@@ -63,16 +63,79 @@ __global__ __launch_bounds__(256) void k_mode_pass(const T *__restrict__ in, T *
   }
 }
 
+// the same mode pass on MFMA when L >= 16: a wave takes a tile of 16
+// consecutive l of one r and all outputs, out(16 o x 16 l) = M(16 o x 4 i)
+// in(4 i x 16 l) accumulated over K-steps of 4 (v_mfma_*_16x16x4): the
+// matrix from LDS, zero-padded to 16 rows and 4 columns, the inputs loaded
+// once from global into registers (zero past nin or L), each output written
+// once.  A lane holds A(row lane % 16, k lane / 16), B(k lane / 16, column
+// lane % 16) and D(row lane / 16 + 4 j for FP64, 4 (lane / 16) + j for FP32,
+// column lane % 16).
+template <typename T>
+__global__ __launch_bounds__(256) void k_mode_mfma(const T *__restrict__ in, T *__restrict__ out,
+                                                   const double *__restrict__ M, int ld, int trans,
+                                                   int64_t L, int nin, int nout, int64_t R, int accum,
+                                                   double scale)
+{
+  using acc_t = T __attribute__((ext_vector_type(4)));
+  constexpr int KMAX = kMaxNq / 4;
+  __shared__ T sA[kMaxNq * kMaxNq];
+  const int nin_p = (nin + 3) & ~3, nout_p = (nout + 15) & ~15;
+  for (int e = threadIdx.x; e < nout_p * nin_p; e += blockDim.x) {
+    const int o = e / nin_p, i = e - o * nin_p;
+    sA[e] = (o < nout && i < nin) ? T(trans ? M[i + (int64_t)ld * o] : M[o + (int64_t)ld * i]) : T(0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ks = nin_p / 4, ot = nout_p / 16;
+  const int64_t ltiles = (L + 15) / 16, ntiles = ltiles * R;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
+    const int64_t r = tile / ltiles;
+    const int64_t l = (tile - r * ltiles) * 16 + (lane & 15);
+    const bool lok = l < L;
+    const T *src = in + l + L * nin * r;
+    T bf[KMAX];
+#pragma unroll
+    for (int s = 0; s < KMAX; ++s) {
+      const int i = 4 * s + (lane >> 4);
+      bf[s] = (s < ks && lok && i < nin) ? src[L * i] : T(0);
+    }
+    T *dst = out + l + L * nout * r;
+    for (int t = 0; t < ot; ++t) {
+      acc_t acc = {T(0), T(0), T(0), T(0)};
+      const T *arow = sA + (16 * t + (lane & 15)) * nin_p + (lane >> 4);
+#pragma unroll
+      for (int s = 0; s < KMAX; ++s)
+        if (s < ks) {
+          if constexpr (sizeof(T) == 8)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[4 * s], bf[s], acc, 0, 0, 0);
+          else
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * s], bf[s], acc, 0, 0, 0);
+        }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int o = 16 * t + (sizeof(T) == 8 ? (lane >> 4) + 4 * j : 4 * (lane >> 4) + j);
+        if (lok && o < nout) {
+          if (accum) dst[L * o] = fma(T(scale), acc[j], dst[L * o]);
+          else dst[L * o] = acc[j];
+        }
+      }
+    }
+  }
+}
+
+// The elementwise kernels below index with 32 bits: the host keeps every
+// tensor of a pass below 2^31 values (the work block is at most 4 GiB).
+
 // out[r k + v] = X[map[r] nvec + c0 + v] (0 for a pruned function, map < 0)
 template <typename T>
 __global__ __launch_bounds__(256) void k_stream_gather(const T *__restrict__ X, int nvec, int c0, int k,
-                                                       const int *__restrict__ map, int64_t nbt,
+                                                       const int *__restrict__ map, unsigned n,
                                                        T *__restrict__ out)
 {
-  const int64_t n = nbt * k;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / k;
-    const int v = (int)(i - r * k);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned r = i / (unsigned)k;
+    const int v = (int)(i - r * (unsigned)k);
     const int pk = map[r];
     out[i] = pk >= 0 ? X[(int64_t)pk * nvec + c0 + v] : T(0);
   }
@@ -83,13 +146,12 @@ __global__ __launch_bounds__(256) void k_stream_gather(const T *__restrict__ X, 
 template <typename T>
 __global__ __launch_bounds__(256) void k_stream_scatter(const T *__restrict__ Y, const T *__restrict__ Yt,
                                                         double w, int nvec, int c0, int k,
-                                                        const int *__restrict__ rowpos, int64_t nbt,
+                                                        const int *__restrict__ rowpos, unsigned n,
                                                         T *__restrict__ R)
 {
-  const int64_t n = nbt * k;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / k;
-    const int v = (int)(i - r * k);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned r = i / (unsigned)k;
+    const int v = (int)(i - r * (unsigned)k);
     const int rp = rowpos[r];
     if (rp < 0) continue;
     const T y = Yt ? T(0.5) * (Y[i] + Yt[i]) : Y[i];
@@ -100,25 +162,23 @@ __global__ __launch_bounds__(256) void k_stream_scatter(const T *__restrict__ Y,
 // grid columns <-> [point][vector]: G[(c0 + v) gld + q] with G at the term's
 // first point
 template <typename T>
-__global__ __launch_bounds__(256) void k_stream_grid_out(const T *__restrict__ Gt, int64_t nqt, int k, int c0,
+__global__ __launch_bounds__(256) void k_stream_grid_out(const T *__restrict__ Gt, unsigned n, int k, int c0,
                                                          T *__restrict__ G, int64_t gld)
 {
-  const int64_t n = nqt * k;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t q = i / k;
-    const int v = (int)(i - q * k);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned q = i / (unsigned)k;
+    const int v = (int)(i - q * (unsigned)k);
     G[(int64_t)(c0 + v) * gld + q] = Gt[i];
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_stream_grid_in(const T *__restrict__ G, int64_t gld, int64_t nqt, int k,
+__global__ __launch_bounds__(256) void k_stream_grid_in(const T *__restrict__ G, int64_t gld, unsigned n, int k,
                                                         int c0, T *__restrict__ Gt)
 {
-  const int64_t n = nqt * k;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t q = i / k;
-    const int v = (int)(i - q * k);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned q = i / (unsigned)k;
+    const int v = (int)(i - q * (unsigned)k);
     Gt[i] = G[(int64_t)(c0 + v) * gld + q];
   }
 }
@@ -126,11 +186,10 @@ __global__ __launch_bounds__(256) void k_stream_grid_in(const T *__restrict__ G,
 // out[q k + v] = V[q] g[q k + v] (V NULL: 0)
 template <typename T>
 __global__ __launch_bounds__(256) void k_stream_vmul(const double *__restrict__ V, const T *__restrict__ g,
-                                                     int64_t nqt, int k, T *__restrict__ out)
+                                                     unsigned n, int k, T *__restrict__ out)
 {
-  const int64_t n = nqt * k;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = V ? T(V[i / k]) * g[i] : T(0);
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = V ? T(V[i / (unsigned)k]) * g[i] : T(0);
 }
 
 // grid of one streamed term: extents and coordinates of its modes
@@ -142,35 +201,39 @@ struct StreamGrid {
 
 // type 10, one i: Q[q k + v] = sum_j c_ij(q) P_j[q k + v] with
 // c_ij(q) = C[i D + j] + sum_m A[(i D + j) D + m] x_m(q) (= -1/2 G_ji(q),
-// OPSG4:1590-1610); trans takes c_ji (the transposed operator)
+// OPSG4:1590-1610); trans takes c_ji (the transposed operator).  P holds D
+// tensors of n values.
 template <typename T>
 __global__ __launch_bounds__(256) void k_stream_gcomb(StreamGrid sg, int i, int trans,
                                                       const double *__restrict__ C,
                                                       const double *__restrict__ A,
                                                       const double *__restrict__ xpool,
-                                                      const T *__restrict__ P, int64_t nqt, int k,
+                                                      const T *__restrict__ P, unsigned n, int k,
                                                       T *__restrict__ Q)
 {
   const int D = sg.D;
-  const int64_t n = nqt * k;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t q = e / k;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     double x[kMaxDev];
     if (A) {
-      int64_t rem = q;
-      for (int d = 0; d < D; ++d) {
-        const int64_t nx = rem / sg.nq[d];
-        x[d] = xpool[sg.offX[d] + (rem - nx * sg.nq[d])];
-        rem = nx;
-      }
+      unsigned rem = e / (unsigned)k;
+#pragma unroll
+      for (int d = 0; d < kMaxDev; ++d)
+        if (d < D) {
+          const unsigned nx = rem / (unsigned)sg.nq[d];
+          x[d] = xpool[sg.offX[d] + (rem - nx * (unsigned)sg.nq[d])];
+          rem = nx;
+        }
     }
     T acc = T(0);
     for (int j = 0; j < D; ++j) {
       const int ij = trans ? j * D + i : i * D + j;
       double c = C[ij];
-      if (A)
-        for (int m = 0; m < D; ++m) c += A[ij * D + m] * x[m];
-      acc = fma(T(c), P[(int64_t)j * n + e], acc);
+      if (A) {
+#pragma unroll
+        for (int m = 0; m < kMaxDev; ++m)
+          if (m < D) c += A[ij * D + m] * x[m];
+      }
+      acc = fma(T(c), P[(size_t)j * n + e], acc);
     }
     Q[e] = acc;
   }

@@ -184,7 +184,7 @@ int evr_psi_read(const evr_basis_desc *basis, int nb0, int cplx, int max_wp, dou
  * functions, slots of at most 31 functions and slabs that fit; any other
  * term (D <= 12) runs streamed: the reference's basis -> grid, grid
  * operator and grid -> basis sequence as whole-device passes over global
- * memory, one mode at a time, with a work block of (8 + D) x nq_t x k
+ * memory, one mode at a time, with a work block of (7 + D) x nq_t x k
  * values made here (k vectors per pass, within 4 GiB).  A plan with
  * streamed terms is not assembled and takes no operator grids per point;
  * its diagonal is read off H on blocks of unit vectors. */
