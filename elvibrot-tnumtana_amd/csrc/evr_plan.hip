@@ -86,6 +86,8 @@ constexpr int kNumClasses = sizeof(kClasses) / sizeof(kClasses[0]);
 // dynamic LDS per workgroup: 160 KiB less a reserve for the kernels' static
 // arrays (about 2 KiB for the term kernels)
 constexpr size_t kLdsLimit = 160 * 1024 - 4096;
+// a one-term plan of at least this many functions runs its term streamed
+constexpr int kStreamOneMin = 256;
 
 }  // namespace
 
@@ -133,6 +135,10 @@ struct evr_plan {
   // ids, the largest term grid among them, the vectors per pass and the work
   // block of (7 + D) tensors of big_nq x big_k words
   std::vector<int64_t> big;
+  // a one-term plan (direct-product basis) whose term fits a workgroup still
+  // runs it streamed for H|psi> and the transforms: the resident kernel would
+  // give the whole device one workgroup per vector group
+  std::vector<int64_t> one;
   int64_t big_nq = 0;
   int big_k = 0;
   DevBuf<double> d_big;
@@ -420,7 +426,8 @@ int apply_classes(evr_plan *p)
 // reads X and writes the grid columns of G, G2B reads the grid columns of Gin
 // and writes R.  Whole-device launches on the plan stream, nothing allocated.
 template <typename T>
-int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *Gin)
+int run_streamed(evr_plan *p, const std::vector<int64_t> &list, int mode, int nvec, const T *X, T *R, T *G,
+                 const T *Gin)
 {
   const int D = p->D;
   const int64_t S = p->big_nq * p->big_k;
@@ -428,7 +435,7 @@ int run_streamed(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const 
   T *b_in = base, *b_g = base + S, *b_a = base + 2 * S, *b_b = base + 3 * S, *b_acc = base + 4 * S,
     *b_y0 = base + 5 * S, *b_y1 = base + 6 * S, *b_P = base + 7 * S;
   auto blocks = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); };
-  for (int64_t t : p->big) {
+  for (int64_t t : list) {
     const evr::TermDev &td = p->terms[t];
     int nb[evr::kMaxDev], nq[evr::kMaxDev];
     const evr::SlotDev *sl[evr::kMaxDev];
@@ -589,6 +596,8 @@ template <typename T>
 int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *Gin, const T *bw,
               const T *keo, const T *V, int diagKeo = 0)
 {
+  if (!p->one.empty() && !diagKeo && (mode == evr::kModeHpsi || mode == evr::kModeB2G || mode == evr::kModeG2B))
+    return run_streamed<T>(p, p->one, mode, nvec, X, R, G, Gin);
   for (int c = 0; c < kNumClasses; ++c) {
     if (p->nunits[c] == 0) continue;
     evr::TermParams<T> prm = base_params<T>(p, mode, nvec);
@@ -618,7 +627,7 @@ int run_terms(evr_plan *p, int mode, int nvec, const T *X, T *R, T *G, const T *
   }
   if (!p->big.empty()) {
     if (diagKeo) return fail(EVR_ERR_ARG, "the kinetic diagonal of a streamed term is not formed here");
-    return run_streamed<T>(p, mode, nvec, X, R, G, Gin);
+    return run_streamed<T>(p, p->big, mode, nvec, X, R, G, Gin);
   }
   return EVR_OK;
 }
@@ -1361,7 +1370,11 @@ int evr_plan_create(const evr_basis_desc *bd, const evr_op_desc *op, int max_nve
   p->tclass = cls;
   p->hC = cC;
   p->hF1 = cF1;
-  if (!p->big.empty()) {
+  if (p->big.empty() && p->nterms == 1 && !compact && p->terms[0].nbt >= kStreamOneMin) {
+    p->one.push_back(0);
+    p->big_nq = p->terms[0].nqt;
+  }
+  if (!p->big.empty() || !p->one.empty()) {
     // vectors per streamed pass: the (7 + D) work tensors within 4 GiB
     const double per_vec = 8.0 * (7 + Dk) * (double)p->big_nq;
     p->big_k = (int)std::max<int64_t>(1, std::min<int64_t>(max_nvec, (int64_t)(4294967296.0 / per_vec)));

@@ -187,7 +187,10 @@ int evr_psi_read(const evr_basis_desc *basis, int nb0, int cplx, int max_wp, dou
  * memory, one mode at a time, with a work block of (7 + D) x nq_t x k
  * values made here (k vectors per pass, within 4 GiB).  A plan with
  * streamed terms is not assembled and takes no operator grids per point;
- * its diagonal is read off H on blocks of unit vectors. */
+ * its diagonal is read off H on blocks of unit vectors.  A one-term plan
+ * (direct-product basis) of at least 256 functions streams its term for
+ * H|psi> and the transforms even when it fits a workgroup, and keeps the
+ * resident kernel for the assembly and the diagonal. */
 int evr_plan_create(const evr_basis_desc *basis, const evr_op_desc *op, int max_nvec,
                     evr_plan **out);
 int evr_plan_destroy(evr_plan *p);
