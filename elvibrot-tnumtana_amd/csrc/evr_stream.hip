@@ -1,8 +1,9 @@
-// Streamed terms: a term whose basis exceeds what one workgroup holds in LDS
-// (more than 8192 functions, e.g. a direct-product basis of 3 modes x 21
-// functions, or an SG4 term of 6 modes at level 2) runs the reference's
-// per-term sequence as whole-device passes over global memory, one mode at a
-// time:
+// Streamed terms: a term beyond what one workgroup holds in LDS (more than
+// 8192 functions, e.g. a direct-product basis of 3 modes x 21 functions or an
+// SG4 term of 6 modes at level 2; a 1D slot of 32 to 64 functions; slabs
+// that do not fit), and the one term of a direct-product plan, runs the
+// reference's per-term sequence as whole-device passes over global memory,
+// one mode at a time:
 //   basis -> grid   BDP_TO_GDP_OF_SmolyakRep (BGSG4:2385-2496)
 //   grid operator   OpPsi_OF_ONEDP_FOR_SG4 (OPSG4:1448-1650): V, the type-1
 //                   F2/F1 derivative sums or the type-10 -1/2 d_i G_ji(q) d_j
