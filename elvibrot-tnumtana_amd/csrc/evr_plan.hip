@@ -541,7 +541,8 @@ int run_streamed(evr_plan *p, const std::vector<int64_t> &list, int mode, int nv
           }
           for (int j = 0; j < D; ++j) deriv(b_g, b_P + (size_t)j * nqk, j, p->d_D1.ptr, trans);
           for (int i = 0; i < D; ++i) {
-            hipLaunchKernelGGL(evr::k_stream_gcomb<T>, dim3(blocks(nqk)), dim3(256), 0, p->stream, sg, i, trans,
+            hipLaunchKernelGGL(evr::k_stream_gcomb<T>, dim3((unsigned)((nqk + 255) / 256)), dim3(256), 0, p->stream,
+                               sg, i, trans,
                                p->d_coefC.ptr, p->hasA ? p->d_coefA.ptr : nullptr, p->d_x.ptr, b_P,
                                (unsigned)nqk, k, b_a);
             if (e == hipSuccess) e = hipGetLastError();

@@ -203,7 +203,9 @@ struct StreamGrid {
 // type 10, one i: Q[q k + v] = sum_j c_ij(q) P_j[q k + v] with
 // c_ij(q) = C[i D + j] + sum_m A[(i D + j) D + m] x_m(q) (= -1/2 G_ji(q),
 // OPSG4:1590-1610); trans takes c_ji (the transposed operator).  P holds D
-// tensors of n values.
+// tensors of n values.  One workgroup per 256 consecutive values: the c_ij of
+// its grid points (at most 256 / k + 2) are formed once in LDS and shared by
+// the k vectors of each point.
 template <typename T>
 __global__ __launch_bounds__(256) void k_stream_gcomb(StreamGrid sg, int i, int trans,
                                                       const double *__restrict__ C,
@@ -212,32 +214,33 @@ __global__ __launch_bounds__(256) void k_stream_gcomb(StreamGrid sg, int i, int 
                                                       const T *__restrict__ P, unsigned n, int k,
                                                       T *__restrict__ Q)
 {
+  __shared__ T sc[258 * kMaxDev];
   const int D = sg.D;
-  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    double x[kMaxDev];
+  const unsigned e0 = blockIdx.x * 256u;
+  const unsigned q0 = e0 / (unsigned)k;
+  const unsigned elast = min(n, e0 + 256u) - 1u;
+  const int np = (int)(elast / (unsigned)k - q0) + 1;
+  for (int t = threadIdx.x; t < np * D; t += blockDim.x) {
+    const int pt = t / D, j = t - pt * D;
+    const int ij = trans ? j * D + i : i * D + j;
+    double c = C[ij];
     if (A) {
-      unsigned rem = e / (unsigned)k;
-#pragma unroll
-      for (int d = 0; d < kMaxDev; ++d)
-        if (d < D) {
-          const unsigned nx = rem / (unsigned)sg.nq[d];
-          x[d] = xpool[sg.offX[d] + (rem - nx * (unsigned)sg.nq[d])];
-          rem = nx;
-        }
-    }
-    T acc = T(0);
-    for (int j = 0; j < D; ++j) {
-      const int ij = trans ? j * D + i : i * D + j;
-      double c = C[ij];
-      if (A) {
-#pragma unroll
-        for (int m = 0; m < kMaxDev; ++m)
-          if (m < D) c += A[ij * D + m] * x[m];
+      unsigned rem = q0 + (unsigned)pt;
+      for (int d = 0; d < D; ++d) {
+        const unsigned nx = rem / (unsigned)sg.nq[d];
+        c += A[ij * D + d] * xpool[sg.offX[d] + (rem - nx * (unsigned)sg.nq[d])];
+        rem = nx;
       }
-      acc = fma(T(c), P[(size_t)j * n + e], acc);
     }
-    Q[e] = acc;
+    sc[t] = T(c);
   }
+  __syncthreads();
+  const unsigned e = e0 + threadIdx.x;
+  if (e >= n) return;
+  const T *c = sc + (int)(e / (unsigned)k - q0) * D;
+  T acc = T(0);
+  for (int j = 0; j < D; ++j) acc = fma(c[j], P[(size_t)j * n + e], acc);
+  Q[e] = acc;
 }
 
 }  // namespace evr
