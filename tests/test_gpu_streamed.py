@@ -7,8 +7,9 @@ with q-linear G, type 1 with F2 and F1, FP64 and FP32), on an SG4 basis
 whose four level-(1,1,1,0) terms have 13^3 x 5 = 10,985 functions beside
 LDS-resident terms (H|psi>, basis <-> grid, the diagonal), on the hermitian
 form, through block Davidson on a coupled harmonic direct-product operator
-whose levels are known in closed form, and through Chebyshev propagation
-(eager and replayed as captured HIP graphs) against march_cheby on the CPU."""
+whose levels are known in closed form, through Chebyshev propagation
+(eager and replayed as captured HIP graphs) against march_cheby on the CPU,
+and through the optimal-control objective and its adjoint gradient."""
 import itertools
 
 import numpy as np
@@ -186,3 +187,53 @@ def test_dp_streamed_chebyshev_graph(evr, oracle_mod):
     assert np.abs(eager - ref).max() < 1e-10
     assert np.abs(graph - ref).max() < 1e-10
     assert np.abs(np.linalg.norm(eager, axis=0) - 1.0).max() < 1e-10
+
+
+def test_dp_streamed_control_gradient(evr, oracle_mod):
+    """The optimal-control objective and adjoint gradient through a streamed
+    one-term plan (DP 3 x 8, type 10, hermitian; the dipole a type-0 plan on
+    the same basis): J and dJ/da against the oracle's exact propagation with
+    the Daleckii-Krein derivative (1e-9), dJ/dV against central differences
+    of the GPU objective at two grid points."""
+    import control
+    from evrhip import hamiltonian as Hm
+    T, N, NK = 400.0, 20, 4
+    d = oracle_mod.DP([8] * 3, [9] * 3)
+    b = evr.Basis.dp([8] * 3, [9] * 3)
+    ham = Hm.smolyak_config(1, 3)
+    q = d.grid()
+    V = ham.potential(q) / Hm.AU_TO_CM
+    G0, Ga = ham.G0 / Hm.AU_TO_CM, ham.Ga_fortran() / Hm.AU_TO_CM
+    M = d.oppsi(oracle_mod.Op(10, V=V, G0=G0, Ga=Ga), np.eye(d.nb_tot))
+    H = 0.5 * (M + M.T)
+    mu = 0.1 * q[:, 0] + 0.05 * q[:, 1] - 0.03 * q[:, 0] * q[:, 2]
+    Mu = d.oppsi(oracle_mod.Op(0, V=mu), np.eye(d.nb_tot))
+    Mu = 0.5 * (Mu + Mu.T)
+    ev, Vec = np.linalg.eigh(H)
+    psi0, phi1 = Vec[:, 0].astype(complex), Vec[:, 1].astype(complex)
+    amp = np.random.default_rng(11).uniform(-0.02, 0.02, (NK, 2))
+
+    def run(Vg, want):
+        plan = evr.Plan(b, type=10, V=Vg, G0=G0, Ga=Ga, max_nvec=8, hermitian=True)
+        mplan = evr.Plan(b, type=0, V=mu, max_nvec=8)
+        return plan.control_gradient(mplan, amp, psi0, phi1, T, N, checkpoint=8, npoly=100,
+                                     poly_tol=1e-15, want=want)
+
+    J, g, _ = run(V, ("a", "V"))
+    E = control.field_values(amp, T, N)
+    Jr, gEr, _ = control.gradient(H, Mu, psi0, phi1, E, T / N)
+    gar = control.field_basis(T, N, NK).T @ gEr
+    h = 1e-6
+    fd = []
+    for p in (100, 364):
+        Vp, Vm = V.copy(), V.copy()
+        Vp[p] += h
+        Vm[p] -= h
+        fd.append((run(Vp, ())[0] - run(Vm, ())[0]) / (2 * h))
+    fd = np.array(fd)
+    print("streamed control: J %.2e, dJ/da %.2e, dJ/dV fd %.2e" % (
+        np.abs(J - Jr).max(), np.abs(g["a"] - gar).max() / np.abs(gar).max(),
+        np.abs(fd - g["V"][[100, 364]]).max() / np.abs(g["V"]).max()))
+    assert np.abs(J - Jr).max() < 1e-12
+    assert np.abs(g["a"] - gar).max() < 1e-9 * np.abs(gar).max()
+    assert np.abs(fd - g["V"][[100, 364]]).max() < 1e-6 * np.abs(g["V"]).max()
