@@ -521,10 +521,10 @@ def auto_path_run(args):
 
 
 def streamed_run(args):
-    """H|psi> on direct-product bases from 3 x 12 functions (one term held in
-    LDS by the term kernels) to 4 x 20 (160,000 functions, streamed over
-    global memory); type 10 with q-linear G, FP64 and FP32, 5 calls timed
-    with HIP events on the plan stream."""
+    """H|psi> on direct-product bases from 3 x 12 to 4 x 20 functions (160,000),
+    each one term streamed over global memory in whole-device mode passes;
+    type 10 with q-linear G, FP64 and FP32, 5 calls timed with HIP events on
+    the plan stream."""
     import numpy as np
     import torch
     import evrhip
