@@ -823,41 +823,57 @@ hipError_t launch_items(int c, const evr::TermParams<double> &prm, int64_t nitem
   }
 }
 
-// H|psi> of a plan with G, J, s per grid point into dh (device, nb_packed x
-// nvec): B->G, the reference's grid form of every term (k_gk_deriv,
-// k_gk_combine, k_gk_out, V included), then the weighted G->B and the
-// deterministic scatter, gvec vectors at a time.
+// H|psi> of a plan with G, J, s (or F2, F1) per grid point into dh (device,
+// nb_packed x nvec): B->G, the reference's grid form of every term
+// (k_gk_deriv, k_gk_combine, k_gk_out or k_t1_out, V included), then the
+// weighted G->B and the deterministic scatter, gvec vectors at a time.  T is
+// the precision of the transforms and of the grid columns (FP32 keeps the
+// tables and the grid arithmetic in FP64; the work grids hold floats).
+template <typename T>
 int keo_grid_hpsi(evr_plan *p, int nvec, const double *dpsi, double *dh)
 {
   const int D = p->D;
   const int64_t nq = p->sum_nq, nb = p->nb_packed;
+  constexpr bool f32 = sizeof(T) == 4;
+  T *Gg = reinterpret_cast<T *>(p->d_Gg.ptr), *Pg = reinterpret_cast<T *>(p->d_Pg.ptr),
+    *Fg = reinterpret_cast<T *>(p->d_Fg.ptr), *R = reinterpret_cast<T *>(p->d_R.ptr);
+  const T *bw = f32 ? reinterpret_cast<const T *>(p->d_bw32.ptr) : reinterpret_cast<const T *>(p->d_bw64.ptr);
+  const T *keo = f32 ? reinterpret_cast<const T *>(p->d_keo32.ptr) : reinterpret_cast<const T *>(p->d_keo64.ptr);
   for (int c0 = 0; c0 < nvec; c0 += p->gvec) {
     const int k = std::min(p->gvec, nvec - c0);
-    int rc = evr::plan_b2g_device(p, k, dpsi + nb * c0, p->d_Gg.ptr);
+    int rc;
+    if constexpr (f32) {
+      T *X = reinterpret_cast<T *>(p->d_X.ptr);
+      hipLaunchKernelGGL(evr::k_pack_in<T>, dim3(blocks_for(nb * k)), dim3(256), 0, p->stream, dpsi + nb * c0,
+                         nb, k, X);
+      HIP_TRY(hipGetLastError());
+      rc = run_terms<T>(p, evr::kModeB2G, k, X, nullptr, Gg, nullptr, bw, keo, nullptr);
+    } else {
+      rc = evr::plan_b2g_device(p, k, dpsi + nb * c0, p->d_Gg.ptr);
+    }
     if (rc) return rc;
     const bool t1 = p->grid_type == 1;
-    hipLaunchKernelGGL(evr::k_gk_deriv, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+    hipLaunchKernelGGL(evr::k_gk_deriv<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
                        p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr,
-                       t1 ? nullptr : p->d_Sg.ptr, p->d_Gg.ptr, nq, k, p->d_Pg.ptr);
+                       t1 ? nullptr : p->d_Sg.ptr, Gg, nq, k, Pg);
     HIP_TRY(hipGetLastError());
     if (t1) {
-      hipLaunchKernelGGL(evr::k_t1_out, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+      hipLaunchKernelGGL(evr::k_t1_out<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
                          p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_D2.ptr, p->d_gitems.ptr,
-                         p->d_F2g.ptr, p->d_F1g.ptr, p->d_V.ptr, p->d_Gg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
+                         p->d_F2g.ptr, p->d_F1g.ptr, p->d_V.ptr, Gg, Pg, nq, k, Fg);
       HIP_TRY(hipGetLastError());
     } else {
-      hipLaunchKernelGGL(evr::k_gk_combine, dim3(blocks_for(nq)), dim3(256), 0, p->stream, D, nq, k,
-                         p->d_GG.ptr, p->d_Jg.ptr, p->d_Pg.ptr);
+      hipLaunchKernelGGL(evr::k_gk_combine<T>, dim3(blocks_for(nq)), dim3(256), 0, p->stream, D, nq, k,
+                         p->d_GG.ptr, p->d_Jg.ptr, Pg);
       HIP_TRY(hipGetLastError());
-      hipLaunchKernelGGL(evr::k_gk_out, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+      hipLaunchKernelGGL(evr::k_gk_out<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
                          p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
-                         p->d_Sg.ptr, p->d_V.ptr, p->d_Gg.ptr, p->d_Pg.ptr, nq, k, p->d_Fg.ptr);
+                         p->d_Sg.ptr, p->d_V.ptr, Gg, Pg, nq, k, Fg);
       HIP_TRY(hipGetLastError());
     }
-    rc = run_terms<double>(p, evr::kModeG2B, k, nullptr, p->d_R.ptr, nullptr, p->d_Fg.ptr, p->d_bw64.ptr,
-                           p->d_keo64.ptr, p->d_V.ptr);
+    rc = run_terms<T>(p, evr::kModeG2B, k, nullptr, R, nullptr, Fg, bw, keo, nullptr);
     if (rc) return rc;
-    HIP_TRY(launch_reduce<double>(p, p->d_R.ptr, k, dh + nb * c0));
+    HIP_TRY(launch_reduce<T>(p, R, k, dh + nb * c0));
   }
   return EVR_OK;
 }
@@ -1939,7 +1955,6 @@ static int oppsi_core(evr_plan *p, int nvec, const double *dpsi, double *dh, int
   const bool fp32 = (flags & EVR_FP32) != 0;
   const int64_t n = p->nb_packed * nvec;
   int rc;
-  if (p->keo_grid && fp32) return fail(EVR_ERR_ARG, "a plan with operator grids per point (G, J, s or F2, F1) is FP64 only");
   const bool use_asm = !(flags & EVR_MATRIX_FREE) &&
                        (fp32 ? p->d_A32.ptr != nullptr : p->d_A64.ptr != nullptr) &&
                        !((flags & EVR_AUTO_PATH) && matrix_free_cheaper(p, nvec, fp32));
@@ -1961,7 +1976,7 @@ static int oppsi_core(evr_plan *p, int nvec, const double *dpsi, double *dh, int
     }
   } else if (!fp32) {
     if (p->keo_grid) {
-      rc = keo_grid_hpsi(p, nvec, dpsi, dh);
+      rc = keo_grid_hpsi<double>(p, nvec, dpsi, dh);
       if (rc) return rc;
     } else {
       double *X = Xbuf, *R = Rbuf;
@@ -1974,6 +1989,7 @@ static int oppsi_core(evr_plan *p, int nvec, const double *dpsi, double *dh, int
       HIP_TRY(launch_reduce<double>(p, R, nvec, dh, Pbuf));
     }
   } else {
+    if (p->keo_grid) return keo_grid_hpsi<float>(p, nvec, dpsi, dh);
     rc = ensure_v32(p);
     if (rc) return rc;
     float *X = reinterpret_cast<float *>(Xbuf), *R = reinterpret_cast<float *>(Rbuf);

@@ -1559,23 +1559,27 @@ __global__ __launch_bounds__(256) void k_zero(T *__restrict__ a, int64_t n)
 // (first mode fastest).  One thread per grid point; items = (term, first point).
 
 // mode-j derivative of column `in` (optionally scaled by sc) at point p of a term
+template <typename T>
 __device__ __forceinline__ double grid_d1(const double *__restrict__ Dm, int nq, int qj, int64_t st,
-                                          const double *__restrict__ in, const double *__restrict__ sc,
+                                          const T *__restrict__ in, const double *__restrict__ sc,
                                           int64_t p0)
 {
   double acc = 0.0;
   for (int k = 0; k < nq; ++k) {
     const int64_t g = p0 + k * st;
-    acc += Dm[qj + nq * k] * (sc ? sc[g] * in[g] : in[g]);
+    acc += Dm[qj + nq * k] * (sc ? sc[g] * (double)in[g] : (double)in[g]);
   }
   return acc;
 }
 
+// The grid-form kernels keep psi_G, the derivative columns and the output in
+// T (FP64, or FP32 for the FP32 H|psi>) and do their arithmetic in FP64.
+template <typename T>
 __global__ __launch_bounds__(256) void k_gk_deriv(int D, const TermDev *__restrict__ terms,
                                                   const SlotDev *__restrict__ slots,
                                                   const double *__restrict__ D1, const int2 *__restrict__ items,
-                                                  const double *__restrict__ s, const double *__restrict__ G,
-                                                  int64_t sum_nq, int k, double *__restrict__ P)
+                                                  const double *__restrict__ s, const T *__restrict__ G,
+                                                  int64_t sum_nq, int k, T *__restrict__ P)
 {
   const int2 it = items[blockIdx.x];
   const TermDev &td = terms[it.x];
@@ -1588,38 +1592,40 @@ __global__ __launch_bounds__(256) void k_gk_deriv(int D, const TermDev *__restri
     const int qj = (int)((p / st) % nq);
     const int64_t p0 = td.off_nq + p - qj * st;
     for (int v = 0; v < k; ++v)
-      P[((int64_t)v * D + j) * sum_nq + td.off_nq + p] = grid_d1(D1 + sl.offD1, nq, qj, st, G + v * sum_nq, s, p0);
+      P[((int64_t)v * D + j) * sum_nq + td.off_nq + p] = T(grid_d1(D1 + sl.offD1, nq, qj, st, G + v * sum_nq, s, p0));
     st *= nq;
   }
 }
 
 // h_i = J sum_j G[(i D + j) sum_nq + g] P_j, in place (all P_j of the point
 // are read before any h_i is written)
+template <typename T>
 __global__ __launch_bounds__(256) void k_gk_combine(int D, int64_t sum_nq, int k, const double *__restrict__ GG,
-                                                    const double *__restrict__ J, double *__restrict__ P)
+                                                    const double *__restrict__ J, T *__restrict__ P)
 {
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < sum_nq;
        g += (int64_t)gridDim.x * blockDim.x) {
     const double jg = J[g];
     for (int v = 0; v < k; ++v) {
       double pj[kMaxDev];
-      for (int j = 0; j < D; ++j) pj[j] = P[((int64_t)v * D + j) * sum_nq + g];
+      for (int j = 0; j < D; ++j) pj[j] = (double)P[((int64_t)v * D + j) * sum_nq + g];
       for (int i = 0; i < D; ++i) {
         double h = 0.0;
         for (int j = 0; j < D; ++j) h += GG[((int64_t)i * D + j) * sum_nq + g] * pj[j];
-        P[((int64_t)v * D + i) * sum_nq + g] = jg * h;
+        P[((int64_t)v * D + i) * sum_nq + g] = T(jg * h);
       }
     }
   }
 }
 
 // out = -1/2 (sum_i d_i h_i) / (J s) + V psi_G (V on the unscaled psi_G, OPSG4:1566-1576)
+template <typename T>
 __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict__ terms,
                                                 const SlotDev *__restrict__ slots, const double *__restrict__ D1,
                                                 const int2 *__restrict__ items, const double *__restrict__ J,
                                                 const double *__restrict__ s, const double *__restrict__ V,
-                                                const double *__restrict__ G, const double *__restrict__ P,
-                                                int64_t sum_nq, int k, double *__restrict__ F)
+                                                const T *__restrict__ G, const T *__restrict__ P,
+                                                int64_t sum_nq, int k, T *__restrict__ F)
 {
   const int2 it = items[blockIdx.x];
   const TermDev &td = terms[it.x];
@@ -1638,7 +1644,7 @@ __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict
                    td.off_nq + p - qi * st);
       st *= nq;
     }
-    F[v * sum_nq + g] = scale * o + V[g] * G[v * sum_nq + g];
+    F[v * sum_nq + g] = T(scale * o + V[g] * (double)G[v * sum_nq + g]);
   }
 }
 
@@ -1650,13 +1656,14 @@ __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict
 // d = dnRGG%d1, d2 = dnRGG%d2 along one mode of the term grid; P holds
 // d_j psi_G (k_gk_deriv without a scale).  F2 is (sum_nq, D, D) Fortran,
 // F1 (sum_nq, D) or NULL.  One thread per grid point.
+template <typename T>
 __global__ __launch_bounds__(256) void k_t1_out(int D, const TermDev *__restrict__ terms,
                                                 const SlotDev *__restrict__ slots, const double *__restrict__ D1,
                                                 const double *__restrict__ D2, const int2 *__restrict__ items,
                                                 const double *__restrict__ F2, const double *__restrict__ F1,
-                                                const double *__restrict__ V, const double *__restrict__ G,
-                                                const double *__restrict__ P, int64_t sum_nq, int k,
-                                                double *__restrict__ F)
+                                                const double *__restrict__ V, const T *__restrict__ G,
+                                                const T *__restrict__ P, int64_t sum_nq, int k,
+                                                T *__restrict__ F)
 {
   const int2 it = items[blockIdx.x];
   const TermDev &td = terms[it.x];
@@ -1673,12 +1680,12 @@ __global__ __launch_bounds__(256) void k_t1_out(int D, const TermDev *__restrict
     s *= nq;
   }
   for (int v = 0; v < k; ++v) {
-    const double *Gv = G + (int64_t)v * sum_nq;
-    double o = V[g] * Gv[g];
+    const T *Gv = G + (int64_t)v * sum_nq;
+    double o = V[g] * (double)Gv[g];
     for (int i = 0; i < D; ++i) {
       const SlotDev &sl = slots[td.slot[i]];
       const int64_t p0 = td.off_nq + p - qd[i] * st[i];
-      if (F1) o += F1[g + sum_nq * i] * P[((int64_t)v * D + i) * sum_nq + g];
+      if (F1) o += F1[g + sum_nq * i] * (double)P[((int64_t)v * D + i) * sum_nq + g];
       const double fii = F2[g + sum_nq * (i + (int64_t)D * i)];
       if (fii != 0.0) o += fii * grid_d1(D2 + sl.offD1, sl.nq, qd[i], st[i], Gv, nullptr, p0);
       for (int j = 0; j < D; ++j) {
@@ -1688,7 +1695,7 @@ __global__ __launch_bounds__(256) void k_t1_out(int D, const TermDev *__restrict
           o += fij * grid_d1(D1 + sl.offD1, sl.nq, qd[i], st[i], P + ((int64_t)v * D + j) * sum_nq, nullptr, p0);
       }
     }
-    F[(int64_t)v * sum_nq + g] = o;
+    F[(int64_t)v * sum_nq + g] = T(o);
   }
 }
 

@@ -203,8 +203,10 @@ int evr_plan_set_potential(evr_plan *p, const double *V);
  * (= 1).  From then on the kinetic part is the reference's grid form
  * (OPSG4:1566-1632): -1/2 (1/(J s)) sum_i d_i [J sum_j G(:, j, i) d_j (s psi)]
  * with d = dnRGG%d1, on every term grid; G0/Ga are no longer used.  SG4 bases
- * with D <= 12, hermitian = 0, FP64 and matrix-free only (EVR_ERR_ARG for an
- * assembly or FP32); the plan diagonal then holds the V part only.  Device
+ * with D <= 12, hermitian = 0, matrix-free only (EVR_ERR_ARG for an
+ * assembly); EVR_FP32 runs the transforms and keeps the grid columns in FP32
+ * (tables and grid arithmetic FP64); the plan diagonal then holds the V part
+ * only.  Device
  * memory: (D^2 + 2) sum_nq tables and (D + 2) sum_nq work doubles per vector
  * processed at once. */
 int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const double *sqRhoOVERJac);
@@ -218,8 +220,9 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
  * coefficient of d_i d_j (the reference's one term per pair i <= j: put its
  * value in one triangle and zero in the other); F1 is (sum_nq, D) or NULL.
  * Replaces the plan's constant F2/F1.  SG4 bases with D <= 12, a type-1 plan
- * with hermitian = 0, FP64 and matrix-free only (EVR_ERR_ARG for an assembly
- * or FP32); the plan diagonal then comes from H on blocks of unit vectors. */
+ * with hermitian = 0, matrix-free only (EVR_ERR_ARG for an assembly);
+ * EVR_FP32 as for evr_plan_set_keo_grid; the plan diagonal then comes from H
+ * on blocks of unit vectors. */
 int evr_plan_set_type1_grid(evr_plan *p, const double *F2, const double *F1);
 /* Generate V on the device: V(q) = sum_i v2_i q_i^2 + sum_{i<=j<=k} c_ijk
  * q_i q_j q_k + sum_{i<j} d_ij q_i^2 q_j^2 + sum_i e_i q_i^4, with c given as a

@@ -1,7 +1,7 @@
 """Type 10 with G, J and s = sqrt(rho/J) tabulated per grid point
 (evr_plan_set_keo_grid: the reference's get_OpGrid_type10 tables and its grid
-form OPSG4:1566-1632) against the oracle, and against the polynomial G path
-when the table is G0 + Ga q."""
+form OPSG4:1566-1632) against the oracle, FP64 (1e-12) and FP32 (2e-5), and
+against the polynomial G path when the table is G0 + Ga q."""
 import numpy as np
 import pytest
 
@@ -26,8 +26,10 @@ def test_keo_grid_vs_oracle(evr, oracle_mod, D, L, nvec):
     X = evr.hamiltonian.test_vectors(s.nb_packed, nvec, D)
     ref = s.oppsi(oracle_mod.Op(10, V=V, Jac=J, s=sq, Gq=Gq), X)
     err = relerr(plan.TabOpPsi(X), ref)
-    print("G, J, s per point D=%d L=%d: rel err %.3e" % (D, L, err))
+    err32 = relerr(plan.TabOpPsi(X, flags=evr.FP32), ref)
+    print("G, J, s per point D=%d L=%d: rel err %.3e, FP32 %.3e" % (D, L, err, err32))
     assert err < 1e-12
+    assert err32 < 2e-5
 
 
 def test_keo_grid_equals_polynomial_g(evr):
@@ -59,8 +61,6 @@ def test_keo_grid_refusals(evr):
     p.set_keo_grid(Gq, J, sq)
     with pytest.raises(RuntimeError, match="not assembled"):
         p.assemble()
-    with pytest.raises(RuntimeError, match="FP64 only"):
-        p.TabOpPsi(np.ones((b.nb_packed, 1)), flags=evr.FP32)
     ph = evr.Plan(b, type=10, V=np.zeros(len(q)), G0=np.eye(D), max_nvec=2, hermitian=True)
     with pytest.raises(RuntimeError, match="symmetric form"):
         ph.set_keo_grid(Gq)
