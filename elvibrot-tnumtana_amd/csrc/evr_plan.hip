@@ -131,6 +131,9 @@ struct evr_plan {
   int64_t n_gitems = 0;
   DevBuf<double> d_GG, d_Jg, d_Sg, d_Gg, d_Pg, d_Fg;
   DevBuf<int2> d_gitems;
+  // hermitian grid form: quadrature weight per point, w / (J s), and the term
+  // rows of the transposed sequence
+  DevBuf<double> d_Wq, d_As, d_R2;
   // streamed terms (a basis beyond one workgroup's LDS, evr_stream.hip): their
   // ids, the largest term grid among them, the vectors per pass and the work
   // block of (7 + D) tensors of big_nq x big_k words
@@ -197,6 +200,7 @@ struct evr_plan {
     d_A64.release(); d_A32.release(); d_asm_items.release(); d_D1.release(); d_D2.release();
     d_GG.release(); d_Jg.release(); d_Sg.release(); d_Gg.release(); d_Pg.release(); d_Fg.release();
     d_F2g.release(); d_F1g.release(); d_gitems.release(); d_dp_perm.release(); d_big.release();
+    d_Wq.release(); d_As.release(); d_R2.release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -855,7 +859,7 @@ int keo_grid_hpsi(evr_plan *p, int nvec, const double *dpsi, double *dh)
     const bool t1 = p->grid_type == 1;
     hipLaunchKernelGGL(evr::k_gk_deriv<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
                        p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr,
-                       t1 ? nullptr : p->d_Sg.ptr, Gg, nq, k, Pg);
+                       t1 ? nullptr : p->d_Sg.ptr, Gg, nq, k, Pg, 0);
     HIP_TRY(hipGetLastError());
     if (t1) {
       hipLaunchKernelGGL(evr::k_t1_out<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
@@ -864,15 +868,38 @@ int keo_grid_hpsi(evr_plan *p, int nvec, const double *dpsi, double *dh)
       HIP_TRY(hipGetLastError());
     } else {
       hipLaunchKernelGGL(evr::k_gk_combine<T>, dim3(blocks_for(nq)), dim3(256), 0, p->stream, D, nq, k,
-                         p->d_GG.ptr, p->d_Jg.ptr, Pg);
+                         p->d_GG.ptr, p->d_Jg.ptr, Pg, 0);
       HIP_TRY(hipGetLastError());
       hipLaunchKernelGGL(evr::k_gk_out<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
                          p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
-                         p->d_Sg.ptr, p->d_V.ptr, Gg, Pg, nq, k, Fg);
+                         p->d_Sg.ptr, p->d_V.ptr, Gg, Pg, nq, k, Fg, (const double *)nullptr);
       HIP_TRY(hipGetLastError());
     }
     rc = run_terms<T>(p, evr::kModeG2B, k, nullptr, R, nullptr, Fg, bw, keo, nullptr);
     if (rc) return rc;
+    if (p->hermitian && !t1) {
+      // H^T psi = W diag(1/w) Op^T diag(w) B psi (W = B^T diag(w) on every
+      // term grid), so the transposed sequence reuses the B->G columns in Gg
+      // and the weighted G->B: d_i^T on (w / (J s)) psi_G, J sum_i G_ji,
+      // -1/2 (s / w) sum_j d_j^T; the rows of H and H^T are then averaged
+      T *R2 = reinterpret_cast<T *>(p->d_R2.ptr);
+      hipLaunchKernelGGL(evr::k_gk_deriv<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                         p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_As.ptr, Gg, nq, k,
+                         Pg, 1);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(evr::k_gk_combine<T>, dim3(blocks_for(nq)), dim3(256), 0, p->stream, D, nq, k,
+                         p->d_GG.ptr, p->d_Jg.ptr, Pg, 1);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(evr::k_gk_out<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                         p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
+                         p->d_Sg.ptr, p->d_V.ptr, Gg, Pg, nq, k, Fg, (const double *)p->d_Wq.ptr);
+      HIP_TRY(hipGetLastError());
+      rc = run_terms<T>(p, evr::kModeG2B, k, nullptr, R2, nullptr, Fg, bw, keo, nullptr);
+      if (rc) return rc;
+      const int64_t nr = p->nrows * k;
+      hipLaunchKernelGGL(evr::k_half_sum<T>, dim3(blocks_for(nr)), dim3(256), 0, p->stream, R, (const T *)R2, nr);
+      HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(launch_reduce<T>(p, R, k, dh + nb * c0));
   }
   return EVR_OK;
@@ -1599,7 +1626,6 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
   if (p->type != 10) return fail(EVR_ERR_ARG, "G per grid point is a type-10 operator");
   if (p->D != p->D_full) return fail(EVR_ERR_ARG, "G per grid point needs D <= 12");
   if (p->dp_order) return fail(EVR_ERR_ARG, "G per grid point is for SG4 bases");
-  if (p->hermitian) return fail(EVR_ERR_ARG, "G per grid point has no symmetric form (hermitian = 0)");
   if (!p->big.empty()) return fail(EVR_ERR_ARG, "G per grid point needs terms of at most 8192 functions");
   const int D = p->D;
   const int64_t nq = p->sum_nq;
@@ -1611,6 +1637,32 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
   if (!Jac || !sqRhoOVERJac) one.assign(nq, 1.0);
   HIP_TRY(p->d_Jg.upload(Jac ? Jac : one.data(), nq, p->stream));
   HIP_TRY(p->d_Sg.upload(sqRhoOVERJac ? sqRhoOVERJac : one.data(), nq, p->stream));
+  if (p->hermitian) {
+    // quadrature weight of every term-grid point (W[0, q] / B[q, 0] per 1D
+    // rule, W = B^T diag(w), first mode fastest) and w / (J s)
+    std::vector<double> bw(p->d_bw64.n);
+    HIP_TRY(hipMemcpy(bw.data(), p->d_bw64.ptr, sizeof(double) * bw.size(), hipMemcpyDeviceToHost));
+    std::vector<double> wq(nq), as(nq);
+    for (int64_t t = 0; t < p->nterms; ++t) {
+      const evr::TermDev &td = p->terms[t];
+      for (int64_t q = 0; q < td.nqt; ++q) {
+        int64_t rem = q;
+        double w = 1.0;
+        for (int d = 0; d < D; ++d) {
+          const evr::SlotDev &sd = p->slots[td.slot[d]];
+          const int64_t iq = rem % sd.nq;
+          rem /= sd.nq;
+          w *= bw[sd.offW + (int64_t)sd.nb * iq] / bw[sd.offB + iq];
+        }
+        const int64_t g = td.off_nq + q;
+        wq[g] = w;
+        as[g] = w / ((Jac ? Jac[g] : 1.0) * (sqRhoOVERJac ? sqRhoOVERJac[g] : 1.0));
+      }
+    }
+    HIP_TRY(p->d_Wq.upload(wq.data(), nq, p->stream));
+    HIP_TRY(p->d_As.upload(as.data(), nq, p->stream));
+    if (!p->d_R2.ptr) HIP_TRY(p->d_R2.alloc((size_t)p->nrows * std::max(1, p->max_nvec)));
+  }
   int rc = grid_form_buffers(p);
   if (rc) return rc;
   // the term kernels keep the B->G and G->B transforms; the operator itself

@@ -1572,6 +1572,20 @@ __device__ __forceinline__ double grid_d1(const double *__restrict__ Dm, int nq,
   return acc;
 }
 
+// the same with the transposed 1D matrix (row qj of Dm^T)
+template <typename T>
+__device__ __forceinline__ double grid_d1t(const double *__restrict__ Dm, int nq, int qj, int64_t st,
+                                           const T *__restrict__ in, const double *__restrict__ sc,
+                                           int64_t p0)
+{
+  double acc = 0.0;
+  for (int k = 0; k < nq; ++k) {
+    const int64_t g = p0 + k * st;
+    acc += Dm[k + nq * qj] * (sc ? sc[g] * (double)in[g] : (double)in[g]);
+  }
+  return acc;
+}
+
 // The grid-form kernels keep psi_G, the derivative columns and the output in
 // T (FP64, or FP32 for the FP32 H|psi>) and do their arithmetic in FP64.
 template <typename T>
@@ -1579,7 +1593,7 @@ __global__ __launch_bounds__(256) void k_gk_deriv(int D, const TermDev *__restri
                                                   const SlotDev *__restrict__ slots,
                                                   const double *__restrict__ D1, const int2 *__restrict__ items,
                                                   const double *__restrict__ s, const T *__restrict__ G,
-                                                  int64_t sum_nq, int k, T *__restrict__ P)
+                                                  int64_t sum_nq, int k, T *__restrict__ P, int trans)
 {
   const int2 it = items[blockIdx.x];
   const TermDev &td = terms[it.x];
@@ -1592,16 +1606,18 @@ __global__ __launch_bounds__(256) void k_gk_deriv(int D, const TermDev *__restri
     const int qj = (int)((p / st) % nq);
     const int64_t p0 = td.off_nq + p - qj * st;
     for (int v = 0; v < k; ++v)
-      P[((int64_t)v * D + j) * sum_nq + td.off_nq + p] = T(grid_d1(D1 + sl.offD1, nq, qj, st, G + v * sum_nq, s, p0));
+      P[((int64_t)v * D + j) * sum_nq + td.off_nq + p] =
+          T(trans ? grid_d1t(D1 + sl.offD1, nq, qj, st, G + v * sum_nq, s, p0)
+                  : grid_d1(D1 + sl.offD1, nq, qj, st, G + v * sum_nq, s, p0));
     st *= nq;
   }
 }
 
 // h_i = J sum_j G[(i D + j) sum_nq + g] P_j, in place (all P_j of the point
-// are read before any h_i is written)
+// are read before any h_i is written); trans takes G[(j D + i) sum_nq + g]
 template <typename T>
 __global__ __launch_bounds__(256) void k_gk_combine(int D, int64_t sum_nq, int k, const double *__restrict__ GG,
-                                                    const double *__restrict__ J, T *__restrict__ P)
+                                                    const double *__restrict__ J, T *__restrict__ P, int trans)
 {
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < sum_nq;
        g += (int64_t)gridDim.x * blockDim.x) {
@@ -1611,28 +1627,32 @@ __global__ __launch_bounds__(256) void k_gk_combine(int D, int64_t sum_nq, int k
       for (int j = 0; j < D; ++j) pj[j] = (double)P[((int64_t)v * D + j) * sum_nq + g];
       for (int i = 0; i < D; ++i) {
         double h = 0.0;
-        for (int j = 0; j < D; ++j) h += GG[((int64_t)i * D + j) * sum_nq + g] * pj[j];
+        for (int j = 0; j < D; ++j) h += GG[((int64_t)(trans ? j * D + i : i * D + j)) * sum_nq + g] * pj[j];
         P[((int64_t)v * D + i) * sum_nq + g] = T(jg * h);
       }
     }
   }
 }
 
-// out = -1/2 (sum_i d_i h_i) / (J s) + V psi_G (V on the unscaled psi_G, OPSG4:1566-1576)
+// out = -1/2 (sum_i d_i h_i) / (J s) + V psi_G (V on the unscaled psi_G,
+// OPSG4:1566-1576).  With wq (the quadrature weight of every point) the
+// transposed form of the weighted operator: out = -1/2 (s / wq) sum_i d_i^T
+// h_i + V psi_G.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict__ terms,
                                                 const SlotDev *__restrict__ slots, const double *__restrict__ D1,
                                                 const int2 *__restrict__ items, const double *__restrict__ J,
                                                 const double *__restrict__ s, const double *__restrict__ V,
                                                 const T *__restrict__ G, const T *__restrict__ P,
-                                                int64_t sum_nq, int k, T *__restrict__ F)
+                                                int64_t sum_nq, int k, T *__restrict__ F,
+                                                const double *__restrict__ wq)
 {
   const int2 it = items[blockIdx.x];
   const TermDev &td = terms[it.x];
   const int p = it.y + threadIdx.x;
   if (p >= td.nqt) return;
   const int64_t g = td.off_nq + p;
-  const double scale = -0.5 / (J[g] * s[g]);
+  const double scale = wq ? -0.5 * s[g] / wq[g] : -0.5 / (J[g] * s[g]);
   for (int v = 0; v < k; ++v) {
     double o = 0.0;
     int64_t st = 1;
@@ -1640,12 +1660,21 @@ __global__ __launch_bounds__(256) void k_gk_out(int D, const TermDev *__restrict
       const SlotDev &sl = slots[td.slot[i]];
       const int nq = sl.nq;
       const int qi = (int)((p / st) % nq);
-      o += grid_d1(D1 + sl.offD1, nq, qi, st, P + ((int64_t)v * D + i) * sum_nq, nullptr,
-                   td.off_nq + p - qi * st);
+      const T *Pi = P + ((int64_t)v * D + i) * sum_nq;
+      o += wq ? grid_d1t(D1 + sl.offD1, nq, qi, st, Pi, (const double *)nullptr, td.off_nq + p - qi * st)
+              : grid_d1(D1 + sl.offD1, nq, qi, st, Pi, (const double *)nullptr, td.off_nq + p - qi * st);
       st *= nq;
     }
     F[v * sum_nq + g] = T(scale * o + V[g] * (double)G[v * sum_nq + g]);
   }
+}
+
+// a[i] = (a[i] + b[i]) / 2: the symmetric part from the rows of H and H^T
+template <typename T>
+__global__ __launch_bounds__(256) void k_half_sum(T *__restrict__ a, const T *__restrict__ b, int64_t n)
+{
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = T(0.5) * (a[i] + b[i]);
 }
 
 // ------------------------------------------- type 1 with F2, F1 per point --

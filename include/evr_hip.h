@@ -203,10 +203,12 @@ int evr_plan_set_potential(evr_plan *p, const double *V);
  * (= 1).  From then on the kinetic part is the reference's grid form
  * (OPSG4:1566-1632): -1/2 (1/(J s)) sum_i d_i [J sum_j G(:, j, i) d_j (s psi)]
  * with d = dnRGG%d1, on every term grid; G0/Ga are no longer used.  SG4 bases
- * with D <= 12, hermitian = 0, matrix-free only (EVR_ERR_ARG for an
- * assembly); EVR_FP32 runs the transforms and keeps the grid columns in FP32
- * (tables and grid arithmetic FP64); the plan diagonal then holds the V part
- * only.  Device
+ * with D <= 12, matrix-free only (EVR_ERR_ARG for an assembly); EVR_FP32
+ * runs the transforms and keeps the grid columns in FP32 (tables and grid
+ * arithmetic FP64).  A hermitian plan applies (H + H^T) / 2 of this grid
+ * form, the transpose as W diag(1/w) Op^T diag(w) B with W = B^T diag(w) on
+ * every term grid.  The plan diagonal then comes from H on blocks of unit
+ * vectors.  Device
  * memory: (D^2 + 2) sum_nq tables and (D + 2) sum_nq work doubles per vector
  * processed at once. */
 int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const double *sqRhoOVERJac);

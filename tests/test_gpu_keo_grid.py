@@ -1,7 +1,8 @@
 """Type 10 with G, J and s = sqrt(rho/J) tabulated per grid point
 (evr_plan_set_keo_grid: the reference's get_OpGrid_type10 tables and its grid
-form OPSG4:1566-1632) against the oracle, FP64 (1e-12) and FP32 (2e-5), and
-against the polynomial G path when the table is G0 + Ga q."""
+form OPSG4:1566-1632) against the oracle, FP64 (1e-12) and FP32 (2e-5), its
+hermitian form (H + H^T) / 2, and against the polynomial G path when the
+table is G0 + Ga q."""
 import numpy as np
 import pytest
 
@@ -61,9 +62,6 @@ def test_keo_grid_refusals(evr):
     p.set_keo_grid(Gq, J, sq)
     with pytest.raises(RuntimeError, match="not assembled"):
         p.assemble()
-    ph = evr.Plan(b, type=10, V=np.zeros(len(q)), G0=np.eye(D), max_nvec=2, hermitian=True)
-    with pytest.raises(RuntimeError, match="symmetric form"):
-        ph.set_keo_grid(Gq)
     p1 = evr.Plan(b, type=1, V=np.zeros(len(q)), F2=-0.5 * np.eye(D), max_nvec=2)
     with pytest.raises(RuntimeError, match="type-10"):
         p1.set_keo_grid(Gq)
@@ -86,3 +84,30 @@ def test_keo_grid_diagonal(evr, oracle_mod):
     err = relerr(plan.diagonal(), np.diag(Hd))
     print("G per point: diagonal rel err %.3e" % err)
     assert err < 1e-12
+
+
+def test_keo_grid_hermitian(evr, oracle_mod):
+    """hermitian = 1 with G, J, s per point: (H + H^T) / 2 of the reference's
+    grid form, through the transposed sequence W diag(1/w) Op^T diag(w) B,
+    against the oracle's dense operator (FP64 1e-12, FP32 2e-5), and its
+    diagonal."""
+    D, L = 3, 4
+    s = oracle_mod.SG4(D, L)
+    b = evr.Basis.sg4(D, L)
+    q = s.grid()
+    Gq, J, sq = tabulated_keo(q, 17)
+    V = 0.5 * (q ** 2).sum(1) + 0.05 * q[:, 0] ** 3
+    n = s.nb_packed
+    M = s.oppsi(oracle_mod.Op(10, V=V, Jac=J, s=sq, Gq=Gq), np.eye(n))
+    Hs = 0.5 * (M + M.T)
+    plan = evr.Plan(b, type=10, V=V, G0=np.eye(D), max_nvec=6, hermitian=True)
+    plan.set_keo_grid(Gq, J, sq)
+    X = evr.hamiltonian.test_vectors(n, 6, 5)
+    ref = Hs @ X
+    err = relerr(plan.TabOpPsi(X), ref)
+    err32 = relerr(plan.TabOpPsi(X, flags=evr.FP32), ref)
+    errd = relerr(plan.diagonal(), np.diag(Hs))
+    asym = np.abs(M - M.T).max() / np.abs(M).max()
+    print("hermitian G per point: %.3e, FP32 %.3e, diagonal %.3e (asymmetry of H %.2e)" % (err, err32, errd, asym))
+    assert asym > 1e-6
+    assert err < 1e-12 and errd < 1e-12 and err32 < 2e-5
