@@ -133,7 +133,7 @@ struct evr_plan {
   DevBuf<int2> d_gitems;
   // hermitian grid form: quadrature weight per point, w / (J s), and the term
   // rows of the transposed sequence
-  DevBuf<double> d_Wq, d_As, d_R2;
+  DevBuf<double> d_Wq, d_As, d_R2, d_F2w, d_F1w;
   // streamed terms (a basis beyond one workgroup's LDS, evr_stream.hip): their
   // ids, the largest term grid among them, the vectors per pass and the work
   // block of (7 + D) tensors of big_nq x big_k words
@@ -200,7 +200,7 @@ struct evr_plan {
     d_A64.release(); d_A32.release(); d_asm_items.release(); d_D1.release(); d_D2.release();
     d_GG.release(); d_Jg.release(); d_Sg.release(); d_Gg.release(); d_Pg.release(); d_Fg.release();
     d_F2g.release(); d_F1g.release(); d_gitems.release(); d_dp_perm.release(); d_big.release();
-    d_Wq.release(); d_As.release(); d_R2.release();
+    d_Wq.release(); d_As.release(); d_R2.release(); d_F2w.release(); d_F1w.release();
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -893,6 +893,23 @@ int keo_grid_hpsi(evr_plan *p, int nvec, const double *dpsi, double *dh)
       hipLaunchKernelGGL(evr::k_gk_out<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
                          p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_Jg.ptr,
                          p->d_Sg.ptr, p->d_V.ptr, Gg, Pg, nq, k, Fg, (const double *)p->d_Wq.ptr);
+      HIP_TRY(hipGetLastError());
+      rc = run_terms<T>(p, evr::kModeG2B, k, nullptr, R2, nullptr, Fg, bw, keo, nullptr);
+      if (rc) return rc;
+      const int64_t nr = p->nrows * k;
+      hipLaunchKernelGGL(evr::k_half_sum<T>, dim3(blocks_for(nr)), dim3(256), 0, p->stream, R, (const T *)R2, nr);
+      HIP_TRY(hipGetLastError());
+    } else if (p->hermitian) {
+      // type 1: the weighted transpose W diag(1/w) Op^T diag(w) B of the grid
+      // operator (k_t1t_u, k_t1t_out) on the same B->G columns
+      T *R2 = reinterpret_cast<T *>(p->d_R2.ptr);
+      hipLaunchKernelGGL(evr::k_t1t_u<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                         p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_gitems.ptr, p->d_F2w.ptr, Gg, nq, k,
+                         Pg);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(evr::k_t1t_out<T>, dim3((unsigned)p->n_gitems), dim3(256), 0, p->stream, D,
+                         p->d_terms.ptr, p->d_slots.ptr, p->d_D1.ptr, p->d_D2.ptr, p->d_gitems.ptr, p->d_F2w.ptr,
+                         p->d_F1w.ptr, p->d_Wq.ptr, p->d_V.ptr, Gg, Pg, nq, k, Fg);
       HIP_TRY(hipGetLastError());
       rc = run_terms<T>(p, evr::kModeG2B, k, nullptr, R2, nullptr, Fg, bw, keo, nullptr);
       if (rc) return rc;
@@ -1620,6 +1637,30 @@ int evr_plan_gen_potential_poly(evr_plan *p, const double *v2, const double *c3,
   return EVR_OK;
 }
 
+// quadrature weight of every term-grid point: W[0, q] / B[q, 0] per 1D rule
+// (W = B^T diag(w)), multiplied over the modes of the term (first mode fastest)
+static int quad_weights(evr_plan *p, std::vector<double> &wq)
+{
+  std::vector<double> bw(p->d_bw64.n);
+  HIP_TRY(hipMemcpy(bw.data(), p->d_bw64.ptr, sizeof(double) * bw.size(), hipMemcpyDeviceToHost));
+  wq.assign(p->sum_nq, 0.0);
+  for (int64_t t = 0; t < p->nterms; ++t) {
+    const evr::TermDev &td = p->terms[t];
+    for (int64_t q = 0; q < td.nqt; ++q) {
+      int64_t rem = q;
+      double w = 1.0;
+      for (int d = 0; d < p->D; ++d) {
+        const evr::SlotDev &sd = p->slots[td.slot[d]];
+        const int64_t iq = rem % sd.nq;
+        rem /= sd.nq;
+        w *= bw[sd.offW + (int64_t)sd.nb * iq] / bw[sd.offB + iq];
+      }
+      wq[td.off_nq + q] = w;
+    }
+  }
+  return EVR_OK;
+}
+
 int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const double *sqRhoOVERJac)
 {
   if (!p || !G) return fail(EVR_ERR_ARG, "NULL argument");
@@ -1638,27 +1679,12 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
   HIP_TRY(p->d_Jg.upload(Jac ? Jac : one.data(), nq, p->stream));
   HIP_TRY(p->d_Sg.upload(sqRhoOVERJac ? sqRhoOVERJac : one.data(), nq, p->stream));
   if (p->hermitian) {
-    // quadrature weight of every term-grid point (W[0, q] / B[q, 0] per 1D
-    // rule, W = B^T diag(w), first mode fastest) and w / (J s)
-    std::vector<double> bw(p->d_bw64.n);
-    HIP_TRY(hipMemcpy(bw.data(), p->d_bw64.ptr, sizeof(double) * bw.size(), hipMemcpyDeviceToHost));
-    std::vector<double> wq(nq), as(nq);
-    for (int64_t t = 0; t < p->nterms; ++t) {
-      const evr::TermDev &td = p->terms[t];
-      for (int64_t q = 0; q < td.nqt; ++q) {
-        int64_t rem = q;
-        double w = 1.0;
-        for (int d = 0; d < D; ++d) {
-          const evr::SlotDev &sd = p->slots[td.slot[d]];
-          const int64_t iq = rem % sd.nq;
-          rem /= sd.nq;
-          w *= bw[sd.offW + (int64_t)sd.nb * iq] / bw[sd.offB + iq];
-        }
-        const int64_t g = td.off_nq + q;
-        wq[g] = w;
-        as[g] = w / ((Jac ? Jac[g] : 1.0) * (sqRhoOVERJac ? sqRhoOVERJac[g] : 1.0));
-      }
-    }
+    // w / (J s) per point for the transposed sequence
+    std::vector<double> wq, as(nq);
+    int rcw = quad_weights(p, wq);
+    if (rcw) return rcw;
+    for (int64_t g = 0; g < nq; ++g)
+      as[g] = wq[g] / ((Jac ? Jac[g] : 1.0) * (sqRhoOVERJac ? sqRhoOVERJac[g] : 1.0));
     HIP_TRY(p->d_Wq.upload(wq.data(), nq, p->stream));
     HIP_TRY(p->d_As.upload(as.data(), nq, p->stream));
     if (!p->d_R2.ptr) HIP_TRY(p->d_R2.alloc((size_t)p->nrows * std::max(1, p->max_nvec)));
@@ -1682,7 +1708,6 @@ int evr_plan_set_type1_grid(evr_plan *p, const double *F2, const double *F1)
   if (p->type != 1) return fail(EVR_ERR_ARG, "operator grids F2, F1 per point belong to a type-1 plan");
   if (p->D != p->D_full) return fail(EVR_ERR_ARG, "operator grids per point need D <= 12");
   if (p->dp_order) return fail(EVR_ERR_ARG, "operator grids per point are for SG4 bases");
-  if (p->hermitian) return fail(EVR_ERR_ARG, "operator grids per point have no symmetric form (hermitian = 0)");
   if (!p->big.empty()) return fail(EVR_ERR_ARG, "operator grids per point need terms of at most 8192 functions");
   const int D = p->D;
   const int64_t nq = p->sum_nq;
@@ -1694,6 +1719,26 @@ int evr_plan_set_type1_grid(evr_plan *p, const double *F2, const double *F1)
     HIP_TRY(p->d_F1g.upload(F1, (size_t)D * nq, p->stream));
   } else {
     p->d_F1g.release();
+  }
+  if (p->hermitian) {
+    // the weighted tables w F2, w F1 of the transposed sequence
+    std::vector<double> wq;
+    int rcw = quad_weights(p, wq);
+    if (rcw) return rcw;
+    std::vector<double> f2w((size_t)D * D * nq), f1w(F1 ? (size_t)D * nq : 0);
+    for (size_t c = 0; c < (size_t)D * D; ++c)
+      for (int64_t g = 0; g < nq; ++g) f2w[c * nq + g] = wq[g] * F2[c * nq + g];
+    for (size_t c = 0; c < f1w.size() / std::max<int64_t>(nq, 1); ++c)
+      for (int64_t g = 0; g < nq; ++g) f1w[c * nq + g] = wq[g] * F1[c * nq + g];
+    HIP_TRY(p->d_Wq.upload(wq.data(), nq, p->stream));
+    HIP_TRY(p->d_F2w.upload(f2w.data(), f2w.size(), p->stream));
+    if (F1) {
+      HIP_TRY(p->d_F1w.upload(f1w.data(), f1w.size(), p->stream));
+    } else {
+      p->d_F1w.release();
+    }
+    if (!p->d_R2.ptr) HIP_TRY(p->d_R2.alloc((size_t)p->nrows * std::max(1, p->max_nvec)));
+    HIP_TRY(hipStreamSynchronize(p->stream));  // the host tables above are freed on return
   }
   int rc = grid_form_buffers(p);
   if (rc) return rc;

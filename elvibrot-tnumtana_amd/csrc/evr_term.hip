@@ -1728,6 +1728,85 @@ __global__ __launch_bounds__(256) void k_t1_out(int D, const TermDev *__restrict
   }
 }
 
+// The transposed type-1 grid form for the hermitian plan: with y = w psi_G
+// (w the quadrature weight of the point) the weighted transpose is
+//   out = V psi_G + (1/w) [sum_i d_i^T (F1_i y) + d2_i^T (F2_ii y)
+//                          + sum_j d_j^T U_j],  U_j = sum_{i != j} d_i^T (F2_ij y),
+// with F1w = w F1 and F2w = w F2 tabulated at set-up.  k_t1t_u writes U_j
+// into P, k_t1t_out the output.  One thread per grid point.
+template <typename T>
+__global__ __launch_bounds__(256) void k_t1t_u(int D, const TermDev *__restrict__ terms,
+                                               const SlotDev *__restrict__ slots, const double *__restrict__ D1,
+                                               const int2 *__restrict__ items, const double *__restrict__ F2w,
+                                               const T *__restrict__ G, int64_t sum_nq, int k, T *__restrict__ P)
+{
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int p = it.y + threadIdx.x;
+  if (p >= td.nqt) return;
+  const int64_t g = td.off_nq + p;
+  int qd[kMaxDev];
+  int64_t st[kMaxDev];
+  int64_t s = 1;
+  for (int d = 0; d < D; ++d) {
+    const int nq = slots[td.slot[d]].nq;
+    qd[d] = (int)((p / s) % nq);
+    st[d] = s;
+    s *= nq;
+  }
+  for (int v = 0; v < k; ++v) {
+    const T *Gv = G + (int64_t)v * sum_nq;
+    for (int j = 0; j < D; ++j) {
+      double u = 0.0;
+      for (int i = 0; i < D; ++i) {
+        if (i == j) continue;
+        const SlotDev &sl = slots[td.slot[i]];
+        u += grid_d1t(D1 + sl.offD1, sl.nq, qd[i], st[i], Gv, F2w + sum_nq * (i + (int64_t)D * j),
+                      td.off_nq + p - qd[i] * st[i]);
+      }
+      P[((int64_t)v * D + j) * sum_nq + g] = T(u);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_t1t_out(int D, const TermDev *__restrict__ terms,
+                                                 const SlotDev *__restrict__ slots, const double *__restrict__ D1,
+                                                 const double *__restrict__ D2, const int2 *__restrict__ items,
+                                                 const double *__restrict__ F2w, const double *__restrict__ F1w,
+                                                 const double *__restrict__ wq, const double *__restrict__ V,
+                                                 const T *__restrict__ G, const T *__restrict__ P, int64_t sum_nq,
+                                                 int k, T *__restrict__ F)
+{
+  const int2 it = items[blockIdx.x];
+  const TermDev &td = terms[it.x];
+  const int p = it.y + threadIdx.x;
+  if (p >= td.nqt) return;
+  const int64_t g = td.off_nq + p;
+  int qd[kMaxDev];
+  int64_t st[kMaxDev];
+  int64_t s = 1;
+  for (int d = 0; d < D; ++d) {
+    const int nq = slots[td.slot[d]].nq;
+    qd[d] = (int)((p / s) % nq);
+    st[d] = s;
+    s *= nq;
+  }
+  for (int v = 0; v < k; ++v) {
+    const T *Gv = G + (int64_t)v * sum_nq;
+    double o = 0.0;
+    for (int i = 0; i < D; ++i) {
+      const SlotDev &sl = slots[td.slot[i]];
+      const int64_t p0 = td.off_nq + p - qd[i] * st[i];
+      if (F1w) o += grid_d1t(D1 + sl.offD1, sl.nq, qd[i], st[i], Gv, F1w + sum_nq * i, p0);
+      o += grid_d1t(D2 + sl.offD1, sl.nq, qd[i], st[i], Gv, F2w + sum_nq * (i + (int64_t)D * i), p0);
+      o += grid_d1t(D1 + sl.offD1, sl.nq, qd[i], st[i], P + ((int64_t)v * D + i) * sum_nq,
+                    (const double *)nullptr, p0);
+    }
+    F[(int64_t)v * sum_nq + g] = T(V[g] * (double)Gv[g] + o / wq[g]);
+  }
+}
+
 // ----------------------------------------------------- grid derivative --
 // DerivOp_TO_RDP_OF_SmolaykRep (BGSG4:2690-2795) on every term grid, ncol
 // columns of the concatenated term grids: i == j >= 0: dnRGG%d2 along mode

@@ -222,9 +222,10 @@ int evr_plan_set_keo_grid(evr_plan *p, const double *G, const double *Jac, const
  * coefficient of d_i d_j (the reference's one term per pair i <= j: put its
  * value in one triangle and zero in the other); F1 is (sum_nq, D) or NULL.
  * Replaces the plan's constant F2/F1.  SG4 bases with D <= 12, a type-1 plan
- * with hermitian = 0, matrix-free only (EVR_ERR_ARG for an assembly);
- * EVR_FP32 as for evr_plan_set_keo_grid; the plan diagonal then comes from H
- * on blocks of unit vectors. */
+ * matrix-free only (EVR_ERR_ARG for an assembly); EVR_FP32 and the
+ * hermitian form (H + H^T) / 2 as for evr_plan_set_keo_grid (the transposed
+ * operator with the tables w F2, w F1 made here); the plan diagonal then
+ * comes from H on blocks of unit vectors. */
 int evr_plan_set_type1_grid(evr_plan *p, const double *F2, const double *F1);
 /* Generate V on the device: V(q) = sum_i v2_i q_i^2 + sum_{i<=j<=k} c_ijk
  * q_i q_j q_k + sum_{i<j} d_ij q_i^2 q_j^2 + sum_i e_i q_i^4, with c given as a

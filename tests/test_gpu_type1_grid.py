@@ -1,6 +1,7 @@
 """Type 1 with the operator grids per grid point (evr_plan_set_type1_grid: the
 reference's get_OpGrid_type1 tables, combined as OPSG4:1490-1546) against the
-oracle, 1e-12 relative max-norm (FP32 2e-5); a constant table equals the constant-F2
+oracle, 1e-12 relative max-norm (FP32 2e-5), and its hermitian form (H + H^T) / 2;
+a constant table equals the constant-F2
 basis-space path; a Henon-Heiles operator written as per-point grids gives the
 reference's 6D Davidson levels."""
 import json
@@ -78,3 +79,29 @@ def test_type1_grid_henon_heiles_levels(evr):
     diff = bench - ev
     print("per-point type 1 HH 6D: levels - reference in [%.2e, %.2e]" % (diff.min(), diff.max()))
     assert diff.min() > -1e-9 and diff.max() < 2e-7
+
+
+def test_type1_grid_hermitian(evr, oracle_mod):
+    """hermitian = 1 with F2, F1 per point: (H + H^T) / 2 of the reference's
+    type-1 grid form (the transpose as W diag(1/w) Op^T diag(w) B), against
+    the oracle's dense operator (FP64 1e-12, FP32 2e-5) and its diagonal."""
+    D, L = 3, 4
+    s = oracle_mod.SG4(D, L)
+    b = evr.Basis.sg4(D, L)
+    q = s.grid()
+    F2, F1 = type1_grids(q, 47)
+    V = 0.5 * (q ** 2).sum(1) + 0.03 * q[:, 0] ** 3
+    n = s.nb_packed
+    M = s.oppsi(oracle_mod.Op(1, V=V, F2q=F2, F1q=F1), np.eye(n))
+    Hs = 0.5 * (M + M.T)
+    plan = evr.Plan(b, type=1, V=V, F2=-0.5 * np.eye(D), max_nvec=6, hermitian=True)
+    plan.set_type1_grid(F2, F1)
+    X = evr.hamiltonian.test_vectors(n, 6, 9)
+    ref = Hs @ X
+    err = relerr(plan.TabOpPsi(X), ref)
+    err32 = relerr(plan.TabOpPsi(X, evr.FP32), ref)
+    errd = relerr(plan.diagonal(), np.diag(Hs))
+    asym = np.abs(M - M.T).max() / np.abs(M).max()
+    print("hermitian type-1 grids: %.3e, FP32 %.3e, diagonal %.3e (asymmetry of H %.2e)" % (err, err32, errd, asym))
+    assert asym > 1e-6
+    assert err < 1e-12 and errd < 1e-12 and err32 < 2e-5
