@@ -354,6 +354,55 @@ def test_dp_matches_dense(oracle_mod):
     assert np.abs(H - Wt @ op @ Bt).max() < 1e-11 * np.abs(H).max()
 
 
+def test_dp_ga_and_type1_match_dense(oracle_mod):
+    """Direct product, D=3 (last mode fastest): the DP type-10 operator with
+    G(q) = G0 + Ga q and the DP type-1 operator with F2 (off-diagonal pairs
+    included) and F1 equal their dense Kronecker forms; these are the DP
+    branches the streamed-term GPU tests compare against."""
+    nb, nq = [3, 4, 3], [5, 5, 4]
+    d = oracle_mod.DP(nb, nq)
+    q = d.grid()
+    D = 3
+    rng = np.random.default_rng(5)
+    G0 = np.diag([1.1, 0.9, 1.3]) + 0.05 * rng.standard_normal((D, D))
+    Ga = 0.02 * rng.standard_normal((D, D, D))
+    V = 0.5 * (q ** 2).sum(1) + 0.04 * q[:, 0] * q[:, 1] * q[:, 2]
+    one = []
+    for k in range(D):
+        x, w = oracle_mod.herm_quadrature(nq[k])
+        B, B1, B2 = oracle_mod.herm_basis(nb[k], x)
+        _, D1, D2 = oracle_mod.herm_dngg(B, B1, B2)
+        one.append((B, np.diag(w) @ B, D1, D2))
+
+    def kron3(a, b, c):
+        return np.kron(a, np.kron(b, c))
+
+    Bt = kron3(one[0][0], one[1][0], one[2][0])
+    Wt = kron3(one[0][1], one[1][1], one[2][1]).T
+    I = [np.eye(n) for n in nq]
+    d1 = [kron3(*(one[m][2] if m == k else I[m] for m in range(D))) for k in range(D)]
+    d2 = [kron3(*(one[m][3] if m == k else I[m] for m in range(D))) for k in range(D)]
+    # type 10: Ga in the Fortran layout Ga(i, j, m) = flat[i + D (j + D m)]
+    Gf = Ga.reshape(-1, order="F")
+    H = d.oppsi(oracle_mod.Op(10, V=V, G0=G0.reshape(-1, order="F"), Ga=Gf), np.eye(d.nb_tot))
+    op = np.diag(V)
+    for i in range(D):
+        for j in range(D):
+            gij = G0[i, j] + q @ Ga[i, j, :]
+            op = op - 0.5 * d1[j] @ np.diag(gij) @ d1[i]
+    assert np.abs(H - Wt @ op @ Bt).max() < 1e-11 * np.abs(H).max()
+    # type 1: F2(i, j) d_i d_j (d2 on the diagonal) + F1(i) d_i
+    F2 = -0.5 * np.diag([1.0, 1.3, 1.7]) + 0.03 * rng.standard_normal((D, D))
+    F1 = np.array([0.02, -0.01, 0.015])
+    H1 = d.oppsi(oracle_mod.Op(1, V=V, F2=F2, F1=F1), np.eye(d.nb_tot))
+    op1 = np.diag(V)
+    for i in range(D):
+        op1 = op1 + F1[i] * d1[i]
+        for j in range(D):
+            op1 = op1 + F2[i, j] * (d2[i] if i == j else d1[i] @ d1[j])
+    assert np.abs(H1 - Wt @ op1 @ Bt).max() < 1e-11 * np.abs(H1).max()
+
+
 def test_grid_derivative_closed_form(oracle_mod):
     """orc_sg4_deriv / orc_dp_deriv on the grid image of a basis vector equal
     the closed-form derivatives of the Hermite functions: d/dq_i and
