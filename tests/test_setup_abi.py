@@ -97,3 +97,22 @@ def test_bessel_coefficients(evr, r):
     j = evr.bessel_j(int(r + 30 * r ** (1 / 3) + 60), r)
     assert abs(j[0] + 2 * j[2::2].sum() - 1.0) < 1e-14
     assert abs(j[0] ** 2 + 2 * (j[1:] ** 2).sum() - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("args", [(2, 8), (2, 1, 1, 40, 1, 41, 1)])
+def test_grid_to_basis_matrix_is_weighted_transpose(evr, args):
+    """W = B^T diag(w) on every 1D slot, with w_q = W[0, q] / B[q, 0] (and
+    B[q, 0] > 0): the identity behind the per-point quadrature weights of the
+    control gradient dJ/dV and the transposed sequence of the hermitian grid
+    forms (W diag(1/w) Op^T diag(w) B).  Also for slots of 41 functions."""
+    b = evr.Basis.sg4(*args)
+    d = b.ptr.contents
+    offB = 0
+    for l in range(d.nlev):
+        nb, nq = d.nb[l], d.nq[l]
+        B = np.ctypeslib.as_array(d.B, shape=(offB + nb * nq,))[offB:].reshape(nb, nq).T
+        W = np.ctypeslib.as_array(d.W, shape=(offB + nb * nq,))[offB:].reshape(nq, nb).T
+        assert (B[:, 0] > 0).all()
+        w = W[0, :] / B[:, 0]
+        assert np.abs(W - B.T * w[None, :]).max() < 1e-13 * np.abs(W).max()
+        offB += nb * nq
