@@ -367,8 +367,9 @@ def pyrazine_run(args):
 
 def keo_grid_run(args):
     """Type 10 with G, J and s tabulated per grid point (the reference's grid
-    form) on D=6 SG4 L=5: H|psi> of 8 vectors per step on the GPU, the same
-    operator through the oracle on the host cores.  G = G0 + Ga q of the
+    form) on D=6 SG4 L=5: H|psi> of 8 vectors per step on the GPU (FP64, and
+    FP32 transforms and grid columns), the same operator through the oracle
+    on the host cores.  G = G0 + Ga q of the
     D=6 L=5 Hamiltonian plus a q^2 term, J and s smooth positive functions."""
     import numpy as np
     import torch
@@ -398,6 +399,17 @@ def keo_grid_run(args):
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
+    # the same with FP32 transforms and grid columns
+    H32 = torch.empty_like(X)
+    for _ in range(max(1, args.warmup)):
+        plan.TabOpPsi_device(X.data_ptr(), H32.data_ptr(), nvec, evrhip.FP32)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.TabOpPsi_device(X.data_ptr(), H32.data_ptr(), nvec, evrhip.FP32)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms32 = e0.elapsed_time(e1) / args.steps
     # algorithmic HBM bytes of the three grid kernels per vector and grid point:
     # deriv reads psi_G and s, writes D columns; combine reads D columns, D^2
     # G values and J, writes D; out reads D columns, J and s, writes 1
@@ -405,6 +417,8 @@ def keo_grid_run(args):
     line = {"workload": "type 10 with G, J, s per grid point, SG4 D=6 L=5 (%d functions, %d grid points), "
                         "%d vectors" % (b.nb_packed, b.sum_nq, nvec),
             "ms_per_hpsi_block": round(ms, 3), "hpsi_per_s_per_vector": round(nvec * 1e3 / ms, 1),
+            "ms_per_hpsi_block_fp32": round(ms32, 3),
+            "fp32_max_rel_diff_vs_fp64": float((H32 - Hx).abs().max() / Hx.abs().max()),
             "grid_kernel_bytes_per_block_gb": round(per_point * b.sum_nq * nvec / 1e9, 3)}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
